@@ -1,0 +1,101 @@
+/* raishin_hip.h -- C-ABI of the MI355X-native RAISHIN GRMHD time step.
+ *
+ * Drop-in boundary for the reference's time-step path (main.f90:743-1149):
+ * plain pointers and sizes, no torch types.  Arrays use the reference's own
+ * column-major layouts with one MPI rank (is1=js1=ks1=1, ie1=imax, ...):
+ *   state  uri/uu(nv=9, imax, jmax, kmax)        main.f90:35-57
+ *   metric gcov/gcon(0:3, 0:3, imax, jmax, kmax) main.f90:62-70
+ *   detg(imax, jmax, kmax), christ(0:3,0:3,0:3, imax, jmax, kmax)
+ *   grids  x1(imax), dx1b(imax-1), ...           main.f90:115-126
+ * Integer arguments of the reference-named routines are passed by address,
+ * as Fortran passes them, so an iso_c_binding interface binds them directly.
+ *
+ * Errors: entry points keep the reference's void signatures; a failure sets a
+ * sticky error read with raishin_last_error().
+ */
+#ifndef RAISHIN_HIP_H
+#define RAISHIN_HIP_H
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Compile-time `module pram` values of the reference (pram.f90:11-80). */
+typedef struct {
+  int imax, jmax, kmax;
+  int irec, irkt, ict, ihll, icha, iwvec, ieos, model, metric, iter;
+  double cfl, gam, akm, dmin, pmin, pmax;
+  int iboux1in[9], iboux1ot[9], iboux2in[9], iboux2ot[9], iboux3in[9], iboux3ot[9];
+} raishin_pram;
+
+/* Event counts; the first five are the events recov1a reports on stdout
+ * (recov.f90:859,921,1124,1130,1154), the floors are silent in the reference. */
+typedef struct {
+  unsigned long long recov_fallback; /* 2D Newton-Raphson failed, entropy inversion used */
+  unsigned long long recov_fail;     /* "Not convergence in recov1-3": previous primitives kept */
+  unsigned long long recov_neg_rho;  /* entropy branch "negative density" */
+  unsigned long long recov_neg_p;    /* entropy branch "negative pressure" */
+  unsigned long long recov_vsq_gt1;  /* "vsq > 1.0 at initial guess" */
+  unsigned long long recov_det0;     /* "det=0 in recov1" */
+  unsigned long long recov_wg_carry; /* initial W inherited from the previous cell */
+  unsigned long long atm_floor;      /* atmosphere1a resets */
+  unsigned long long pmm_scale;      /* pminmax2b 1.05 rescaling entered */
+  unsigned long long pmm_floor;      /* pminmax2b min/max/ratio clamps */
+  unsigned long long guard;          /* iteration guards hit (unbounded loops in the reference) */
+} raishin_counters;
+
+/* Sticky error: returns 0 when no error, else copies the message. */
+int raishin_last_error(char *buf, int len);
+void raishin_clear_error(void);
+
+/* ---- resident model: the production path ------------------------------- */
+typedef struct raishin_model raishin_model;
+
+/* One model on the current device: pram + grid (grid.f90:2) + metric arrays
+ * (coord.f90:5 coord1, coord.f90:1089 calchrist), all host arrays in the
+ * reference layout.  The metric must not depend on j (phi): it is stored as
+ * (i,k) tables; a j-dependent metric is rejected.  Replaces the set-up half of
+ * main.f90:436-448.  Returns NULL on error. */
+raishin_model *raishin_model_create(const raishin_pram *p, const double *x1, const double *x2,
+                                    const double *x3, const double *dx1b, const double *dx2b,
+                                    const double *dx3b, const double *gcov, const double *gcon,
+                                    const double *gcovi, const double *gconi, const double *gcovj,
+                                    const double *gconj, const double *gcovk, const double *gconk,
+                                    const double *detg, const double *detgi, const double *detgj,
+                                    const double *detgk, const double *christ);
+void raishin_model_destroy(raishin_model *m);
+
+/* Primitive (uri) and conserved (uu) state, reference layout (9,imax,jmax,kmax).
+ * set: host -> device (the restart path of restar.f90:273-317 / the end of
+ * init, main.f90:530-539); uu may be NULL, then uu = caluu1(uri) is computed
+ * on the device as main.f90:530 does.  get: device -> host (before
+ * store1/output1); either pointer may be NULL. */
+void raishin_model_set_state(raishin_model *m, const double *uri, const double *uu, double time);
+void raishin_model_get_state(raishin_model *m, double *uri, double *uu, double *time);
+
+/* nsteps full time steps: cdtcfl4 + 2 or 3 RK stages each (main.f90:743-1149),
+ * all on the device, stream-ordered; no host synchronisation inside.
+ * *time and *dt_last are written when non-NULL (that synchronises). */
+void raishin_model_step(raishin_model *m, int nsteps, double *time, double *dt_last);
+
+/* Only the CFL reduction (cdtcfl4 + MIN): dtcfl and dt1..dt3 of the current state. */
+void raishin_model_cdtcfl4(raishin_model *m, double *dtcfl, double *dt1, double *dt2, double *dt3);
+
+void raishin_model_counters(raishin_model *m, raishin_counters *c);
+void raishin_model_reset_counters(raishin_model *m);
+void raishin_model_sync(raishin_model *m);
+/* Device address of the stream the model's kernels run on (hipStream_t). */
+void *raishin_model_stream(raishin_model *m);
+
+/* ---- reference-named entry points (argument lists of the reference) ---- */
+/* cdtcfl4: cdtcfl.f90:2-3.  Host arrays in, host scalars out. */
+void raishin_cdtcfl4(const double *uri, const double *gcov, const double *gcon, double *dtcfl,
+                     const double *dx1, const double *dx2, const double *dx3, double *dt1,
+                     double *dt2, double *dt3, const int *nm1, const int *is1, const int *ie1,
+                     const int *js1, const int *je1, const int *ks1, const int *ke1);
+/* Set the `module pram` values once (pram.f90), before any reference-named call. */
+void raishin_init(const raishin_pram *p);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,52 @@
+/* raishin_setup.h -- host-side set-up of a model (runs once per model).
+ *
+ * These build the inputs of the time step in the reference's layouts: the
+ * grid (grid.f90:2), the metric and sqrt(-g) at centres and faces
+ * (coord.f90:5 coord1, coord.f90:1016 caldetg), the Christoffel symbols
+ * (coord.f90:1089 calchrist) and a magnetised Fishbone-Moncrief torus
+ * (mdgrmhd.f90:879 mdtorus1).  Metrics built: metric=303 (Kerr-Schild) and
+ * metric=1 (Minkowski, coord.f90:149-198).
+ */
+#ifndef RAISHIN_SETUP_H
+#define RAISHIN_SETUP_H
+#include "raishin_hip.h"
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Grid extents (pram.f90:52-61): ix1=1 uniform, 2 logarithmic radius. */
+typedef struct {
+  int ix1;
+  double xmin, xmax, ymin, ymax, zmin, zmax;
+} raishin_grid_ext;
+
+/* Torus parameters (mdgrmhd.f90:938-947 fixes rin=6, rmax=12, beta=100). */
+typedef struct {
+  double rin, rmax;     /* inner edge, pressure maximum */
+  double beta;          /* minimum plasma beta after normalisation */
+  double acut;          /* A_phi = max(rho/rho_max - acut, 0) */
+  double noise;         /* p <- p (1 + noise (U - 0.5)) */
+  unsigned long long seed;
+} raishin_torus;
+
+/* grid.f90:2.  x*, dx*, x*a, x*b have n entries, dx*b has n-1 (n = imax, jmax, kmax). */
+void raishin_grid(const raishin_pram *p, const raishin_grid_ext *ext, double *x1, double *x2, double *x3,
+                  double *dx1, double *dx2, double *dx3, double *dx1b, double *dx2b, double *dx3b,
+                  double *x1a, double *x2a, double *x3a, double *x1b, double *x2b, double *x3b);
+/* coord.f90:5 with caldetg: metrics (4,4,imax,jmax,kmax), sqrt(-g) (imax,jmax,kmax). */
+void raishin_coord1(const raishin_pram *p, double *gcov, double *gcovi, double *gcovj, double *gcovk,
+                    double *gcon, double *gconi, double *gconj, double *gconk, double *detg,
+                    double *detgi, double *detgj, double *detgk, const double *x1, const double *x2,
+                    const double *x3, const double *x1a, const double *x2a, const double *x3a);
+/* coord.f90:1089: christ(4,4,4,imax,jmax,kmax), christ(l,m,n) = Gamma^l_{mn}. */
+void raishin_calchrist(const raishin_pram *p, double *christ, const double *x1, const double *x2,
+                       const double *x3);
+/* Primitive variables uri(9,imax,jmax,kmax) of the torus; returns 0 on success. */
+int raishin_mdtorus1(const raishin_pram *p, const raishin_torus *t, double *uri, const double *gcov,
+                     const double *gcon, const double *detg, const double *x1, const double *x2,
+                     const double *x3);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,111 @@
+/* CPU restatement of the RAISHIN GRMHD time step -- TEST INFRASTRUCTURE ONLY.
+ *
+ * This is the oracle: a literal, whole-array C restatement of the reference
+ * Fortran routines, in the reference's column-major layouts, used by tests/
+ * and by bench.py's cpu_baseline leg to check and to time against.  The
+ * product (raishinsource_amd/) never links or calls it.
+ *
+ * Layouts (1-based Fortran indices, one MPI rank so is1=js1=ks1=1):
+ *   state  a(n,i,j,k)      -> a[(n-1) + 9*cell]
+ *   metric g(m,n,i,j,k)    -> g[m + 4n + 16*cell]      (0:3,0:3)
+ *   christ c(l,m,n,i,j,k)  -> c[l + 4m + 16n + 64*cell]
+ *   flux   ww(d,n,i,j,k)   -> ww[(d-1) + 3*((n-1) + 9*cell)]
+ *   source sf(n,i,j,k)     -> sf[(n-2) + 4*cell]       (2:5)
+ *   cell = (i-1) + imax*((j-1) + jmax*(k-1))
+ */
+#ifndef RAISHIN_ORACLE_H
+#define RAISHIN_ORACLE_H
+#include <stddef.h>
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct {
+  int imax, jmax, kmax;
+  int irec, irkt, ict, ihll, icha, iwvec, ieos, model, metric, iter;
+  double cfl, gam, akm, dmin, pmin, pmax;
+  int iboux1in[9], iboux1ot[9], iboux2in[9], iboux2ot[9], iboux3in[9], iboux3ot[9];
+} orc_pram;
+
+/* Event counters (the reference prints the first three; floors are silent). */
+typedef struct {
+  long long recov_fallback;   /* 2D NR failed -> entropy NR (recov.f90:975-978) */
+  long long recov_fail;       /* both failed: "Not convergence in recov1-3" */
+  long long recov_neg_rho;    /* entropy branch "negative density" */
+  long long recov_neg_p;      /* entropy branch "negative pressure" */
+  long long recov_vsq_gt1;    /* "vsq > 1.0 at initial guess" */
+  long long recov_det0;       /* "det=0 in recov1" */
+  long long recov_wg_carry;   /* initial W taken from the previous cell */
+  long long atm_floor;        /* atmosphere1a resets */
+  long long pmm_scale;        /* pminmax2b 1.05 rescale loops entered */
+  long long pmm_floor;        /* pminmax2b min/max/ratio clamps applied */
+} orc_counters;
+
+void orc_set_pram(const orc_pram *p);
+void orc_get_counters(orc_counters *c);
+void orc_reset_counters(void);
+
+/* hot-path routines (reference argument meaning, 1-based ghost widths) */
+void orc_caluu1(const double *uri, double *uu, const double *gcov, const double *gcon,
+                const double *detg);
+void orc_rec2(const double *uri, const double *x1, const double *x2, const double *x3,
+              const double *dx1b, const double *dx2b, const double *dx3b,
+              double *uriir, double *urijr, double *urikr,
+              double *uriil, double *urijl, double *urikl,
+              double *uuir, double *uujr, double *uukr,
+              double *uuil, double *uujl, double *uukl,
+              double *wwir, double *wwjr, double *wwkr,
+              double *wwil, double *wwjl, double *wwkl,
+              const double *gcovi, const double *gcovj, const double *gcovk,
+              const double *gconi, const double *gconj, const double *gconk,
+              const double *detgi, const double *detgj, const double *detgk, int nm0);
+void orc_calcha(const double *uriir, const double *urijr, const double *urikr,
+                const double *uriil, const double *urijl, const double *urikl,
+                const double *gcovi, const double *gcovj, const double *gcovk,
+                const double *gconi, const double *gconj, const double *gconk,
+                double *cmaxi, double *cmini, double *cmaxj, double *cminj,
+                double *cmaxk, double *cmink, int nm0);
+void orc_hll(double *ww, const double *uuir, const double *uujr, const double *uukr,
+             const double *uuil, const double *uujl, const double *uukl,
+             const double *wwir, const double *wwjr, const double *wwkr,
+             const double *wwil, const double *wwjl, const double *wwkl,
+             const double *cmaxi, const double *cmini, const double *cmaxj,
+             const double *cminj, const double *cmaxk, const double *cmink, int nm0);
+void orc_ct(double *ww, int nm0, int nm1);
+void orc_caltenr(const double *uri, double *tenr, const double *gcov, const double *gcon);
+void orc_calsf(double *sf, const double *tenr, const double *christ, const double *detg);
+void orc_rk2fst(double *uh, const double *ww, const double *uu, int nm1,
+                const double *a1, const double *a2, const double *a3);
+void orc_rk2snd(double *uu, const double *ww, const double *uo, const double *uh, int nm1,
+                const double *a1, const double *a2, const double *a3);
+void orc_rk3snd(double *uu, const double *ww, const double *uo, const double *uh, int nm1,
+                const double *a1, const double *a2, const double *a3);
+void orc_rk3trd(double *uu, const double *ww, const double *uo, const double *us, int nm1,
+                const double *a1, const double *a2, const double *a3);
+void orc_rkadsf(double *uu, const double *sf, double dt, int nm1, int which);
+void orc_recov1a(double *uu, double *uri, const double *gcov, const double *gcon,
+                 const double *detg, const double *x1, int nm1);
+void orc_bnd4(double *uri, const double *uo, int nm1);
+void orc_infchk(double *uri, const double *gcov, const double *gcon, int nm1);
+void orc_atmosphere1a(double *uri, const double *gcov, const double *gcon, const double *x1);
+void orc_pminmax2b(double *uri, const double *gcov, const double *gcon, const double *x1);
+void orc_cdtcfl4(const double *uri, const double *gcov, const double *gcon, double *dtcfl,
+                 const double *dx1, const double *dx2, const double *dx3,
+                 double *dt1, double *dt2, double *dt3, int nm1);
+
+/* One full time step as main.f90:743-1149 sequences it.  Arrays: uu, uri
+ * (state, updated in place), metric sets (centre, i-face, j-face, k-face),
+ * christ, grids.  Returns dt; *time is advanced. */
+typedef struct {
+  double *uu, *uri;
+  const double *gcov, *gcon, *gcovi, *gconi, *gcovj, *gconj, *gcovk, *gconk;
+  const double *detg, *detgi, *detgj, *detgk, *christ;
+  const double *x1, *x2, *x3, *dx1b, *dx2b, *dx3b;
+} orc_state;
+double orc_step(orc_state *s, double *time);
+void orc_free_work(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

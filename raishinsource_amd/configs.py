@@ -1,0 +1,102 @@
+"""Run configurations expressed as the reference's compile-time ``module pram``.
+
+Names the parameter sets that the HIP library, the benchmark and the tests
+run with.  Every field mirrors one ``pram.f90`` parameter (reference
+``pram.f90:11-80``); the derived ghost widths follow ``main.f90:179-211``.
+"""
+from dataclasses import dataclass, field, replace
+import math
+
+PI = 3.14159265358979323  # pram.f90:31
+
+
+def _xmin_ks(akm):
+    # pram.f90:52  xmin=0.98d0*(1.+sqrt(1.-akm*akm))
+    return 0.98 * (1.0 + math.sqrt(1.0 - akm * akm))
+
+
+@dataclass(frozen=True)
+class Pram:
+    name: str
+    imax: int
+    jmax: int
+    kmax: int
+    irec: int = 5          # 5 PPM, 6 MP5, 7 WENO5
+    irkt: int = 2
+    ict: int = 1
+    ihll: int = 1
+    icha: int = 2
+    iwvec: int = 1
+    ieos: int = 0
+    model: int = 7
+    metric: int = 303
+    cfl: float = 0.4
+    gam: float = 4.0 / 3.0
+    akm: float = 0.9375
+    dmin: float = 1.0e-4
+    pmin: float = 1.0e-6
+    pmax: float = 1.0e6
+    iter: int = 5000
+    ix1: int = 2
+    ix2: int = 1
+    ix3: int = 1
+    xmin: float = None
+    xmax: float = 50.0
+    ymin: float = 0.0
+    ymax: float = None      # default: periodic phi, dx2 = 2 pi / n_interior
+    zmin: float = 0.0
+    zmax: float = PI
+    iboux1in: tuple = (5, 5, 5, 5, 5, 5, 5, 5, 5)
+    iboux1ot: tuple = (5, 18, 18, 18, 5, 5, 5, 5, 5)
+    iboux2in: tuple = (1,) * 9
+    iboux2ot: tuple = (1,) * 9
+    iboux3in: tuple = (14, 14, 14, 12, 14, 14, 14, 14, 12)
+    iboux3ot: tuple = (14, 14, 14, 12, 14, 14, 14, 14, 12)
+
+    # ghost widths: main.f90:179-211
+    @property
+    def nm0(self):
+        if self.irec in (1, 2, 3, 11, 12, 13):
+            return 2
+        if self.irec in (4, 6, 7, 8, 9, 10):
+            return 3
+        if self.irec == 5:
+            return 4
+        return 1
+
+    @property
+    def nm1(self):
+        return self.nm0 + {0: 0, 1: 1, 2: 2, 3: 3}[self.ict]
+
+    def resolved(self):
+        """Fill the grid extents that default from other parameters."""
+        p = self
+        if p.xmin is None:
+            p = replace(p, xmin=_xmin_ks(p.akm))
+        if p.ymax is None:
+            nint = p.jmax - 2 * p.nm1
+            dx2 = 2.0 * PI / max(nint, 1)
+            p = replace(p, ymax=p.ymin + dx2 * (p.jmax - 1))
+        return p
+
+    @property
+    def interior(self):
+        n = self.nm1
+        return ((self.imax - 2 * n), (self.jmax - 2 * n), (self.kmax - 2 * n))
+
+
+# 2D axisymmetric FM torus, PPM + RK2 (one phi cell plus ghosts)
+C1 = Pram("c1", 128, 11, 128, irec=5, irkt=2, cfl=0.4)
+# one GPU's 3D subdomain, MP5 + RK3
+C2 = Pram("c2", 256, 128, 128, irec=6, irkt=3, cfl=0.5)
+C2_PPM = replace(C2, name="c2ppm", irec=5, jmax=128)
+C2_WENO = replace(C2, name="c2weno", irec=7)
+
+# small grids used for unit-level golden vectors and short trajectories
+U_PPM = Pram("uppm", 24, 12, 22, irec=5, irkt=2, cfl=0.4)
+U_MP5 = Pram("ump5", 20, 12, 18, irec=6, irkt=3, cfl=0.5)
+U_WENO = Pram("uweno", 20, 12, 18, irec=7, irkt=3, cfl=0.5)
+T2D = Pram("t2d", 64, 11, 64, irec=5, irkt=2, cfl=0.4)
+T3D = Pram("t3d", 32, 16, 32, irec=6, irkt=3, cfl=0.5)
+
+ALL = {p.name: p for p in (C1, C2, C2_PPM, C2_WENO, U_PPM, U_MP5, U_WENO, T2D, T3D)}

@@ -1,0 +1,977 @@
+// HIP kernels of the RAISHIN GRMHD time step for gfx950 (FP64 path).
+//
+// Device layout (structure of arrays, i fastest):
+//   state planes   A[n*N + c],  c = (i-1) + ni*((j-1) + nj*(k-1)),  n = 0..8
+//   fluxes         F[d][n*N + c] holds ww(d,n,i,j,k) = flux at i+1/2 (j+1/2, k+1/2)
+//   metric tables  (i,k) planes: the Kerr-Schild metric has no phi dependence,
+//                  so each location set is 33 planes of ni*nk values and the
+//                  Christoffel symbols 64 planes.
+// Each kernel cites the reference routines it restates.
+#include "rh_device.hpp"
+#include "rh_kernels.hpp"
+
+namespace rh {
+
+RH_DEV long long cidx(int i, int j, int k, const Geo& G) {
+  return (long long)(i - 1) + (long long)G.ni * ((long long)(j - 1) + (long long)G.nj * (long long)(k - 1));
+}
+RH_DEV long long q2(int i, int k, const Geo& G) { return (long long)(i - 1) + (long long)G.ni * (long long)(k - 1); }
+
+// 1-based (i,j,k) of linear thread t over a box [lo_a, hi_a] per axis; false if outside.
+RH_DEV bool box_index(long long t, int ilo, int ihi, int jlo, int jhi, int klo, int khi, int& i, int& j, int& k) {
+  const long long ci = ihi - ilo + 1, cj = jhi - jlo + 1, ck = khi - klo + 1;
+  if (ci <= 0 || cj <= 0 || ck <= 0 || t >= ci * cj * ck) return false;
+  i = ilo + (int)(t % ci);
+  const long long r = t / ci;
+  j = jlo + (int)(r % cj);
+  k = klo + (int)(r / cj);
+  return true;
+}
+
+// Counter slots (raishin_counters field order).
+enum { C_FALLBACK = 0, C_FAIL, C_NEGRHO, C_NEGP, C_VSQGT1, C_DET0, C_WGCARRY, C_ATM, C_PMMSCALE, C_PMMFLOOR,
+       C_GUARD, C_NCOUNT };
+
+RH_DEV void count_wave(unsigned long long* cnt, int slot, bool flag) {
+  // one atomic per wavefront: ballot the event, lane 0 adds the popcount
+  const unsigned long long b = __ballot(flag);
+  if (b && (threadIdx.x % warpSize) == (unsigned)(__ffsll((long long)b) - 1))
+    atomicAdd(&cnt[slot], (unsigned long long)__popcll(b));
+}
+
+// ------------------------------------------------------------------------
+// PPM reconstruction of one cell (rec.f90:1261-2056), direction D.
+// Returns the cell's left edge L (uri*r(c-1)) and right edge R (uri*l(c)).
+// p is the 1-based index of the cell along D; the loads are along stride s.
+// dxs[d] holds the dummy dx*b(d) of rec2 (= actual dx*b(d-1), rec.f90:58).
+// ------------------------------------------------------------------------
+template <int D>
+RH_DEV void ppm_cell(const double* __restrict__ W, long long N, long long c, long long s, int p,
+                     const double* __restrict__ dxs, const double* __restrict__ xs, double gam,
+                     double L[9], double R[9]) {
+  const double dm2 = dxs[p - 2], dm1 = dxs[p - 1], d0 = dxs[p], dp1 = dxs[p + 1], dp2 = dxs[p + 2];
+  // interp coefficients for dma at q = p-1, p, p+1 (rec.f90:1353-1355)
+  auto dma_of = [&](double um, double u0, double up, double dl, double dc, double dr) -> double {
+    const double t1a = dc / (dl + dc + dr);
+    const double t1b = (2.0 * dl + dc) / (dr + dc);
+    const double t1c = (dc + 2.0 * dr) / (dl + dc);
+    const double sp1 = (up - u0);
+    const double sm1 = (u0 - um);
+    const double spm1 = sp1 * sm1;
+    const double da = t1a * (t1b * sp1 + t1c * sm1);
+    if (spm1 <= 0.0) return 0.0;
+    const double t1 = (da < 0.0) ? -1.0 : 1.0;
+    const double t1d = fminf_(fabs(sm1), fabs(sp1));
+    if (D == 3) return t1 * fminf_(fabs(da), t1d);   // rec.f90:1414
+    return t1 * fminf_(fabs(da), 2.0 * t1d);
+  };
+  // equation (51) at index q: value at q+1/2 (rec.f90:1427-1436)
+  auto eq51 = [&](double uq, double uq1, double dmq, double dmq1, double dA, double dB, double dC, double dD) -> double {
+    // dA..dD = dx(q-1), dx(q), dx(q+1), dx(q+2)
+    const double t2 = (dB / (dB + dC)) * (uq1 - uq);
+    const double t3a = 1.0 / (dA + dB + dC + dD);
+    const double t3b = (2.0 * dC * dB) / (dB + dC);
+    const double t3c = ((dA + dB) / (2.0 * dB + dC)) - ((dD + dC) / (2.0 * dC + dB));
+    const double t3 = t3a * t3b * t3c * (uq1 - uq);
+    const double t4 = t3a * dB * ((dA + dB) / (2.0 * dB + dC));
+    const double t5 = t3a * dC * ((dC + dD) / (dB + 2.0 * dC));
+    return uq + t2 + t3 - t4 * dmq1 + t5 * dmq;
+  };
+
+  // --- flattening coefficient f2 of this cell (rec.f90:1736-1858) ---
+  const int VEL = D;  // velocity component along D is variable D+1 (0-based D)
+  const double* P5 = W + 4 * N + c;
+  const double* PV = W + (long long)VEL * N + c;
+  double f1[3];
+#pragma unroll
+  for (int q = -1; q <= 1; q++) {
+    const double pp1 = P5[(q + 1) * s], pm1 = P5[(q - 1) * s], pp2 = P5[(q + 2) * s], pm2 = P5[(q - 2) * s];
+    const double t3a = fabs(pp1 - pm1);
+    const double t1 = 1.0 * fminf_(pp1, pm1) - t3a;
+    const double t2 = PV[(q + 1) * s] - PV[(q - 1) * s];
+    const double omg1 = (t1 < 0.0 && t2 < 0.0) ? 1.0 : 0.0;
+    const double t3b = (pp2 - pm2);
+    double t4;
+    if (t3b == 0.0) t4 = (t3a == 0.0) ? -0.52 : 1.0 - 0.52;
+    else t4 = (t3a / t3b) - 0.52;
+    f1[q + 1] = fminf_(1.0, omg1 * fmaxf_(0.0, t4 * 10.0));
+  }
+  const double f1b = (P5[s] - P5[-s] < 0.0) ? f1[2] : f1[0];
+  const double f2 = fmaxf_(f1[1], f1b);
+
+  // --- contact steepening parameter eta (density only; rec.f90:1537-1629) ---
+  double eta;
+  {
+    const double* P1 = W + c;
+    const double um2 = P1[-2 * s], um1 = P1[-s], u0 = P1[0], up1 = P1[s], up2 = P1[2 * s];
+    const double ddm = (1.0 / (dm2 + dm1 + d0)) * ((u0 - um1) / (d0 + dm1) - (um1 - um2) / (dm1 + dm2));
+    const double ddp = (1.0 / (d0 + dp1 + dp2)) * ((up2 - up1) / (dp2 + dp1) - (up1 - u0) / (dp1 + d0));
+    const double t1 = ddp * ddm;
+    const double t2 = fabs(up1 - um1) - 0.1 * fminf_(fabs(up1), fabs(um1));
+    double etatil;
+    if (t1 >= 0.0 && t2 <= 0.0) etatil = 0.0;
+    else {
+      const double xm = xs[p - 2], x0 = xs[p - 1], xp = xs[p];
+      const double t2a = -(ddp - ddm) / (xp - xm);
+      const double dl = x0 - xm, dr = xp - x0;
+      const double t2b = ((dl * dl * dl) + (dr * dr * dr)) / (up1 - um1);
+      etatil = t2a * t2b;
+    }
+    eta = fmaxf_(0.0, fminf_(5.0 * (etatil - 0.05), 1.0));
+    const double t3 = fabs(up1 - um1) / fminf_(up1, um1);
+    const double t4 = fabs(P5[s] - P5[-s]) / fminf_(P5[s], P5[-s]);
+    if (gam * 1.0 * t3 < t4) eta = 0.0;
+  }
+
+#pragma unroll
+  for (int n = 0; n < 9; n++) {
+    const double* Pn = W + (long long)n * N + c;
+    const double um2 = Pn[-2 * s], um1 = Pn[-s], u0 = Pn[0], up1 = Pn[s], up2 = Pn[2 * s];
+    const double dmm1 = dma_of(um2, um1, u0, dxs[p - 2], dm1, d0);
+    const double dm0 = dma_of(um1, u0, up1, dm1, d0, dp1);
+    const double dmp1 = dma_of(u0, up1, up2, d0, dp1, dp2);
+    double r = eq51(u0, up1, dm0, dmp1, dm1, d0, dp1, dp2);     // value at p+1/2
+    double l = eq51(um1, u0, dmm1, dm0, dm2, dm1, d0, dp1);     // value at p-1/2
+    if (n == 0) {
+      r = r * (1.0 - eta) + (up1 - 0.5 * dmp1) * eta;
+      l = l * (1.0 - eta) + (um1 + 0.5 * dmm1) * eta;
+    }
+    r = u0 * f2 + r * (1.0 - f2);
+    l = u0 * f2 + l * (1.0 - f2);
+    if ((r - u0) * (l - u0) >= 0.0) { r = u0; l = u0; }            // equation (73)
+    const double t3 = 6.0 * (r - l) * (u0 - 0.5 * (r + l));         // equations (74), (75)
+    const double t4 = (r - l) * (r - l);
+    if (t3 > t4) l = 3.0 * u0 - 2.0 * r;
+    if (-t3 > t4) r = 3.0 * u0 - 2.0 * l;
+    L[n] = l; R[n] = r;
+  }
+}
+
+// ------------------------------------------------------------------------
+// Direction sweep: reconstruction + face fluxes + wave speeds + HLLE.
+// rec2 (rec.f90:2-142) -> caluu3/calflx3* -> calcha (calcha.f90:2-126) -> hlle
+// (hll.f90:47-192), fused per interface.  One thread per interface c+1/2.
+// ------------------------------------------------------------------------
+template <int D, int IREC>
+__global__ void __launch_bounds__(256) k_sweep(Geo G, const double* __restrict__ W, const double* __restrict__ metF,
+                                               const double* __restrict__ dxs, const double* __restrict__ xs,
+                                               double gam, double* __restrict__ F) {
+  int i, j, k;
+  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (!box_index(t, G.nm0, G.ni - G.nm0, G.nm0, G.nj - G.nm0, G.nm0, G.nk - G.nm0, i, j, k)) return;
+  const long long N = G.N;
+  const long long s = (D == 1) ? 1LL : (D == 2) ? (long long)G.ni : (long long)G.ni * G.nj;
+  const long long c = cidx(i, j, k, G);
+  const int p = (D == 1) ? i : (D == 2) ? j : k;
+  double wl[9], wr[9];
+  if (IREC == 5) {
+    double Lc[9], Rc[9], Ln[9], Rn[9];
+    ppm_cell<D>(W, N, c, s, p, dxs, xs, gam, Lc, Rc);
+    ppm_cell<D>(W, N, c + s, s, p + 1, dxs, xs, gam, Ln, Rn);
+#pragma unroll
+    for (int n = 0; n < 9; n++) { wl[n] = Rc[n]; wr[n] = Ln[n]; }
+  } else {
+#pragma unroll
+    for (int n = 0; n < 9; n++) {
+      const double* Pn = W + (long long)n * N + c;
+      const double a = Pn[-2 * s], b = Pn[-s], cc = Pn[0], d = Pn[s], e = Pn[2 * s], f = Pn[3 * s];
+      if (IREC == 6) {
+        const double sl[5] = {a, b, cc, d, e};
+        const double sr[5] = {b, cc, d, e, f};
+        wl[n] = mp5_right(sl);
+        wr[n] = mp5_left(sr);
+      } else {
+        wl[n] = weno_edge(a, b, cc, d, e);
+        wr[n] = weno_edge(f, e, d, cc, b);
+      }
+    }
+  }
+  const Met M{metF, G.P2, q2(i, k, G)};
+  const double detg = M.detg();
+  double Ul[9], Fl[9], vpl, vml;
+  {
+    Prel sl;
+    prelude(wl, gam, M, sl);
+    const double pw = pow(sl.de, gam);
+    tmunu_row(sl, 0, detg, pw, Ul);
+    tmunu_row(sl, D, detg, pw, Fl);
+    face_speed(sl, D, gam, M, vpl, vml);
+  }
+  double Ur[9], Fr[9], vpr, vmr;
+  {
+    Prel sr;
+    prelude(wr, gam, M, sr);
+    const double pw = pow(sr.de, gam);
+    tmunu_row(sr, 0, detg, pw, Ur);
+    tmunu_row(sr, D, detg, pw, Fr);
+    face_speed(sr, D, gam, M, vpr, vmr);
+  }
+  const double cmax = fmaxf_(fmaxf_(0.0, vpr), vpl);
+  const double cmin = (D == 3) ? fminf_(fminf_(0.0, vpr), vml)   // calcha.f90:107
+                               : fminf_(fminf_(0.0, vmr), vml);
+  double* Fo = F + c;
+#pragma unroll
+  for (int n = 0; n < 9; n++) {
+    double o;
+    if (cmin > 0.0) o = Fl[n];
+    else if (cmin <= 0.0 && cmax > 0.0) {
+      const double c1 = fmaxf_(fabs(cmax), fabs(cmin));
+      o = 0.5 * (Fl[n] + Fr[n] - c1 * (Ur[n] - Ul[n]));
+    } else if (cmax <= 0.0) o = Fr[n];
+    else continue;  // unordered speeds: hlle leaves ww as it was
+    if (fabs(o) < 1.0e-40) o = 0.0;
+    Fo[(long long)n * N] = o;
+  }
+}
+
+// ------------------------------------------------------------------------
+// Flux-CT (Toth 2000): fluxct, ct.f90:30-118.  Writes the six averaged
+// transverse B fluxes to CT planes: 0 ww(2,7) 1 ww(3,7) 2 ww(1,8) 3 ww(3,8)
+// 4 ww(1,9) 5 ww(2,9).
+// ------------------------------------------------------------------------
+__global__ void __launch_bounds__(256) k_fluxct(Geo G, const double* __restrict__ F1, const double* __restrict__ F2,
+                                                const double* __restrict__ F3, double* __restrict__ CT) {
+  int i, j, k;
+  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (!box_index(t, G.nm1, G.ni - G.nm1, G.nm1, G.nj - G.nm1, G.nm1, G.nk - G.nm1, i, j, k)) return;
+  const long long N = G.N, c = cidx(i, j, k, G);
+  const long long si = 1, sj = G.ni, sk = (long long)G.ni * G.nj;
+  const double* fyx = F2 + 6 * N;  // ww(2,7)
+  const double* fzx = F3 + 6 * N;  // ww(3,7)
+  const double* fxy = F1 + 7 * N;  // ww(1,8)
+  const double* fzy = F3 + 7 * N;  // ww(3,8)
+  const double* fxz = F1 + 8 * N;  // ww(1,9)
+  const double* fyz = F2 + 8 * N;  // ww(2,9)
+  const double c8 = 1.0 / 8.0;
+  CT[0 * N + c] = c8 * (2.0 * fyx[c] + fyx[c + si] + fyx[c - si] - fxy[c] - fxy[c + sj] - fxy[c - si] - fxy[c - si + sj]);
+  CT[1 * N + c] = c8 * (2.0 * fzx[c] + fzx[c + si] + fzx[c - si] - fxz[c] - fxz[c + sk] - fxz[c - si] - fxz[c - si + sk]);
+  CT[2 * N + c] = c8 * (2.0 * fxy[c] + fxy[c + sj] + fxy[c - sj] - fyx[c] - fyx[c + si] - fyx[c - sj] - fyx[c + si - sj]);
+  CT[3 * N + c] = c8 * (2.0 * fzy[c] + fzy[c + sj] + fzy[c - sj] - fyz[c] - fyz[c + sk] - fyz[c - sj] - fyz[c - sj + sk]);
+  CT[4 * N + c] = c8 * (2.0 * fxz[c] + fxz[c + sk] + fxz[c - sk] - fzx[c] - fzx[c + si] - fzx[c - sk] - fzx[c + si - sk]);
+  CT[5 * N + c] = c8 * (2.0 * fyz[c] + fyz[c + sk] + fyz[c - sk] - fzy[c] - fzy[c + sj] - fzy[c - sk] - fzy[c + sj - sk]);
+}
+
+// ------------------------------------------------------------------------
+// Recovery of primitives, recov1a (recov.f90:578-1228), one interior cell.
+// u[9] = stage conserved (after RK + sources); wold = previous primitives
+// (initial guess and fallback).  Writes w[9].  The initial W of a cell whose
+// quadratic has no non-negative root is inherited from the previous cell in
+// the reference's loop order; that cell is flagged in *carry.
+// ------------------------------------------------------------------------
+struct RecovOut { double w[9]; int fallback, fail, negrho, negp, vsqgt1, det0, guard; };
+
+RH_DEV void recov_cell(const double u[9], const double wold[9], const Met& M, double gam, double dmin,
+                       double pmin, int iter, double x1pow15, double x1pow25, double wg_in, bool& carry,
+                       RecovOut& o, double& wg_out) {
+  const double safty = 1.0e-4, saftybig = 1.0e5, dv = 1.0e-2;
+  const double gam1 = gam / (gam - 1.0);
+  const double gam2 = 1.0 / gam1;
+  o.fallback = o.fail = o.negrho = o.negp = o.vsqgt1 = o.det0 = o.guard = 0;
+  double delta = 1.0, deltaend = 1.0e-7;
+  int irecov = 0;
+  const double alpha1 = 1.0 / sqrt(-M.gi(0, 0));
+  const double detg1 = M.detg();
+  const double dro = alpha1 * u[0] * (1.0 / detg1);
+  double qcov[4], qcon[4], bcon[4], bcov[4], oncov[4], oncon[4];
+  qcov[0] = alpha1 * (u[4] - u[0]) * (1.0 / detg1);
+  qcov[1] = alpha1 * u[1] * (1.0 / detg1);
+  qcov[2] = alpha1 * u[2] * (1.0 / detg1);
+  qcov[3] = alpha1 * u[3] * (1.0 / detg1);
+  const double gss = alpha1 * u[5] * (1.0 / detg1);
+  bcon[0] = 0.0;
+  bcon[1] = alpha1 * u[6] * (1.0 / detg1);
+  bcon[2] = alpha1 * u[7] * (1.0 / detg1);
+  bcon[3] = alpha1 * u[8] * (1.0 / detg1);
+  double util[4] = {0.0, wold[1], wold[2], wold[3]};
+  const double ro1 = wold[0], pp1 = wold[4];
+  lower(bcon, bcov, M);
+  upper(qcov, qcon, M);
+  const double bsq = bcon[1] * bcov[1] + bcon[2] * bcov[2] + bcon[3] * bcov[3];
+  const double qdotb = qcov[0] * bcon[0] + qcov[1] * bcon[1] + qcov[2] * bcon[2] + qcov[3] * bcon[3];
+  const double qdotbsq = qdotb * qdotb;
+  oncov[0] = -1.0 * alpha1; oncov[1] = 0.0; oncov[2] = 0.0; oncov[3] = 0.0;
+  upper(oncov, oncon, M);
+  const double qdotn = qcon[0] * oncov[0];
+  const double qsq = qcov[0] * qcon[0] + qcov[1] * qcon[1] + qcov[2] * qcon[2] + qcov[3] * qcon[3];
+  const double qtsq = qsq + (qdotn * qdotn);
+  double utsq = 0.0;
+#pragma unroll
+  for (int m = 1; m <= 3; m++)
+#pragma unroll
+    for (int n = 1; n <= 3; n++) utsq = utsq + M.g(m, n) * util[m] * util[n];
+  if (utsq < 0.0 && fabs(utsq) <= safty) utsq = fabs(utsq);
+  if (fabs(utsq) > 1.0e4) utsq = 1.0e4;
+  double gfsq = 1.0 + utsq;
+  double gf = sqrt(gfsq);
+  double vsq = (gfsq - 1.0) * (1.0 / gfsq);
+  const double t1a = 4.0 - vsq;
+  const double t1b = 4.0 * (bsq + qdotn);
+  const double t1c = qtsq + bsq * bsq + 2.0 * bsq * qdotn;
+  double t1d = (t1b * t1b) - 4.0 * t1a * t1c;
+  if (t1d <= 0.0) t1d = fabs(t1d);
+  const double wgp = (-t1b + sqrt(t1d)) * (1.0 / (2.0 * t1a));
+  const double wgm = (-t1b - sqrt(t1d)) * (1.0 / (2.0 * t1a));
+  double wg = wg_in;
+  carry = false;
+  if (wgp > 0.0) wg = wgp;
+  else if (wgm >= 0.0) wg = wgm;
+  else carry = true;
+  double pp = 0.0, ro = 0.0;
+  // recov.f90:823-850 ("goto 95"); bounded here, bound hits are counted
+  int guard = 0;
+  for (; guard < 100000; guard++) {
+    pp = gam2 * (wg * gf - dro * sqrt(gf));
+    const double t3a = (wg * wg * wg) * (wg + 2.0 * bsq) - qdotbsq * (2.0 * wg + bsq);
+    const double t3b = (wg * wg) * (qtsq - bsq * bsq);
+    const double t3c = (wg * wg * wg) + (wg * wg) * (0.5 * bsq + qdotn - pp) - 0.5 * qdotbsq;
+    if (t3a <= t3b) { wg = fabs(1.5 * wg); continue; }
+    if (t3c > 0.0) { wg = fabs((double)0.9f * wg); continue; }
+    break;
+  }
+  if (guard >= 100000) o.guard = 1;
+  double wsq = wg * wg;
+  const double xsq = (bsq + wg) * (bsq + wg);
+  vsq = fabs((wsq * qtsq + qdotbsq * (bsq + 2.0 * wg)) * (1.0 / (wsq * xsq)));
+  if (vsq > 1.0) { o.vsqgt1 = 1; vsq = 1.0 - dv; }
+  const double vsqo = vsq;
+  double xxn = wg, yyn = vsq, w3, gtmp;
+  // 2D Newton-Raphson on (W, v^2): recov.f90:877-960.  Trips after convergence
+  // change nothing in the reference, so the loop stops there.
+  for (int nnn = 1; nnn <= iter; nnn++) {
+    if (!(fabs(delta) >= deltaend && irecov == 0)) break;
+    if (nnn == iter) { irecov = 1; break; }
+    wg = xxn; vsq = yyn;
+    wsq = wg * wg; w3 = wsq * wg;
+    gtmp = 1.0 - vsq;
+    if (gtmp < 0.0 || gtmp < safty) { gtmp = safty; vsq = 1.0 - gtmp; }
+    const double ptmp = gam2 * (wg * gtmp - dro * sqrt(gtmp));
+    const double f1 = qtsq - vsq * (wg + bsq) * (wg + bsq) + (qdotbsq * (bsq + 2.0 * wg)) * (1.0 / wsq);
+    const double g1 = -qdotn - 0.5 * bsq * (1.0 + vsq) + 0.5 * (qdotbsq * (1.0 / wsq)) - wg + ptmp;
+    const double dpdw = gam2 * gtmp;
+    const double dpdvsq = -gam2 * wg + 0.5 * gam2 * dro * (1.0 / (sqrt(gtmp)));
+    const double dfx = -2.0 * (wg + bsq) * (vsq + qdotbsq * (1.0 / w3));
+    const double dfy = -((wg + bsq) * (wg + bsq));
+    const double dgx = -1.0 - (qdotbsq * (1.0 / w3)) + dpdw;
+    const double dgy = -0.5 * bsq + dpdvsq;
+    double det = dfx * dgy - dfy * dgx;
+    if (det == 0.0) { o.det0 = 1; det = 1.0; }
+    const double dx = (-dgy * f1 + dfy * g1) * (1.0 / det);
+    const double dy = (dgx * f1 - dfx * g1) * (1.0 / det);
+    delta = 0.5 * (fabs(dx) + fabs(dy));
+    xxn = wg + dx;
+    yyn = vsq + dy;
+    if (xxn < 0.0) xxn = fabs(xxn);
+    if (xxn > saftybig) xxn = saftybig;
+    if (yyn < 0.0) yyn = fabs(yyn);
+    if (fabs(yyn) > 1.0) yyn = 1.0 - dv;
+  }
+  if (irecov == 0) {
+    wg = xxn; vsq = yyn;
+    gtmp = 1.0 - vsq;
+    if (gtmp < 0.0 || gtmp < safty) { gtmp = safty; vsq = 1.0 - gtmp; }
+    gf = 1.0 / sqrt(gtmp);
+    ro = dro * (1.0 / gf);
+    const double w2 = wg * (gtmp);
+    pp = gam2 * (w2 - ro);
+    if (ro < 0.0 || pp < 0.0) irecov = 1;
+    if (irecov == 0) {
+#pragma unroll
+      for (int m = 1; m <= 3; m++) {
+        const double qt = qcon[m] + oncon[m] * qdotn;
+        util[m] = (gf * (1.0 / (wg + bsq))) * (qt + qdotb * bcon[m] * (1.0 / wg));
+      }
+    }
+    wg = xxn;
+  }
+  if (irecov == 1) {
+    // entropy-based 1D inversion on v^2: recov.f90:975-1147
+    o.fallback = 1;
+    delta = 1.0; deltaend = 1.0e-6; irecov = 0;
+    xxn = vsqo;
+    for (int nnn = 1; nnn <= iter; nnn++) {
+      if (!(fabs(delta) >= deltaend && irecov == 0)) break;
+      if (nnn == iter) { irecov = 1; break; }
+      vsq = xxn;
+      gtmp = 1.0 - vsq;
+      if (gtmp < 0.0 || gtmp < safty) { gtmp = safty; vsq = 1.0 - gtmp; }
+      gf = 1.0 / sqrt(gtmp);
+      gfsq = gf * gf;
+      ro = dro * (1.0 / gf);
+      pp = (gss / dro) * (pow(ro, gam));
+      const double roh = ro + gam1 * pp;
+      wg = fabs(gfsq * roh);
+      wsq = wg * wg; w3 = wsq * wg;
+      const double dgfdvsq = 0.5 * pow(gtmp, -1.5);
+      const double dwdvsq = dro * dgfdvsq + gam1 * gss * (pow(dro, gam - 1.0)) * (2.0 - gam) * (pow(gf, 1.0 - gam)) * dgfdvsq;
+      const double f = qtsq - vsq * (wg + bsq) * (wg + bsq) + (qdotbsq * (bsq + 2.0 * wg)) / wsq;
+      const double a = -((wg + bsq) * (wg + bsq));
+      const double b = -2.0 * vsq * (wg + bsq);
+      const double cc = -2.0 * qdotbsq * (bsq + 2.0 * wg) * (1.0 / w3);
+      const double d = 2.0 * qdotbsq * (1.0 / wsq);
+      const double df = a + (b + cc + d) * dwdvsq;
+      delta = f / df;
+      xxn = vsq - delta;
+      if (xxn < 0.0) xxn = fabs(xxn);
+      if (fabs(xxn) >= 1.0) xxn = 1.0 - dv;
+    }
+    if (irecov == 0) {
+      vsq = xxn;
+      gtmp = 1.0 - vsq;
+      if (gtmp < 0.0 || gtmp < safty) gtmp = safty;
+      gf = 1.0 / sqrt(gtmp);
+      gfsq = gf * gf;
+      ro = dro * (1.0 / gf);
+      pp = (gss * (1.0 / gf)) * (pow(ro, gam - 1.0));
+      const double roh = ro + gam1 * pp;
+      if (ro < 0.0) { o.negrho = 1; ro = dmin * x1pow15; }
+      if (pp < 0.0) { o.negp = 1; pp = (gam - 1.0) * pmin * x1pow25; }
+      wg = fabs(gfsq * roh);
+#pragma unroll
+      for (int m = 1; m <= 3; m++) {
+        const double qt = qcon[m] + oncon[m] * qdotn;
+        util[m] = (gf / (wg + bsq)) * (qt + qdotb * bcon[m] * (1.0 / wg));
+      }
+    }
+  }
+  if (irecov == 0) {
+    o.w[0] = ro; o.w[1] = util[1]; o.w[2] = util[2]; o.w[3] = util[3]; o.w[4] = pp;
+    o.w[5] = ro * pp / pow(ro, gam - 1.0);
+  } else {
+    o.fail = 1;
+    o.w[0] = ro1; o.w[1] = wold[1]; o.w[2] = wold[2]; o.w[3] = wold[3]; o.w[4] = pp1;
+    o.w[5] = ro1 * pp1 / pow(ro1, gam);
+  }
+  o.w[6] = u[6] * (1.0 / detg1);
+  o.w[7] = u[7] * (1.0 / detg1);
+  o.w[8] = u[8] * (1.0 / detg1);
+  wg_out = wg;
+}
+
+// ------------------------------------------------------------------------
+// Update of one interior cell: flux divergence (with CT'd B fluxes), the
+// geometric source (caltenr + calsf, calsf.f90:6-151), the TVD-RK stage
+// (rkt.f90:2-180) plus its source add (rkt.f90:183-320), then recov1a.
+// STAGE: 1 rk2fst, 2 rk2snd, 3 rk3snd, 4 rk3trd.
+// ------------------------------------------------------------------------
+template <int STAGE>
+__global__ void __launch_bounds__(256) k_update(Geo G, StageArgs A) {
+  int i, j, k;
+  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (!box_index(t, G.nm1 + 1, G.ni - G.nm1, G.nm1 + 1, G.nj - G.nm1, G.nm1 + 1, G.nk - G.nm1, i, j, k)) return;
+  const long long N = G.N, c = cidx(i, j, k, G);
+  const long long sj = G.ni, sk = (long long)G.ni * G.nj;
+  const double gam = A.gam;
+  // dt and dt/dx (main.f90:760-780), recomputed per thread from the CFL minimum
+  const double dtcfl = __longlong_as_double((long long)A.dtbits[0]);
+  const double dt = A.cfl * dtcfl;
+  const double a1 = dt / A.dx1b[i - 1], a2 = dt / A.dx2b[j - 1], a3 = dt / A.dx3b[k - 1];
+  const Met M{A.metC, G.P2, q2(i, k, G)};
+
+  double w[9];
+#pragma unroll
+  for (int n = 0; n < 9; n++) w[n] = A.W[(long long)n * N + c];
+
+  // geometric source: sf(2..5) = sqrt(-g) T^m_n Gamma^n_{(1,2,3,0) m}
+  double sf[4];
+  {
+    Prel s;
+    prelude(w, gam, M, s);
+    double ta = 0.0, tb = 0.0, tc = 0.0, td = 0.0;
+    const long long q = q2(i, k, G);
+#pragma unroll
+    for (int n = 0; n < 4; n++)
+#pragma unroll
+      for (int m = 0; m < 4; m++) {
+        const double dl = (m == n) ? 1.0 : 0.0;
+        const double tv = (s.roh + s.bbsq) * s.ucon[m] * s.ucov[n] + (s.pr + 0.5 * s.bbsq) * dl - s.bb[m] * s.bbcov[n];
+        ta = ta + tv * A.chr[(long long)(n + 4 * 1 + 16 * m) * G.P2 + q];
+        tb = tb + tv * A.chr[(long long)(n + 4 * 2 + 16 * m) * G.P2 + q];
+        tc = tc + tv * A.chr[(long long)(n + 4 * 3 + 16 * m) * G.P2 + q];
+        td = td + tv * A.chr[(long long)(n + 4 * 0 + 16 * m) * G.P2 + q];
+      }
+    const double dg = M.detg();
+    sf[0] = dg * ta; sf[1] = dg * tb; sf[2] = dg * tc; sf[3] = dg * td;
+  }
+
+  double u[9];
+#pragma unroll
+  for (int m = 0; m < 9; m++) {
+    const long long o = (long long)m * N + c;
+    double f1c = A.F1[o], f1m = A.F1[o - 1];
+    double f2c = A.F2[o], f2m = A.F2[o - sj];
+    double f3c = A.F3[o], f3m = A.F3[o - sk];
+    if (m == 7) { f1c = A.CT[2 * N + c]; f1m = A.CT[2 * N + c - 1]; f3c = A.CT[3 * N + c]; f3m = A.CT[3 * N + c - sk]; }
+    if (m == 8) { f1c = A.CT[4 * N + c]; f1m = A.CT[4 * N + c - 1]; f2c = A.CT[5 * N + c]; f2m = A.CT[5 * N + c - sj]; }
+    if (m == 6) { f2c = A.CT[0 * N + c]; f2m = A.CT[0 * N + c - sj]; f3c = A.CT[1 * N + c]; f3m = A.CT[1 * N + c - sk]; }
+    const double d1 = f1c - f1m, d2 = f2c - f2m, d3 = f3c - f3m;
+    double v;
+    if (STAGE == 1) {
+      v = A.Ua[o] - a1 * d1 - a2 * d2 - a3 * d3;
+    } else if (STAGE == 2) {
+      v = (A.Ua[o] + A.Ub[o]) * 0.5 + (-a1 * d1 - a2 * d2 - a3 * d3) * 0.5;
+    } else if (STAGE == 3) {
+      v = (3.0 * A.Ua[o] + A.Ub[o]) * 0.25 + (-a1 * d1 - a2 * d2 - a3 * d3) * 0.25;
+    } else {
+      v = (A.Ua[o] + 2.0 * A.Ub[o]) * (double)(1.0f / 3.0f) + (-a1 * d1 - a2 * d2 - a3 * d3) * (double)(2.0f / 3.0f);
+    }
+    if (m >= 1 && m <= 4) {
+      const double sdt = sf[m - 1] * dt;
+      if (STAGE == 1) v = v + sdt;
+      else if (STAGE == 2) v = v + sdt * 0.5;
+      else if (STAGE == 3) v = v + sdt * 0.25;
+      else v = v + sdt * (double)(2.0f / 3.0f);
+    }
+    u[m] = v;
+  }
+
+  RecovOut ro;
+  bool carry;
+  double wgout;
+  recov_cell(u, w, M, gam, A.dmin, A.pmin, A.iter, A.r15[i - 1], A.r25[i - 1], 0.0, carry, ro, wgout);
+#pragma unroll
+  for (int n = 0; n < 9; n++) A.Wpre[(long long)n * N + c] = ro.w[n];
+  A.wgout[c] = wgout;
+  count_wave(A.cnt, C_FALLBACK, ro.fallback);
+  count_wave(A.cnt, C_FAIL, ro.fail);
+  count_wave(A.cnt, C_NEGRHO, ro.negrho);
+  count_wave(A.cnt, C_NEGP, ro.negp);
+  count_wave(A.cnt, C_VSQGT1, ro.vsqgt1);
+  count_wave(A.cnt, C_DET0, ro.det0);
+  count_wave(A.cnt, C_WGCARRY, carry);
+  count_wave(A.cnt, C_GUARD, ro.guard);
+  if (carry) {
+    // remember the cell and its RK result for the in-order fix-up pass
+#pragma unroll
+    for (int n = 0; n < 9; n++) A.Ukeep[(long long)n * N + c] = u[n];
+    const unsigned long long slot = atomicAdd(&A.carry[0], 1ULL);
+    if (slot < (unsigned long long)A.carry_cap) A.carry[1 + slot] = (unsigned long long)c;
+  }
+}
+
+// ------------------------------------------------------------------------
+// Ghost fill + floors + conserved variables, every cell.
+// bnd4 (bnd.f90:4-1059) is composed per cell: the value a ghost holds after
+// the periodic pass, then the x1, x2, x3 passes.  Then infchk (bnd.f90:1636),
+// atmosphere1a (correction.f90:226), pminmax2b (correction.f90:744) and
+// caluu1 (caluu.f90:2).
+// ------------------------------------------------------------------------
+struct BndCtx {
+  const Geo* G;
+  const double* Wpre;   // post-recov interior values
+  const double* Wold;   // values before this stage (ghosts not rewritten keep these)
+  const double* Wo;     // primitives at the start of the step (fixed-value type 2)
+  const int* bc;        // [axis*2+side][m] boundary codes
+};
+
+RH_DEV bool is_ghost(int p, int n, int nm1) { return p <= nm1 || p > n - nm1; }
+
+// value after the periodic pass (bndpefb4)
+RH_DEV double bnd_s0(const BndCtx& B, int m, int i, int j, int k) {
+  const Geo& G = *B.G;
+  const int nm1 = G.nm1;
+  const bool gi = is_ghost(i, G.ni, nm1), gj = is_ghost(j, G.nj, nm1), gk = is_ghost(k, G.nk, nm1);
+  const int ng = (int)gi + (int)gj + (int)gk;
+  if (ng == 0) return B.Wpre[(long long)m * G.N + cidx(i, j, k, G)];
+  if (ng == 1) {
+    if (gi && (B.bc[0 * 9 + m] == 1 || B.bc[1 * 9 + m] == 1)) {
+      const int Wd = G.ni - 2 * nm1;
+      const int ii = nm1 + 1 + (((i - (nm1 + 1)) % Wd) + Wd) % Wd;
+      return B.Wpre[(long long)m * G.N + cidx(ii, j, k, G)];
+    }
+    if (gj && (B.bc[2 * 9 + m] == 1 || B.bc[3 * 9 + m] == 1)) {
+      const int Wd = G.nj - 2 * nm1;
+      const int jj = nm1 + 1 + (((j - (nm1 + 1)) % Wd) + Wd) % Wd;
+      return B.Wpre[(long long)m * G.N + cidx(i, jj, k, G)];
+    }
+    if (gk && (B.bc[4 * 9 + m] == 1 || B.bc[5 * 9 + m] == 1)) {
+      const int Wd = G.nk - 2 * nm1;
+      const int kk = nm1 + 1 + (((k - (nm1 + 1)) % Wd) + Wd) % Wd;
+      return B.Wpre[(long long)m * G.N + cidx(i, j, kk, G)];
+    }
+  }
+  return B.Wold[(long long)m * G.N + cidx(i, j, k, G)];
+}
+
+// Value of variable m at ghost (i,j,k) after the three physical passes.  Each
+// pass maps the cell to its source in the previous pass (copy, mirror, with a
+// sign for antisymmetric codes) or ends in a constant: walking x3 -> x2 -> x1
+// and then reading the periodic-pass value composes the passes exactly.
+RH_DEV double bnd_value(const BndCtx& B, int m, int i, int j, int k) {
+  const Geo& G = *B.G;
+  const int nm1 = G.nm1;
+  double sgn = 1.0;
+  int c[4] = {0, i, j, k};
+  const int n[4] = {0, G.ni, G.nj, G.nk};
+#pragma unroll
+  for (int ax = 3; ax >= 1; ax--) {
+    const int p = c[ax];
+    if (!is_ghost(p, n[ax], nm1)) continue;
+    const bool outer = p > n[ax] - nm1;
+    const int type = B.bc[((ax - 1) * 2 + (outer ? 1 : 0)) * 9 + m];
+    switch (type) {
+      case 2: return sgn * B.Wo[(long long)m * G.N + cidx(c[1], c[2], c[3], G)];
+      case 5: c[ax] = outer ? n[ax] - nm1 : nm1 + 1; break;
+      case 12: sgn = -sgn; c[ax] = outer ? 2 * n[ax] - 2 * nm1 + 1 - p : 2 * nm1 - p + 1; break;
+      case 13: return sgn * 0.0;
+      case 14: c[ax] = outer ? 2 * n[ax] - 2 * nm1 + 1 - p : 2 * nm1 - p + 1; break;
+      case 18: {
+        // bnd.f90:598-610 (x1 outer only): no inflow through the outer boundary
+        const double v2 = bnd_s0(B, 1, n[ax] - nm1, c[2], c[3]);
+        if (v2 < 0.0) return sgn * 0.0;
+        c[ax] = n[ax] - nm1;
+        break;
+      }
+      default: break;  // 1: periodic, filled by the periodic pass
+    }
+  }
+  return sgn * bnd_s0(B, m, c[1], c[2], c[3]);
+}
+
+__global__ void __launch_bounds__(256) k_post(Geo G, PostArgs A) {
+  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  int i, j, k;
+  if (!box_index(t, 1, G.ni, 1, G.nj, 1, G.nk, i, j, k)) return;
+  const long long N = G.N, c = cidx(i, j, k, G);
+  const double gam = A.gam;
+  const Met M{A.metC, G.P2, q2(i, k, G)};
+  double w[9];
+  const bool ghost = is_ghost(i, G.ni, G.nm1) || is_ghost(j, G.nj, G.nm1) || is_ghost(k, G.nk, G.nm1);
+  if (!ghost) {
+#pragma unroll
+    for (int n = 0; n < 9; n++) w[n] = A.Wpre[(long long)n * N + c];
+  } else {
+    const BndCtx B{&G, A.Wpre, A.Wold, A.Wo, A.bc};
+#pragma unroll 1
+    for (int n = 0; n < 9; n++) w[n] = bnd_value(B, n, i, j, k);
+  }
+  // infchk, outer radial ghosts (bnd.f90:1740-1819); model 7..20
+  if (A.model >= 7 && A.model <= 20 && i > G.ni - G.nm1) {
+    const double gfl = calgfl(w[1], w[2], w[3], M);
+    double vel[4];
+    vel[1] = w[1] * (1.0 / gfl);
+    vel[2] = w[2] * (1.0 / gfl);
+    vel[3] = w[3] * (1.0 / gfl);
+    if (vel[1] < 0.0) {
+      const double alpha1 = 1.0 / sqrt(-M.gi(0, 0));
+      const double beta1 = alpha1 * alpha1 * M.gi(0, 1);
+      vel[1] = beta1 * (1.0 / alpha1);
+      double vsq = 0.0;
+#pragma unroll
+      for (int m = 1; m <= 3; m++)
+#pragma unroll
+        for (int nn = 1; nn <= 3; nn++) vsq = vsq + M.g(m, nn) * vel[m] * vel[nn];
+      if (vsq >= 1.0) vsq = 1.0 - 1.0 / (10.0 * 10.0);
+      const double g2 = 1.0 / sqrt(1.0 - vsq);
+      w[1] = vel[1] * g2; w[2] = vel[2] * g2; w[3] = vel[3] * g2;
+    }
+  }
+  const double demin = A.dmin * A.r15[i - 1];
+  const double prmin = (gam - 1.0) * A.pmin * A.r25[i - 1];
+  // atmosphere1a (model 7..15): correction.f90:276-393
+  bool atm = false;
+  if (A.model >= 7 && A.model <= 15) {
+    const double dthe = 1.0 * (1.0 + 1.0e-2) * demin;
+    const double dthe1 = 1.0 * (1.0 + 1.0e-2) * prmin;
+    const double bsq = w[6] * w[6] + w[7] * w[7] + w[8] * w[8];
+    if (w[0] < dthe || (w[4] < dthe1 && bsq <= 1.0e-20)) {
+      atm = true;
+      double vsq = 0.0;
+      const double vel[4] = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+      for (int m = 1; m <= 3; m++)
+#pragma unroll
+        for (int nn = 1; nn <= 3; nn++) vsq = vsq + M.g(m, nn) * vel[m] * vel[nn];
+      if (vsq >= 1.0) vsq = 1.0 - 1.0 / (10.0 * 10.0);
+      const double g2 = 1.0 / sqrt(1.0 - vsq);
+      w[0] = demin; w[4] = prmin;
+      w[1] = vel[1] * g2; w[2] = vel[2] * g2; w[3] = vel[3] * g2;
+    }
+    w[5] = 0.0;
+  }
+  count_wave(A.cnt, C_ATM, atm);
+  // pminmax2b: correction.f90:808-961
+  bool scaled = false, hit = false;
+  {
+    const double gflmax = 20.0;
+    double g3[9], gi3[9];
+    cal3met(M, g3, gi3);
+    double gfl = calgfl(w[1], w[2], w[3], M);
+    double vel[4];
+    vel[1] = w[1] * (1.0 / gfl);
+    vel[2] = w[2] * (1.0 / gfl);
+    vel[3] = w[3] * (1.0 / gfl);
+    double vsq = 0.0;
+#pragma unroll
+    for (int m = 1; m <= 3; m++)
+#pragma unroll
+      for (int nn = 1; nn <= 3; nn++) vsq = vsq + g3[(m - 1) + 3 * (nn - 1)] * vel[m] * vel[nn];
+    if (vsq >= 1.0 || gfl > gflmax) vsq = 1.0 - 1.0 / (gflmax * gflmax);
+    gfl = 1.0 / sqrt(1.0 - vsq);
+    Prel s;
+    const double wl[9] = {w[0], vel[1] * gfl, vel[2] * gfl, vel[3] * gfl, w[4], w[5], w[6], w[7], w[8]};
+    prelude(wl, gam, M, s);  // gfl, u^mu, b^mu of the limited velocity
+    gfl = s.gfl;
+    const double bcon1[4] = {0.0, w[6], w[7], w[8]};
+    double bcov1[4];
+    lower(bcon1, bcov1, M);
+    const double bsq = bcon1[1] * bcov1[1] + bcon1[2] * bcov1[2] + bcon1[3] * bcov1[3];
+    double de = w[0], pr = w[4];
+    int guard = 0;
+    for (; guard < 100000; guard++) {
+      const double roh = de + (gam / (gam - 1.0)) * pr;
+      const double t1a = roh * gfl * gfl - pr;
+      const double t1b = fabs((double)0.05f * bsq);
+      if (t1a <= t1b) { de = (double)1.05f * de; pr = (double)1.05f * pr; scaled = true; continue; }
+      break;
+    }
+    if (de != w[0]) w[0] = de;
+    else if (pr != w[4]) w[4] = pr;
+    if (pr < prmin) { w[4] = prmin; hit = true; }
+    if (pr > A.pmax) { w[4] = A.pmax; hit = true; }
+    if (de < demin) { w[0] = demin; hit = true; }
+    const double prro = w[4] * (1.0 / w[0]);
+    if (prro > 1.0e6) { w[0] = w[4] / 1.0e6; hit = true; }
+    const double bbro = s.bbsq * (1.0 / w[0]);
+    const double bbpr = s.bbsq * (1.0 / w[4]);
+    if (bbro > 50.0) { w[0] = s.bbsq / 50.0; hit = true; }
+    if (bbpr > 2.0e3) { w[4] = s.bbsq / 2.0e3; hit = true; }
+  }
+  count_wave(A.cnt, C_PMMSCALE, scaled);
+  count_wave(A.cnt, C_PMMFLOOR, hit);
+#pragma unroll
+  for (int n = 0; n < 9; n++) A.Wout[(long long)n * N + c] = w[n];
+  // caluu1: conserved variables of every cell
+  {
+    Prel s;
+    prelude(w, gam, M, s);
+    double u[9];
+    tmunu_row(s, 0, M.detg(), pow(s.de, gam), u);
+#pragma unroll
+    for (int n = 0; n < 9; n++) A.Uout[(long long)n * N + c] = u[n];
+  }
+}
+
+// ------------------------------------------------------------------------
+// CFL: cdtcfl4 (cdtcfl.f90:2-129) with calcha4 centre speeds; the minimum is
+// taken per wavefront with shuffles, then per block, then with one 64-bit
+// atomicMin on the bit pattern (positive doubles order like their bits).
+// ------------------------------------------------------------------------
+RH_DEV double wave_min(double v) {
+#pragma unroll
+  for (int off = 32; off >= 1; off >>= 1) {
+    const double o = __shfl_xor(v, off, 64);
+    v = fminf_(v, o);
+  }
+  return v;
+}
+
+__global__ void __launch_bounds__(256) k_cfl(Geo G, CflArgs A) {
+  __shared__ double red[4][4];
+  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  int i, j, k;
+  double d[4] = {1.0, 1.0, 1.0, 1.0};  // dtcfl, dtx1, dtx2, dtx3 identities (cdtcfl.f90:55-58)
+  if (box_index(t, 2, G.ni - 1, 2, G.nj - 1, 2, G.nk - 1, i, j, k)) {
+    const long long N = G.N, c = cidx(i, j, k, G);
+    const Met M{A.metC, G.P2, q2(i, k, G)};
+    double w[9];
+#pragma unroll
+    for (int n = 0; n < 9; n++) w[n] = A.W[(long long)n * N + c];
+    Prel s;
+    prelude(w, A.gam, M, s);
+    double vp[4], vm[4];
+    centre_speeds(s, A.gam, M, vp, vm);
+    const int nmax = G.nm1 * 2 + (G.nm1 - 1);
+    const int nd[4] = {0, G.ni, G.nj, G.nk};
+    const double dxv[4] = {0.0, A.dx1b[i - 1], A.dx2b[j - 1], A.dx3b[k - 1]};
+    double dta[4];
+#pragma unroll
+    for (int q = 1; q <= 3; q++) {
+      double vf = fmaxf_(fmaxf_(0.0, fabs(vp[q])), fabs(vm[q]));
+      if (nd[q] > nmax && vf > 0.0) {
+        if (vf > 1.0) vf = 1.0;
+        dta[q] = fabs(dxv[q] / (vf + 1.0e-12));
+      } else {
+        dta[q] = 1.0e10;
+      }
+    }
+    d[0] = fminf_(fminf_(fminf_(1.0, dta[1]), dta[2]), dta[3]);
+    d[1] = fminf_(1.0, dta[1]);
+    d[2] = fminf_(1.0, dta[2]);
+    d[3] = fminf_(1.0, dta[3]);
+  }
+  const int lane = threadIdx.x % 64, wid = threadIdx.x / 64;
+#pragma unroll
+  for (int q = 0; q < 4; q++) {
+    const double v = wave_min(d[q]);
+    if (lane == 0) red[wid][q] = v;
+  }
+  __syncthreads();
+  if (threadIdx.x < 4) {
+    double v = red[0][threadIdx.x];
+    for (int w = 1; w < (int)(blockDim.x / 64); w++) v = fminf_(v, red[w][threadIdx.x]);
+    atomicMin(&A.dtbits[threadIdx.x], (unsigned long long)__double_as_longlong(v));
+  }
+}
+
+// Reset the CFL accumulators and advance nothing: stream-ordered, graph-safe.
+__global__ void k_cfl_reset(unsigned long long* dtbits) {
+  if (threadIdx.x < 4) dtbits[threadIdx.x] = (unsigned long long)__double_as_longlong(1.0);
+}
+
+// time <- time + cfl*dtcfl ; last dt kept (main.f90:760-781)
+__global__ void k_advance_time(const unsigned long long* dtbits, double cfl, double* tdt) {
+  if (threadIdx.x == 0) {
+    const double dt = cfl * __longlong_as_double((long long)dtbits[0]);
+    tdt[0] = tdt[0] + dt;
+    tdt[1] = dt;
+  }
+}
+
+// In-order fix-up of cells whose initial W is inherited from the previous cell
+// (recov.f90:815-817 leaves wg unchanged).  One thread walks the flagged cells
+// in the reference's (k,j,i) order; each is recomputed with the final W of its
+// predecessor, which the update kernel stored in wgout.
+__global__ void k_recov_carry(Geo G, FixArgs A) {
+  if (blockIdx.x != 0 || threadIdx.x != 0) return;
+  unsigned long long n = A.carry[0];
+  if (n == 0) return;
+  if (n > (unsigned long long)A.carry_cap) { A.cnt[C_GUARD] += 1; n = (unsigned long long)A.carry_cap; }
+  // insertion sort by cell index (flagged cells are rare)
+  unsigned long long* list = A.carry + 1;
+  for (unsigned long long a = 1; a < n; a++) {
+    const unsigned long long v = list[a];
+    long long b = (long long)a - 1;
+    while (b >= 0 && list[b] > v) { list[b + 1] = list[b]; b--; }
+    list[b + 1] = v;
+  }
+  const long long N = G.N;
+  const long long sj = G.ni, sk = (long long)G.ni * G.nj;
+  for (unsigned long long e = 0; e < n; e++) {
+    const long long c = (long long)list[e];
+    const int i = (int)(c % G.ni) + 1, j = (int)((c / sj) % G.nj) + 1, k = (int)(c / sk) + 1;
+    // predecessor in loop order over the interior box
+    int pi = i - 1, pj = j, pk = k;
+    if (pi < G.nm1 + 1) { pi = G.ni - G.nm1; pj = j - 1; if (pj < G.nm1 + 1) { pj = G.nj - G.nm1; pk = k - 1; } }
+    double wg_prev = 0.0;  // recov.f90:632 initialises wg once per call
+    if (pk >= G.nm1 + 1) wg_prev = A.wgout[cidx(pi, pj, pk, G)];
+    const Met M{A.metC, G.P2, q2(i, k, G)};
+    double u[9], w[9];
+    for (int m = 0; m < 9; m++) { u[m] = A.Ustage[(long long)m * N + c]; w[m] = A.W[(long long)m * N + c]; }
+    RecovOut ro;
+    bool carry;
+    double wgout;
+    recov_cell(u, w, M, A.gam, A.dmin, A.pmin, A.iter, A.r15[i - 1], A.r25[i - 1], wg_prev, carry, ro, wgout);
+    for (int m = 0; m < 9; m++) A.Wpre[(long long)m * N + c] = ro.w[m];
+    A.wgout[c] = wgout;
+  }
+  A.carry[0] = 0;
+}
+
+// caluu1 (caluu.f90:2-126) on every cell: conserved variables from primitives.
+__global__ void __launch_bounds__(256) k_caluu1(Geo G, const double* __restrict__ W, const double* __restrict__ metC,
+                                                double gam, double* __restrict__ U) {
+  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  int i, j, k;
+  if (!box_index(t, 1, G.ni, 1, G.nj, 1, G.nk, i, j, k)) return;
+  const long long N = G.N, c = cidx(i, j, k, G);
+  const Met M{metC, G.P2, q2(i, k, G)};
+  double w[9], u[9];
+#pragma unroll
+  for (int n = 0; n < 9; n++) w[n] = W[(long long)n * N + c];
+  Prel s;
+  prelude(w, gam, M, s);
+  tmunu_row(s, 0, M.detg(), pow(s.de, gam), u);
+#pragma unroll
+  for (int n = 0; n < 9; n++) U[(long long)n * N + c] = u[n];
+}
+
+// ------------------------------------------------------------------------
+// Layout conversion between the reference's (n,i,j,k) arrays and planes.
+// ------------------------------------------------------------------------
+__global__ void k_aos_to_soa(const double* __restrict__ aos, double* __restrict__ soa, long long N) {
+  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= N * 9) return;
+  const long long c = t / 9, n = t % 9;
+  soa[n * N + c] = aos[t];
+}
+__global__ void k_soa_to_aos(const double* __restrict__ soa, double* __restrict__ aos, long long N) {
+  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= N * 9) return;
+  const long long c = t / 9, n = t % 9;
+  aos[t] = soa[n * N + c];
+}
+
+// ------------------------------------------------------------------------
+// Launch helpers (host side)
+// ------------------------------------------------------------------------
+static inline unsigned nblk(long long n, int b) { return (unsigned)((n + b - 1) / b); }
+static inline long long box(int lo, int hi) { return hi >= lo ? (long long)(hi - lo + 1) : 0; }
+
+hipError_t launch_sweep(const Geo& G, int dir, int irec, const double* W, const double* metF, const double* dxs,
+                        const double* xs, double gam, double* F, hipStream_t st) {
+  const long long n = box(G.nm0, G.ni - G.nm0) * box(G.nm0, G.nj - G.nm0) * box(G.nm0, G.nk - G.nm0);
+  if (n == 0) return hipSuccess;
+  const unsigned nb = nblk(n, 256);
+#define L(Dd, R) hipLaunchKernelGGL((k_sweep<Dd, R>), dim3(nb), dim3(256), 0, st, G, W, metF, dxs, xs, gam, F)
+  if (irec == 5) { if (dir == 1) L(1, 5); else if (dir == 2) L(2, 5); else L(3, 5); }
+  else if (irec == 6) { if (dir == 1) L(1, 6); else if (dir == 2) L(2, 6); else L(3, 6); }
+  else { if (dir == 1) L(1, 7); else if (dir == 2) L(2, 7); else L(3, 7); }
+#undef L
+  return hipGetLastError();
+}
+
+hipError_t launch_fluxct(const Geo& G, const double* F1, const double* F2, const double* F3, double* CT, hipStream_t st) {
+  const long long n = box(G.nm1, G.ni - G.nm1) * box(G.nm1, G.nj - G.nm1) * box(G.nm1, G.nk - G.nm1);
+  if (n == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_fluxct, dim3(nblk(n, 256)), dim3(256), 0, st, G, F1, F2, F3, CT);
+  return hipGetLastError();
+}
+
+hipError_t launch_update(const Geo& G, int stage, const StageArgs& A, hipStream_t st) {
+  const long long n = box(G.nm1 + 1, G.ni - G.nm1) * box(G.nm1 + 1, G.nj - G.nm1) * box(G.nm1 + 1, G.nk - G.nm1);
+  if (n == 0) return hipSuccess;
+  const unsigned nb = nblk(n, 256);
+  if (stage == 1) hipLaunchKernelGGL(k_update<1>, dim3(nb), dim3(256), 0, st, G, A);
+  else if (stage == 2) hipLaunchKernelGGL(k_update<2>, dim3(nb), dim3(256), 0, st, G, A);
+  else if (stage == 3) hipLaunchKernelGGL(k_update<3>, dim3(nb), dim3(256), 0, st, G, A);
+  else hipLaunchKernelGGL(k_update<4>, dim3(nb), dim3(256), 0, st, G, A);
+  return hipGetLastError();
+}
+
+hipError_t launch_recov_carry(const Geo& G, const FixArgs& A, hipStream_t st) {
+  hipLaunchKernelGGL(k_recov_carry, dim3(1), dim3(64), 0, st, G, A);
+  return hipGetLastError();
+}
+
+hipError_t launch_post(const Geo& G, const PostArgs& A, hipStream_t st) {
+  hipLaunchKernelGGL(k_post, dim3(nblk(G.N, 256)), dim3(256), 0, st, G, A);
+  return hipGetLastError();
+}
+
+hipError_t launch_cfl(const Geo& G, const CflArgs& A, hipStream_t st) {
+  hipLaunchKernelGGL(k_cfl_reset, dim3(1), dim3(64), 0, st, A.dtbits);
+  const long long n = box(2, G.ni - 1) * box(2, G.nj - 1) * box(2, G.nk - 1);
+  if (n > 0) hipLaunchKernelGGL(k_cfl, dim3(nblk(n, 256)), dim3(256), 0, st, G, A);
+  return hipGetLastError();
+}
+
+hipError_t launch_advance_time(const unsigned long long* dtbits, double cfl, double* tdt, hipStream_t st) {
+  hipLaunchKernelGGL(k_advance_time, dim3(1), dim3(64), 0, st, dtbits, cfl, tdt);
+  return hipGetLastError();
+}
+
+hipError_t launch_caluu1(const Geo& G, const double* W, const double* metC, double gam, double* U, hipStream_t st) {
+  hipLaunchKernelGGL(k_caluu1, dim3(nblk(G.N, 256)), dim3(256), 0, st, G, W, metC, gam, U);
+  return hipGetLastError();
+}
+
+hipError_t launch_aos_to_soa(const double* aos, double* soa, long long N, hipStream_t st) {
+  hipLaunchKernelGGL(k_aos_to_soa, dim3(nblk(N * 9, 256)), dim3(256), 0, st, aos, soa, N);
+  return hipGetLastError();
+}
+hipError_t launch_soa_to_aos(const double* soa, double* aos, long long N, hipStream_t st) {
+  hipLaunchKernelGGL(k_soa_to_aos, dim3(nblk(N * 9, 256)), dim3(256), 0, st, soa, aos, N);
+  return hipGetLastError();
+}
+
+}  // namespace rh

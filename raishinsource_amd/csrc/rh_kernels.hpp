@@ -1,0 +1,64 @@
+// Kernel argument blocks and launchers of the GRMHD step (see rh_kernels.hip).
+#pragma once
+#include <hip/hip_runtime.h>
+#include "rh_device.hpp"
+
+namespace rh {
+
+struct StageArgs {
+  const double *W;                 // primitives at stage start (9 planes)
+  const double *Ua, *Ub;           // RK inputs: stage1 uu; stage2 uo,uh; stage3 uo,us
+  const double *F1, *F2, *F3;      // HLLE fluxes per direction (9 planes each)
+  const double *CT;                // six CT'd transverse B fluxes
+  const double *metC, *chr;        // centre metric tables, Christoffel tables
+  const double *dx1b, *dx2b, *dx3b;
+  const double *r15, *r25;         // x1(i)**(-3/2), x1(i)**(-5/2) from the host libm
+  const unsigned long long *dtbits;
+  double *Wpre;                    // recovered primitives (interior)
+  double *wgout;                   // final W per cell (inherited initial guesses)
+  double *Ukeep;                   // RK result of flagged cells (for the fix-up)
+  unsigned long long *cnt;
+  unsigned long long *carry;       // [0] count, [1..cap] flagged cells
+  long long carry_cap;
+  double gam, cfl, dmin, pmin;
+  int iter;
+};
+
+struct PostArgs {
+  const double *Wpre, *Wold, *Wo;
+  const double *metC, *r15, *r25;
+  const int *bc;                   // boundary codes [axis*2+side][9]
+  double *Wout, *Uout;
+  unsigned long long *cnt;
+  double gam, dmin, pmin, pmax;
+  int model;
+};
+
+struct CflArgs {
+  const double *W, *metC, *dx1b, *dx2b, *dx3b;
+  unsigned long long *dtbits;      // [0] dtcfl [1..3] dt1..dt3 as ordered bit patterns
+  double gam;
+};
+
+struct FixArgs {
+  const double *W, *Ustage, *metC, *r15, *r25;
+  double *Wpre, *wgout;
+  unsigned long long *carry, *cnt;
+  long long carry_cap;
+  double gam, dmin, pmin;
+  int iter;
+};
+
+hipError_t launch_sweep(const Geo& G, int dir, int irec, const double* W, const double* metF, const double* dxs,
+                        const double* xs, double gam, double* F, hipStream_t st);
+hipError_t launch_fluxct(const Geo& G, const double* F1, const double* F2, const double* F3, double* CT, hipStream_t st);
+hipError_t launch_update(const Geo& G, int stage, const StageArgs& A, hipStream_t st);
+hipError_t launch_recov_carry(const Geo& G, const FixArgs& A, hipStream_t st);
+hipError_t launch_post(const Geo& G, const PostArgs& A, hipStream_t st);
+hipError_t launch_cfl(const Geo& G, const CflArgs& A, hipStream_t st);
+hipError_t launch_advance_time(const unsigned long long* dtbits, double cfl, double* tdt, hipStream_t st);
+hipError_t launch_caluu1(const Geo& G, const double* W, const double* metC, double gam, double* U, hipStream_t st);
+hipError_t launch_aos_to_soa(const double* aos, double* soa, long long N, hipStream_t st);
+hipError_t launch_soa_to_aos(const double* soa, double* aos, long long N, hipStream_t st);
+
+}  // namespace rh

@@ -1,0 +1,454 @@
+// Host-side set-up of a model: grid, metric, Christoffel symbols and the
+// magnetised Fishbone-Moncrief torus.  These produce the inputs of the time
+// step; they run once per model on the host, in the reference's layouts.
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <vector>
+
+#include "../../include/raishin_hip.h"
+#include "../../include/raishin_setup.h"
+
+namespace {
+
+constexpr double PI = 3.14159265358979323;  // pram.f90:31
+
+inline size_t cell(const raishin_pram& p, int i, int j, int k) {  // 1-based
+  return (size_t)(i - 1) + (size_t)p.imax * ((size_t)(j - 1) + (size_t)p.jmax * (size_t)(k - 1));
+}
+#define GM(g, m, n) (g)[(m) + 4 * (n)]
+
+// kshcov1 / kshcon1: coord.f90:535-690
+void kshcov1(double akm, double* g, double rr, double x3) {
+  const double small = 1.0e-12;
+  const double theta = (x3 > PI) ? 2.0 * PI - x3 : x3;
+  double sth = std::sin(theta);
+  if (std::fabs(sth) < small) sth = (sth > 0.0) ? small : -small;
+  const double s2 = sth * sth;
+  double cth = std::cos(theta);
+  if (std::fabs(cth) < small) cth = (cth > 0.0) ? small : -small;
+  const double a2 = akm * akm, r2 = rr * rr;
+  const double del = r2 - (2.0 * rr) + a2;
+  const double sig = r2 + a2 * cth * cth;
+  const double aa = ((r2 + a2) * (r2 + a2)) - a2 * del * s2;
+  for (int e = 0; e < 16; e++) g[e] = 0.0;
+  GM(g, 0, 0) = -1.0 + 2.0 * rr / sig;
+  GM(g, 0, 1) = 2.0 * rr / sig;
+  GM(g, 0, 2) = -2.0 * akm * rr * s2 * (1.0 / sig);
+  GM(g, 1, 0) = GM(g, 0, 1);
+  GM(g, 1, 1) = 1.0 + 2.0 * rr / sig;
+  GM(g, 1, 2) = -1.0 * akm * s2 * (1.0 + 2.0 * rr / sig);
+  GM(g, 2, 0) = GM(g, 0, 2);
+  GM(g, 2, 1) = GM(g, 1, 2);
+  GM(g, 2, 2) = aa * s2 * (1.0 / sig);
+  GM(g, 3, 3) = sig;
+}
+void kshcon1(double akm, double* g, double rr, double x3) {
+  const double small = 1.0e-12;
+  const double theta = (x3 > PI) ? 2.0 * PI - x3 : x3;
+  double sth = std::sin(theta);
+  if (std::fabs(sth) < small) sth = (sth > 0.0) ? small : -small;
+  const double s2 = sth * sth;
+  double cth = std::cos(theta);
+  if (std::fabs(cth) < small) cth = (cth > 0.0) ? small : -small;
+  const double a2 = akm * akm, r2 = rr * rr;
+  const double del = r2 - (2.0 * rr) + a2;
+  const double sig = r2 + a2 * cth * cth;
+  for (int e = 0; e < 16; e++) g[e] = 0.0;
+  GM(g, 0, 0) = -1.0 - 2.0 * rr / sig;
+  GM(g, 0, 1) = 2.0 * rr / sig;
+  GM(g, 1, 0) = GM(g, 0, 1);
+  GM(g, 1, 1) = del / sig;
+  GM(g, 1, 2) = akm * (1.0 / sig);
+  GM(g, 2, 1) = GM(g, 1, 2);
+  GM(g, 2, 2) = 1.0 / (sig * s2);
+  GM(g, 3, 3) = 1.0 / sig;
+}
+// mincov1 / mincon1: coord.f90:149-198
+void minkowski(double* g, double sgn00) {
+  for (int e = 0; e < 16; e++) g[e] = 0.0;
+  GM(g, 0, 0) = sgn00; GM(g, 1, 1) = 1.0; GM(g, 2, 2) = 1.0; GM(g, 3, 3) = 1.0;
+}
+void metric_cov(const raishin_pram& p, double* g, double r, double th) {
+  if (p.metric == 303) kshcov1(p.akm, g, r, th);
+  else minkowski(g, -1.0);
+}
+void metric_con(const raishin_pram& p, double* g, double r, double th) {
+  if (p.metric == 303) kshcon1(p.akm, g, r, th);
+  else minkowski(g, -1.0);
+}
+
+// kercov1: coord.f90:369-450 (Boyer-Lindquist, used by the torus set-up)
+void kercov1(double akm, double* g, double rr, double x3) {
+  const double small = 1.0e-12;
+  const double theta = (x3 > PI) ? 2.0 * PI - x3 : x3;
+  double sth = std::sin(theta);
+  if (std::fabs(sth) < small) sth = (sth > 0.0) ? small : -small;
+  const double s2 = sth * sth;
+  double cth = std::cos(theta);
+  if (std::fabs(cth) < small) cth = (cth > 0.0) ? small : -small;
+  const double a2 = akm * akm, r2 = rr * rr;
+  const double del = 1.0 - (2.0 / rr) + (a2 * (1.0 / r2));
+  const double sig = 1.0 + a2 * cth * cth * (1.0 / r2);
+  for (int e = 0; e < 16; e++) g[e] = 0.0;
+  GM(g, 0, 0) = -(1.0 - 2.0 * (1.0 / (rr * sig)));
+  GM(g, 0, 2) = -2.0 * akm * s2 * (1.0 / (rr * sig));
+  GM(g, 1, 1) = sig / del;
+  GM(g, 2, 0) = GM(g, 0, 2);
+  GM(g, 2, 2) = r2 * s2 * (1.0 + (a2 * (1.0 / r2)) + 2.0 * a2 * s2 * (1.0 / (r2 * rr * sig)));
+  GM(g, 3, 3) = r2 * sig;
+}
+
+// transbl2ks: physfunc.f90:366-505 (metric 303 branch): BL u~ -> KS u~
+void transbl2ks(double akm, const double ut[4], double out[4], double rr, double th) {
+  const double rbh1 = 1.0 + std::sqrt(1.0 - akm * akm);
+  double gi[16];
+  kshcon1(akm, gi, rr, th);
+  const double alpha1 = 1.0 / std::sqrt(-GM(gi, 0, 0));
+  double beta1[4] = {0.0, alpha1 * alpha1 * GM(gi, 0, 1), alpha1 * alpha1 * GM(gi, 0, 2), alpha1 * alpha1 * GM(gi, 0, 3)};
+  if (rr > rbh1) {
+    double g[16];
+    kercov1(akm, g, rr, th);
+    double ucon[4] = {0.0, ut[1], ut[2], ut[3]};
+    const double aa = GM(g, 0, 0);
+    const double bb = 2.0 * (GM(g, 0, 1) * ucon[1] + GM(g, 0, 2) * ucon[2] + GM(g, 0, 3) * ucon[3]);
+    const double cc = 1.0 + GM(g, 1, 1) * ucon[1] * ucon[1] + GM(g, 2, 2) * ucon[2] * ucon[2] +
+                      GM(g, 3, 3) * ucon[3] * ucon[3] +
+                      2.0 * (GM(g, 1, 2) * ucon[1] * ucon[2] + GM(g, 1, 3) * ucon[1] * ucon[3] + GM(g, 2, 3) * ucon[2] * ucon[3]);
+    const double discr = bb * bb - 4.0 * aa * cc;
+    ucon[0] = (-bb - std::sqrt(discr)) * (1.0 / (2.0 * aa));
+    double tr[16] = {0.0};
+    const double tmp2 = rr * rr - (2.0 * rr) + akm * akm;
+    if (tmp2 > 0.0) {
+      GM(tr, 0, 0) = 1.0; GM(tr, 0, 1) = 2.0 * rr * (1.0 / tmp2); GM(tr, 1, 1) = 1.0;
+      GM(tr, 2, 2) = 1.0; GM(tr, 2, 1) = akm * (1.0 / tmp2); GM(tr, 3, 3) = 1.0;
+    }
+    double u1[4] = {0.0, 0.0, 0.0, 0.0};
+    for (int n = 0; n < 4; n++)
+      for (int m = 0; m < 4; m++) u1[m] = u1[m] + GM(tr, m, n) * ucon[n];
+    const double gfl = u1[0] * alpha1;
+    for (int m = 1; m <= 3; m++) out[m] = u1[m] + gfl * beta1[m] * (1.0 / alpha1);
+  } else {
+    for (int m = 1; m <= 3; m++) out[m] = beta1[m] * (1.0 / alpha1);
+  }
+}
+
+// callfish: mdgrmhd.f90:808-834 (angular momentum of the FM torus)
+double callfish(double akm, double rmax) {
+  const double t1a = ((akm * akm) - 2.0 * akm * std::sqrt(rmax) + (rmax * rmax));
+  const double t1b = (-2.0 * akm * rmax) * t1a;
+  const double t1c = std::sqrt(2.0 * akm * std::sqrt(rmax) + (-3.0 + rmax) * rmax);
+  const double t1d = (akm + (-2.0 + rmax) * std::sqrt(rmax));
+  const double t1e = ((rmax * rmax * rmax) + (akm * akm) * (2.0 + rmax));
+  const double t1f = std::sqrt(1.0 + (2.0 * akm / std::pow(rmax, 1.5)) - (3.0 / rmax));
+  const double t2a = (rmax * rmax * rmax) * std::sqrt(2.0 * akm * std::sqrt(rmax) + (-3.0 + rmax) * rmax);
+  const double t2b = ((akm * akm) + (-2.0 + rmax) * rmax);
+  return (t1a * ((t1b / t1c) + t1d * t1e / t1f)) / (t2a * t2b);
+}
+
+// b^2 with the reference's helpers (physfunc.f90) for the beta normalisation
+double bsq_of(const double* w, const double* g, const double* gi) {
+  double ut[4] = {0.0, w[1], w[2], w[3]};
+  double utsq = 0.0;
+  for (int m = 1; m <= 3; m++)
+    for (int n = 1; n <= 3; n++) utsq = utsq + GM(g, m, n) * ut[m] * ut[n];
+  if (utsq < 0.0) utsq = (std::fabs(utsq) <= 1.0e-10) ? std::fabs(utsq) : 1.0e-10;
+  if (std::fabs(utsq) > 5.0e2) utsq = 5.0e2;
+  const double gfl = std::sqrt(1.0 + utsq);
+  const double alpha1 = 1.0 / std::sqrt(-GM(gi, 0, 0));
+  double ucon[4], ucov[4], bb[4], bbcov[4];
+  ucon[0] = gfl * (1.0 / alpha1);
+  for (int m = 1; m <= 3; m++) ucon[m] = ut[m] - gfl * (alpha1 * alpha1 * GM(gi, 0, m)) * (1.0 / alpha1);
+  for (int m = 0; m < 4; m++) ucov[m] = GM(g, m, 0) * ucon[0] + GM(g, m, 1) * ucon[1] + GM(g, m, 2) * ucon[2] + GM(g, m, 3) * ucon[3];
+  const double b[4] = {0.0, w[6], w[7], w[8]};
+  bb[0] = b[1] * ucov[1] + b[2] * ucov[2] + b[3] * ucov[3];
+  for (int m = 1; m <= 3; m++) bb[m] = (b[m] + bb[0] * ucon[m]) * (1.0 / ucon[0]);
+  for (int m = 0; m < 4; m++) bbcov[m] = GM(g, m, 0) * bb[0] + GM(g, m, 1) * bb[1] + GM(g, m, 2) * bb[2] + GM(g, m, 3) * bb[3];
+  return bb[0] * bbcov[0] + bb[1] * bbcov[1] + bb[2] * bbcov[2] + bb[3] * bbcov[3];
+}
+
+inline double splitmix_uniform(uint64_t& s) {
+  uint64_t z = (s += 0x9E3779B97F4A7C15ULL);
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
+  z = z ^ (z >> 31);
+  return (double)(z >> 11) * (1.0 / 9007199254740992.0);
+}
+
+}  // namespace
+
+extern "C" {
+
+void raishin_grid(const raishin_pram* pp, const raishin_grid_ext* ext, double* x1, double* x2, double* x3,
+                  double* dx1, double* dx2, double* dx3, double* dx1b, double* dx2b, double* dx3b, double* x1a,
+                  double* x2a, double* x3a, double* x1b, double* x2b, double* x3b) {
+  const raishin_pram& p = *pp;
+  const int n[3] = {p.imax, p.jmax, p.kmax};
+  const int ix[3] = {ext->ix1, 1, 1};
+  const double lo[3] = {ext->xmin, ext->ymin, ext->zmin}, hi[3] = {ext->xmax, ext->ymax, ext->zmax};
+  double* X[3] = {x1, x2, x3};
+  double* DX[3] = {dx1, dx2, dx3};
+  double* DXB[3] = {dx1b, dx2b, dx3b};
+  double* XA[3] = {x1a, x2a, x3a};
+  double* XB[3] = {x1b, x2b, x3b};
+  for (int a = 0; a < 3; a++) {
+    const int nn = n[a];
+    double *x = X[a], *dx = DX[a], *xa = XA[a], *xb = XB[a], *dxb = DXB[a];
+    if (ix[a] == 2) {
+      // logarithmic radius: grid.f90:31-50; 1./float(imax-1) is single precision
+      const double xx0 = 0.0;
+      const double s0 = std::log(lo[a] - xx0), s1 = std::log(hi[a] - xx0);
+      const double dxt = (s1 - s0) * (double)(1.0f / (float)(nn - 1));
+      for (int i = 1; i <= nn; i++) {
+        const double xt = s0 + (double)(float)(i - 1) * dxt;
+        x[i - 1] = std::exp(xt) + xx0;
+        xb[i - 1] = xt;
+      }
+      for (int i = 2; i <= nn; i++) dx[i - 1] = x[i - 1] - x[i - 2];
+      dx[0] = dx[1];
+    } else {
+      // uniform: grid.f90:22-28, 146-152, 175-181
+      for (int i = 1; i <= nn; i++) {
+        dx[i - 1] = (hi[a] - lo[a]) / (double)(float)(nn - 1);
+        x[i - 1] = lo[a] + dx[i - 1] * (double)(float)(i - 1);
+        xb[i - 1] = x[i - 1];
+      }
+    }
+    for (int i = 1; i <= nn - 1; i++) xa[i - 1] = 0.5 * (x[i - 1] + x[i]);
+    xa[nn - 1] = xa[nn - 2] + dx[nn - 1];
+    // dx*b(i) = x*a(i) - x*a(i-1), i = 2..n-1 kept (grid.f90:103-108 writes one past the end)
+    for (int i = 2; i <= nn - 1; i++) dxb[i - 1] = xa[i - 1] - xa[i - 2];
+    dxb[0] = (nn > 2) ? dxb[1] : xa[1] - xa[0];
+  }
+}
+
+void raishin_coord1(const raishin_pram* pp, double* gcov, double* gcovi, double* gcovj, double* gcovk, double* gcon,
+                    double* gconi, double* gconj, double* gconk, double* detg, double* detgi, double* detgj,
+                    double* detgk, const double* x1, const double* x2, const double* x3, const double* x1a,
+                    const double* x2a, const double* x3a) {
+  // coord.f90:5-146 (metric 1 and 303) and caldetg (coord.f90:1016-1086)
+  const raishin_pram& p = *pp;
+  (void)x2; (void)x2a;
+  for (int k = 1; k <= p.kmax; k++)
+    for (int j = 1; j <= p.jmax; j++)
+      for (int i = 1; i <= p.imax; i++) {
+        const size_t c = cell(p, i, j, k);
+        double g[16], gi[16], gk[16], h[16], hi[16], hk[16];
+        metric_cov(p, g, x1[i - 1], x3[k - 1]);
+        metric_cov(p, gi, x1a[i - 1], x3[k - 1]);
+        metric_cov(p, gk, x1[i - 1], x3a[k - 1]);
+        metric_con(p, h, x1[i - 1], x3[k - 1]);
+        metric_con(p, hi, x1a[i - 1], x3[k - 1]);
+        metric_con(p, hk, x1[i - 1], x3a[k - 1]);
+        std::memcpy(&gcov[16 * c], g, 128); std::memcpy(&gcovi[16 * c], gi, 128);
+        std::memcpy(&gcovj[16 * c], g, 128); std::memcpy(&gcovk[16 * c], gk, 128);
+        std::memcpy(&gcon[16 * c], h, 128); std::memcpy(&gconi[16 * c], hi, 128);
+        std::memcpy(&gconj[16 * c], h, 128); std::memcpy(&gconk[16 * c], hk, 128);
+      }
+  double* D[4] = {detg, detgi, detgj, detgk};
+  const double* Gs[4] = {gcov, gcovi, gcovj, gcovk};
+  const size_t N = (size_t)p.imax * p.jmax * p.kmax;
+  for (int s = 0; s < 4; s++)
+    for (size_t c = 0; c < N; c++) {
+      const double* g = &Gs[s][16 * c];
+      double aa, bb, cc, pp_, qq, rr, xx, yy, zz;
+      aa = GM(g, 1, 1); bb = GM(g, 1, 2); cc = GM(g, 1, 3); pp_ = GM(g, 2, 1); qq = GM(g, 2, 2); rr = GM(g, 2, 3);
+      xx = GM(g, 3, 1); yy = GM(g, 3, 2); zz = GM(g, 3, 3);
+      const double t1 = aa * qq * zz + bb * rr * xx + cc * pp_ * yy - cc * qq * xx - bb * pp_ * zz - aa * rr * yy;
+      aa = GM(g, 0, 1); bb = GM(g, 0, 2); cc = GM(g, 0, 3);
+      const double t2 = aa * qq * zz + bb * rr * xx + cc * pp_ * yy - cc * qq * xx - bb * pp_ * zz - aa * rr * yy;
+      pp_ = GM(g, 1, 1); qq = GM(g, 1, 2); rr = GM(g, 1, 3);
+      const double t3 = aa * qq * zz + bb * rr * xx + cc * pp_ * yy - cc * qq * xx - bb * pp_ * zz - aa * rr * yy;
+      xx = GM(g, 2, 1); yy = GM(g, 2, 2); zz = GM(g, 2, 3);
+      const double t4 = aa * qq * zz + bb * rr * xx + cc * pp_ * yy - cc * qq * xx - bb * pp_ * zz - aa * rr * yy;
+      D[s][c] = std::sqrt(-1.0 * (GM(g, 0, 0) * t1 - GM(g, 1, 0) * t2 + GM(g, 2, 0) * t3 - GM(g, 3, 0) * t4));
+    }
+}
+
+void raishin_calchrist(const raishin_pram* pp, double* christ, const double* x1, const double* x2, const double* x3) {
+  // calchrist: coord.f90:1089-1321 (finite-difference metric derivatives, delta=1e-5)
+  const raishin_pram& p = *pp;
+  const double delta = 1.0e-5, small = 1.0e-12;
+  for (int k = 1; k <= p.kmax; k++)
+    for (int j = 1; j <= p.jmax; j++)
+      for (int i = 1; i <= p.imax; i++) {
+        const double r = x1[i - 1], ph = x2[j - 1], th = x3[k - 1];
+        const double ria = r + delta, rib = r - delta, pja = ph + delta, pjb = ph - delta, tka = th + delta,
+                     tkb = th - delta;
+        double gia[16], gib[16], gja[16], gjb[16], gka[16], gkb[16], gi[16];
+        metric_cov(p, gia, ria, th); metric_cov(p, gib, rib, th);
+        metric_cov(p, gja, r, th); metric_cov(p, gjb, r, th);  // no phi dependence
+        metric_cov(p, gka, r, tka); metric_cov(p, gkb, r, tkb);
+        metric_con(p, gi, r, th);
+        double dg[4][4][4];  // dg[l][m][n] = d g_lm / dx^n
+        for (int m = 0; m < 4; m++)
+          for (int l = 0; l < 4; l++) {
+            dg[l][m][0] = 0.0;
+            dg[l][m][1] = (GM(gia, l, m) - GM(gib, l, m)) / (ria - rib + small);
+            dg[l][m][2] = (GM(gja, l, m) - GM(gjb, l, m)) / (pja - pjb + small);
+            dg[l][m][3] = (GM(gka, l, m) - GM(gkb, l, m)) / (tka - tkb + small);
+          }
+        double t2[4][4][4];
+        for (int n = 0; n < 4; n++)
+          for (int m = 0; m < 4; m++)
+            for (int l = 0; l < 4; l++) t2[l][m][n] = 0.5 * (dg[m][l][n] + dg[n][l][m] - dg[n][m][l]);
+        double* ch = &christ[64 * cell(p, i, j, k)];
+        for (int n = 0; n < 4; n++)
+          for (int m = 0; m < 4; m++)
+            for (int l = 0; l < 4; l++)
+              ch[l + 4 * m + 16 * n] = GM(gi, l, 0) * t2[0][m][n] + GM(gi, l, 1) * t2[1][m][n] +
+                                       GM(gi, l, 2) * t2[2][m][n] + GM(gi, l, 3) * t2[3][m][n];
+      }
+}
+
+int raishin_mdtorus1(const raishin_pram* pp, const raishin_torus* tp, double* uri, const double* gcov,
+                     const double* gcon, const double* detg, const double* x1, const double* x2, const double* x3) {
+  // Fishbone-Moncrief torus with a single poloidal loop, after mdtorus1
+  // (mdgrmhd.f90:879-1654): atmosphere, torus, density normalisation, seeded
+  // pressure noise, A_phi = max(rho/rho_max - 0.2, 0), field scaled to beta_min.
+  const raishin_pram& p = *pp;
+  const raishin_torus& t = *tp;
+  (void)x2;
+  const double gam = p.gam, akm = p.akm, kpol1 = 1.0e-3;
+  const double rin = t.rin, rmax = t.rmax;
+  const double thedge = (double)0.02f * PI, thedge1 = (double)0.98f * PI, small = 1.0e-12;
+  const int ni = p.imax, nj = p.jmax, nk = p.kmax;
+  const size_t N = (size_t)ni * nj * nk;
+  std::vector<double> alnh(N, 0.0);
+  for (int k = 1; k <= nk; k++)
+    for (int j = 1; j <= nj; j++)
+      for (int i = 1; i <= ni; i++) {
+        double* w = &uri[9 * cell(p, i, j, k)];
+        const double rr = x1[i - 1];
+        const double de = p.dmin * std::pow(rr, -1.5), pr = p.pmin * (gam - 1.0) * std::pow(rr, -2.5);
+        w[0] = de; w[1] = w[2] = w[3] = 0.0; w[4] = pr; w[5] = pr * (1.0 / std::pow(de, gam - 1.0));
+        w[6] = w[7] = w[8] = 0.0;
+      }
+  if (p.metric != 303) return 0;
+  const double clm = callfish(akm, rmax);
+  const double delin = rin * rin - 2.0 * rin + akm * akm;
+  const double aain = (rin * rin + akm * akm) * (rin * rin + akm * akm) - delin * akm * akm;
+  const double sigin = rin * rin;
+  const double t2a = 4.0 * (clm * clm * sigin * sigin) * delin * (1.0 / (aain * aain));
+  const double t2c = sigin * delin * (1.0 / aain);
+  const double t2d = 0.5 * std::log((1.0 + std::sqrt(1.0 + t2a)) * (1.0 / t2c));
+  const double t2e = 0.5 * std::sqrt(1.0 + t2a);
+  const double t2f = 2.0 * akm * rin * clm * (1.0 / aain);
+  double rhomax = 0.0;
+  for (int k = 1; k <= nk; k++)
+    for (int j = 1; j <= nj; j++)
+      for (int i = 1; i <= ni; i++) {
+        const double rr = x1[i - 1], th = x3[k - 1];
+        double sth = std::sin(th);
+        if (std::fabs(sth) < small) sth = (sth >= 0.0) ? small : -small;
+        const double cth = std::cos(th);
+        const double del = (rr * rr) - (2.0 * rr) + (akm * akm);
+        const double aa = (rr * rr + akm * akm) * (rr * rr + akm * akm) - del * akm * akm * sth * sth;
+        const double sig = rr * rr + akm * akm * cth * cth;
+        double al = 0.0;
+        if (rr >= rin && th >= thedge && th <= thedge1) {
+          const double t1a = 4.0 * (clm * clm * sig * sig) * del * (1.0 / (aa * sth * aa * sth));
+          const double t1c = sig * del * (1.0 / aa);
+          const double t1d = 0.5 * std::log((1.0 + std::sqrt(1.0 + t1a)) * (1.0 / t1c));
+          const double t1e = 0.5 * std::sqrt(1.0 + t1a);
+          const double t1f = 2.0 * akm * rr * clm * (1.0 / aa);
+          al = t1d - t1e - t1f - (t2d - t2e - t2f);
+          if (al <= 0.0) al = 0.0;
+        }
+        alnh[cell(p, i, j, k)] = al;
+        const double hm1 = std::exp(al) - 1.0;
+        if (hm1 > 0.0 && rr >= rin && th >= thedge && th <= thedge1) {
+          const double de = std::pow(hm1 * (gam - 1.0) / (kpol1 * gam), 1.0 / (gam - 1.0));
+          const double pr = kpol1 * std::pow(de, gam);
+          const double expm2chi = sig * sig * del * (1.0 / (aa * aa * sth * sth));
+          const double up1 = std::sqrt((-1.0 + std::sqrt(1.0 + 4.0 * clm * clm * expm2chi)) / 2.0);
+          const double t3a = 2.0 * akm * rr * std::sqrt(1.0 + up1 * up1);
+          const double t3b = std::sqrt(aa * sig * del);
+          const double t3c = std::sqrt(sig * (1.0 / aa)) * up1 * (1.0 / sth);
+          double ut[4] = {0.0, 0.0, t3a * (1.0 / t3b) + t3c, 0.0}, un[4];
+          transbl2ks(akm, ut, un, rr, th);
+          double* w = &uri[9 * cell(p, i, j, k)];
+          w[0] = de; w[1] = un[1]; w[2] = un[2]; w[3] = un[3]; w[4] = pr;
+          w[5] = pr * (1.0 / std::pow(de, gam - 1.0));
+          if (de > rhomax) rhomax = de;
+        }
+      }
+  if (rhomax <= 0.0) return -1;
+  // normalise density to 1 and perturb the pressure.  The noise is drawn in the
+  // reference's (k,j,i) order from a seeded splitmix64 stream; with one interior
+  // phi cell the model is axisymmetric and one value is drawn per (i,k).
+  const bool axisym = (nj - 2 * ((p.irec == 5 ? 4 : 3) + 1)) <= 1;
+  uint64_t seed = (uint64_t)t.seed;
+  std::vector<double> noise((size_t)ni * nk, 0.0);
+  for (int k = 1; k <= nk; k++)
+    for (int j = 1; j <= nj; j++)
+      for (int i = 1; i <= ni; i++) {
+        const double rr = x1[i - 1], th = x3[k - 1];
+        const double rhomin1 = p.dmin * std::pow(rr, -1.5), prmin1 = p.pmin * (gam - 1.0) * std::pow(rr, -2.5);
+        const double hm1 = std::exp(alnh[cell(p, i, j, k)]) - 1.0;
+        if (!(hm1 > 0.0 && rr >= rin && th >= thedge && th <= thedge1)) continue;
+        double* w = &uri[9 * cell(p, i, j, k)];
+        double de = w[0] * (1.0 / rhomax), pr = w[4] * (1.0 / rhomax);
+        double u;
+        if (axisym) {
+          double& nz = noise[(size_t)(i - 1) + (size_t)ni * (k - 1)];
+          if (j == 1) nz = splitmix_uniform(seed);
+          u = nz;
+        } else {
+          u = splitmix_uniform(seed);
+        }
+        pr = pr * (1.0 + t.noise * (u - 0.5));
+        if (de <= rhomin1 || pr <= prmin1) { de = rhomin1; w[1] = w[2] = w[3] = 0.0; pr = prmin1; }
+        w[0] = de; w[4] = pr; w[5] = pr * (1.0 / std::pow(de, gam - 1.0));
+      }
+  // vector potential at cell corners and B = curl A / sqrt(-g): mdgrmhd.f90:1350-1460
+  std::vector<double> pom(N, 0.0);
+  for (int k = 2; k <= nk; k++)
+    for (int j = 1; j <= nj; j++)
+      for (int i = 2; i <= ni; i++) {
+        const double rhoav = 0.25 * (uri[9 * cell(p, i, j, k)] + uri[9 * cell(p, i - 1, j, k)] +
+                                     uri[9 * cell(p, i, j, k - 1)] + uri[9 * cell(p, i - 1, j, k - 1)]);
+        const double pm = (rhoav != 0.0) ? rhoav - t.acut : 0.0;
+        if (pm > 0.0) pom[cell(p, i, j, k)] = pm;
+      }
+  for (int j = 1; j <= nj; j++)
+    for (int i = 1; i <= ni; i++) pom[cell(p, i, j, 1)] = pom[cell(p, i, j, 2)];
+  for (int k = 1; k <= nk; k++)
+    for (int j = 1; j <= nj; j++) pom[cell(p, 1, j, k)] = pom[cell(p, 2, j, k)];
+  for (int k = 2; k <= nk - 1; k++)
+    for (int j = 1; j <= nj; j++)
+      for (int i = 2; i <= ni - 1; i++) {
+        const size_t c = cell(p, i, j, k);
+        const double ta = 0.5 * (pom[cell(p, i + 1, j, k + 1)] + pom[cell(p, i, j, k + 1)] - pom[cell(p, i + 1, j, k)] - pom[c]);
+        const double dx3a = 0.5 * (x3[k] + x3[k - 1]) - 0.5 * (x3[k - 1] + x3[k - 2]);
+        const double tb = 0.5 * (pom[cell(p, i + 1, j, k + 1)] + pom[cell(p, i + 1, j, k)] - pom[cell(p, i, j, k + 1)] - pom[c]);
+        const double dx1a = 0.5 * (x1[i] + x1[i - 1]) - 0.5 * (x1[i - 1] + x1[i - 2]);
+        uri[9 * c + 6] = (ta != 0.0) ? -ta * (1.0 / (dx3a * detg[c])) : 0.0;
+        uri[9 * c + 8] = (tb != 0.0) ? tb * (1.0 / (dx1a * detg[c])) : 0.0;
+      }
+  for (int j = 1; j <= nj; j++)
+    for (int i = 1; i <= ni; i++)
+      for (int q : {6, 8}) {
+        uri[9 * cell(p, i, j, 1) + q] = uri[9 * cell(p, i, j, 2) + q];
+        uri[9 * cell(p, i, j, nk) + q] = uri[9 * cell(p, i, j, nk - 1) + q];
+      }
+  for (int k = 1; k <= nk; k++)
+    for (int j = 1; j <= nj; j++)
+      for (int q : {6, 8}) {
+        uri[9 * cell(p, 1, j, k) + q] = uri[9 * cell(p, 2, j, k) + q];
+        uri[9 * cell(p, ni, j, k) + q] = uri[9 * cell(p, ni - 1, j, k) + q];
+      }
+  // beta normalisation: B *= sqrt(beta_act / beta), beta_act = min 2p/b^2
+  double bmin = 1.0e5;
+  for (size_t c = 0; c < N; c++) {
+    const double b2 = bsq_of(&uri[9 * c], &gcov[16 * c], &gcon[16 * c]);
+    const double pb = (b2 != 0.0) ? 2.0 * uri[9 * c + 4] / b2 : 1.0e5;
+    if (pb < bmin) bmin = pb;
+  }
+  const double bnorm = (t.beta == 0.0) ? 0.0 : std::sqrt(bmin / t.beta);
+  for (size_t c = 0; c < N; c++)
+    for (int q = 6; q <= 8; q++) uri[9 * c + q] = uri[9 * c + q] * bnorm;
+  return 0;
+}
+
+}  // extern "C"

@@ -1,0 +1,237 @@
+"""ctypes binding of libraishin_hip.so (include/raishin_hip.h, include/raishin_setup.h).
+
+Host-side mirror of the reference's time-step driver (main.f90:586-1151) for
+Python callers: build a model from a `module pram` configuration, set its
+state, advance it on the GPU, read it back.  Arrays are numpy arrays in the
+reference's Fortran layouts, e.g. uri.shape == (9, imax, jmax, kmax) order F.
+
+The library must be built (raishinsource_amd/_build.py); there is no CPU
+fallback: every entry point raises if the library or a GPU is missing.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+from .configs import Pram
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libraishin_hip.so")
+
+_lib = None
+
+
+class PramS(C.Structure):
+    _fields_ = ([(n, C.c_int) for n in ("imax", "jmax", "kmax", "irec", "irkt", "ict", "ihll", "icha",
+                                          "iwvec", "ieos", "model", "metric", "iter")]
+                + [(n, C.c_double) for n in ("cfl", "gam", "akm", "dmin", "pmin", "pmax")]
+                + [(n, C.c_int * 9) for n in ("iboux1in", "iboux1ot", "iboux2in", "iboux2ot",
+                                               "iboux3in", "iboux3ot")])
+
+
+class CountersS(C.Structure):
+    _fields_ = [(n, C.c_ulonglong) for n in (
+        "recov_fallback", "recov_fail", "recov_neg_rho", "recov_neg_p", "recov_vsq_gt1", "recov_det0",
+        "recov_wg_carry", "atm_floor", "pmm_scale", "pmm_floor", "guard")]
+
+
+class GridExtS(C.Structure):
+    _fields_ = [("ix1", C.c_int)] + [(n, C.c_double) for n in ("xmin", "xmax", "ymin", "ymax", "zmin", "zmax")]
+
+
+class TorusS(C.Structure):
+    _fields_ = [(n, C.c_double) for n in ("rin", "rmax", "beta", "acut", "noise")] + [("seed", C.c_ulonglong)]
+
+
+EXPORTS = (
+    "raishin_last_error", "raishin_clear_error", "raishin_model_create", "raishin_model_destroy",
+    "raishin_model_set_state", "raishin_model_get_state", "raishin_model_step", "raishin_model_cdtcfl4",
+    "raishin_model_counters", "raishin_model_reset_counters", "raishin_model_sync", "raishin_model_stream",
+    "raishin_cdtcfl4", "raishin_init", "raishin_grid", "raishin_coord1", "raishin_calchrist",
+    "raishin_mdtorus1",
+)
+
+
+class RaishinError(RuntimeError):
+    pass
+
+
+def lib():
+    """Load the in-tree library (no fallback)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RaishinError(f"{LIB_PATH} is not built; run raishinsource_amd/_build.py")
+        L = C.CDLL(LIB_PATH)
+        vp = C.c_void_p
+        L.raishin_model_create.restype = vp
+        L.raishin_model_create.argtypes = [C.POINTER(PramS)] + [vp] * 19
+        L.raishin_model_destroy.argtypes = [vp]
+        L.raishin_model_set_state.argtypes = [vp, vp, vp, C.c_double]
+        L.raishin_model_get_state.argtypes = [vp, vp, vp, C.POINTER(C.c_double)]
+        L.raishin_model_step.argtypes = [vp, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_double)]
+        L.raishin_model_cdtcfl4.argtypes = [vp] + [C.POINTER(C.c_double)] * 4
+        L.raishin_model_counters.argtypes = [vp, C.POINTER(CountersS)]
+        L.raishin_model_reset_counters.argtypes = [vp]
+        L.raishin_model_sync.argtypes = [vp]
+        L.raishin_model_stream.restype = vp
+        L.raishin_model_stream.argtypes = [vp]
+        L.raishin_last_error.argtypes = [C.c_char_p, C.c_int]
+        L.raishin_last_error.restype = C.c_int
+        L.raishin_grid.argtypes = [C.POINTER(PramS), C.POINTER(GridExtS)] + [vp] * 15
+        L.raishin_coord1.argtypes = [C.POINTER(PramS)] + [vp] * 18
+        L.raishin_calchrist.argtypes = [C.POINTER(PramS)] + [vp] * 4
+        L.raishin_mdtorus1.argtypes = [C.POINTER(PramS), C.POINTER(TorusS)] + [vp] * 7
+        L.raishin_mdtorus1.restype = C.c_int
+        _lib = L
+    return _lib
+
+
+def check():
+    buf = C.create_string_buffer(512)
+    if lib().raishin_last_error(buf, 512):
+        msg = buf.value.decode()
+        lib().raishin_clear_error()
+        raise RaishinError(msg)
+
+
+def pram_struct(p: Pram):
+    p = p.resolved()
+    s = PramS()
+    for f, _ in PramS._fields_:
+        v = getattr(p, f)
+        if isinstance(v, tuple):
+            getattr(s, f)[:] = v
+        else:
+            setattr(s, f, v)
+    return s
+
+
+def _p(a):
+    if a is None:
+        return None
+    assert a.dtype == np.float64 and (a.flags.f_contiguous or a.ndim == 1)
+    return a.ctypes.data
+
+
+def fzeros(*shape):
+    return np.zeros(shape, dtype=np.float64, order="F")
+
+
+def setup_grid(p: Pram):
+    """grid.f90:2 for the configuration's extents."""
+    p = p.resolved()
+    ni, nj, nk = p.imax, p.jmax, p.kmax
+    g = {}
+    for ax, n in (("1", ni), ("2", nj), ("3", nk)):
+        for nm in ("x%s", "dx%s", "x%sa", "x%sb"):
+            g[nm % ax] = np.zeros(n)
+        g["dx%sb" % ax] = np.zeros(n)  # n-1 used
+    ext = GridExtS(p.ix1, p.xmin, p.xmax, p.ymin, p.ymax, p.zmin, p.zmax)
+    ps = pram_struct(p)
+    lib().raishin_grid(C.byref(ps), C.byref(ext), *[_p(g[k]) for k in (
+        "x1", "x2", "x3", "dx1", "dx2", "dx3", "dx1b", "dx2b", "dx3b", "x1a", "x2a", "x3a", "x1b", "x2b", "x3b")])
+    return g
+
+
+def setup_metric(p: Pram, g):
+    """coord1 + caldetg (coord.f90:5,1016) and calchrist (coord.f90:1089)."""
+    p = p.resolved()
+    ni, nj, nk = p.imax, p.jmax, p.kmax
+    m = {nm: fzeros(4, 4, ni, nj, nk) for nm in ("gcov", "gcovi", "gcovj", "gcovk", "gcon", "gconi", "gconj", "gconk")}
+    for nm in ("detg", "detgi", "detgj", "detgk"):
+        m[nm] = fzeros(ni, nj, nk)
+    ps = pram_struct(p)
+    lib().raishin_coord1(C.byref(ps), *[_p(m[k]) for k in (
+        "gcov", "gcovi", "gcovj", "gcovk", "gcon", "gconi", "gconj", "gconk", "detg", "detgi", "detgj", "detgk")],
+        _p(g["x1"]), _p(g["x2"]), _p(g["x3"]), _p(g["x1a"]), _p(g["x2a"]), _p(g["x3a"]))
+    m["christ"] = fzeros(4, 4, 4, ni, nj, nk)
+    lib().raishin_calchrist(C.byref(ps), _p(m["christ"]), _p(g["x1"]), _p(g["x2"]), _p(g["x3"]))
+    return m
+
+
+def setup_torus(p: Pram, g, m, rin=6.0, rmax=12.0, beta=100.0, acut=0.2, noise=0.02, seed=0):
+    """Magnetised Fishbone-Moncrief torus (after mdgrmhd.f90:879 mdtorus1)."""
+    p = p.resolved()
+    uri = fzeros(9, p.imax, p.jmax, p.kmax)
+    t = TorusS(rin, rmax, beta, acut, noise, seed)
+    ps = pram_struct(p)
+    rc = lib().raishin_mdtorus1(C.byref(ps), C.byref(t), _p(uri), _p(m["gcov"]), _p(m["gcon"]), _p(m["detg"]),
+                                _p(g["x1"]), _p(g["x2"]), _p(g["x3"]))
+    if rc != 0:
+        raise RaishinError("torus set-up found no torus cells")
+    return uri
+
+
+class Model:
+    """One resident model on the GPU (raishin_model_* of include/raishin_hip.h)."""
+
+    def __init__(self, p: Pram, g, m):
+        self.p = p.resolved()
+        self._ps = pram_struct(self.p)
+        L = lib()
+        self.h = L.raishin_model_create(C.byref(self._ps), _p(g["x1"]), _p(g["x2"]), _p(g["x3"]), _p(g["dx1b"]),
+                                        _p(g["dx2b"]), _p(g["dx3b"]), _p(m["gcov"]), _p(m["gcon"]),
+                                        _p(m["gcovi"]), _p(m["gconi"]), _p(m["gcovj"]), _p(m["gconj"]),
+                                        _p(m["gcovk"]), _p(m["gconk"]), _p(m["detg"]), _p(m["detgi"]),
+                                        _p(m["detgj"]), _p(m["detgk"]), _p(m["christ"]))
+        if not self.h:
+            check()
+            raise RaishinError("raishin_model_create failed")
+
+    def set_state(self, uri, uu=None, time=0.0):
+        lib().raishin_model_set_state(self.h, _p(np.asfortranarray(uri)), _p(None if uu is None else np.asfortranarray(uu)), time)
+        check()
+
+    def get_state(self):
+        p = self.p
+        uri = fzeros(9, p.imax, p.jmax, p.kmax)
+        uu = fzeros(9, p.imax, p.jmax, p.kmax)
+        t = C.c_double(0.0)
+        lib().raishin_model_get_state(self.h, _p(uri), _p(uu), C.byref(t))
+        check()
+        return uri, uu, t.value
+
+    def step(self, n=1, sync=True):
+        if sync:
+            t, dt = C.c_double(0.0), C.c_double(0.0)
+            lib().raishin_model_step(self.h, n, C.byref(t), C.byref(dt))
+            check()
+            return t.value, dt.value
+        lib().raishin_model_step(self.h, n, None, None)
+        check()
+        return None
+
+    def cdtcfl4(self):
+        v = [C.c_double(0.0) for _ in range(4)]
+        lib().raishin_model_cdtcfl4(self.h, *[C.byref(x) for x in v])
+        check()
+        return tuple(x.value for x in v)
+
+    def counters(self):
+        c = CountersS()
+        lib().raishin_model_counters(self.h, C.byref(c))
+        check()
+        return {f: int(getattr(c, f)) for f, _ in CountersS._fields_}
+
+    def reset_counters(self):
+        lib().raishin_model_reset_counters(self.h)
+
+    def sync(self):
+        lib().raishin_model_sync(self.h)
+        check()
+
+    @property
+    def stream(self):
+        return lib().raishin_model_stream(self.h)
+
+    def close(self):
+        if self.h:
+            lib().raishin_model_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
