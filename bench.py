@@ -1,0 +1,203 @@
+#!/usr/bin/env python3
+"""Benchmark of the GRMHD time step on one MI355X.
+
+Workload: one GPU's 3D subdomain at its real shape -- a magnetised
+Fishbone-Moncrief torus in Kerr-Schild coordinates, 256 (r) x 128 (phi) x
+128 (theta) cells with ghosts, PPM reconstruction, HLLE, flux-CT, RK3, FP64
+throughout.  A step is one full time step (CFL reduction + three RK stages)
+over the whole subdomain with the state resident in HBM.  (With MP5 or WENO5
+the reference itself turns this torus to NaN within a few steps; --config c2
+runs MP5 anyway.)
+
+Prints ONE JSON line: zone-updates/s (interior zones x steps / wall time of
+the timed steps), the roofline of the dominant kernel from HIP-event timings
+taken on the model's stream during the timed steps, and the CPU baseline
+(the C restatement in oracle/, one core, on a phi-slab of the same workload).
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0  # MI355X HBM3E peak, GB/s
+
+
+def algorithmic_bytes(p, kind):
+    """Compulsory HBM bytes of one launch of a kernel kind (FP64 state).
+
+    sweep_*: read the 9 primitive planes once, write 9 flux planes, per
+             interface, plus the face metric table (33 doubles per (i,k)).
+    update:  read W, U (one RK input at stage 1, two after), 3x9 fluxes,
+             6 CT fluxes; write 9 recovered planes and W per cell; centre
+             metric + Christoffel tables (97 doubles per (i,k)).
+    post:    read 9 recovered planes, write 9 primitive + 9 conserved
+             planes per cell, centre metric table.
+    """
+    ni, nj, nk, nm0, nm1 = p.imax, p.jmax, p.kmax, p.nm0, p.nm1
+    P2 = ni * nk
+    if kind.startswith("sweep"):
+        units = (ni - 2 * nm0 + 1) * (nj - 2 * nm0 + 1) * (nk - 2 * nm0 + 1)
+        return units, 144.0 * units + 264.0 * P2, "interfaces"
+    if kind == "update":
+        units = (ni - 2 * nm1) * (nj - 2 * nm1) * (nk - 2 * nm1)
+        per = 72 + (72 if p.irkt == 2 else (72 + 2 * 144) / 3.0) + 216 + 48 + 72 + 8
+        return units, per * units + 776.0 * P2, "interior cells"
+    if kind == "post":
+        units = ni * nj * nk
+        return units, 216.0 * units + 264.0 * P2, "cells"
+    if kind == "fluxct":
+        units = (ni - 2 * nm1 + 1) * (nj - 2 * nm1 + 1) * (nk - 2 * nm1 + 1)
+        return units, 96.0 * units, "cells"
+    if kind == "cfl":
+        units = (ni - 2) * (nj - 2) * (nk - 2)
+        return units, 72.0 * units + 264.0 * P2, "cells"
+    return 0, 0.0, ""
+
+
+def cpu_baseline(p, phi_cells):
+    """The oracle (C restatement, one core) on a bounded sample of the same
+    workload: the same r-theta grid and torus with `phi_cells` interior phi
+    cells; one warm-up step, then one timed step.  Returns (zone-updates/s,
+    seconds timed, sample grid)."""
+    from dataclasses import replace
+
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    from harness import Oracle, fzeros  # noqa: E402
+    from raishinsource_amd import native
+
+    q = replace(p, jmax=phi_cells + 2 * p.nm1, ymax=None).resolved()
+    g = native.setup_grid(q)
+    m = native.setup_metric(q, g)
+    uri = native.setup_torus(q, g, m, seed=0, noise=0.02)
+    o = Oracle(q)
+    uu = fzeros(9, q.imax, q.jmax, q.kmax)
+    o.call("caluu1", uri, uu, m["gcov"], m["gcon"], m["detg"])
+    _, t = o.step(uu, uri, g, m, 0.0)
+    t0 = time.perf_counter()
+    o.step(uu, uri, g, m, t)
+    el = time.perf_counter() - t0
+    o.lib.orc_free_work()
+    nint = (q.imax - 2 * q.nm1) * (q.jmax - 2 * q.nm1) * (q.kmax - 2 * q.nm1)
+    return nint / el, el, q
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--config", default="c2ppm")
+    ap.add_argument("--cpu-phi", type=int, default=22,
+                    help="interior phi cells of the CPU-baseline sample (0: skip the CPU baseline)")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only (asked for {args.gpus}): every box here has one MI355X and "
+              "this path has no multi-GPU form; the shard design is described in DESIGN.md.", file=sys.stderr)
+        return 2
+
+    from raishinsource_amd import configs, native
+
+    p = configs.ALL[args.config].resolved()
+    t_setup = time.perf_counter()
+    g = native.setup_grid(p)
+    m = native.setup_metric(p, g)
+    uri = native.setup_torus(p, g, m, seed=0, noise=0.02)
+    model = native.Model(p, g, m)
+    model.set_state(uri)
+    t_setup = time.perf_counter() - t_setup
+    print(f"[bench] {args.config} {p.imax}x{p.jmax}x{p.kmax} set up in {t_setup:.1f}s", file=sys.stderr, flush=True)
+
+    for _ in range(args.warmup):
+        model.step(1, sync=False)
+    model.sync()
+    print(f"[bench] warmup {args.warmup} steps done", file=sys.stderr, flush=True)
+    model.reset_counters()
+    model.set_timing(True)
+    model.sync()
+    t0 = time.perf_counter()
+    model.step(args.steps, sync=False)
+    model.sync()
+    el = time.perf_counter() - t0
+    kt = model.get_timing()
+    print(f"[bench] timed {args.steps} steps: {el:.3f}s", file=sys.stderr, flush=True)
+    model.set_timing(False)
+    sim_time, dt = model.step(0)
+    counters = model.counters()
+
+    nint = (p.imax - 2 * p.nm1) * (p.jmax - 2 * p.nm1) * (p.kmax - 2 * p.nm1)
+    zups = nint * args.steps / el
+    # dominant kernel by total time in the timed steps
+    dom = max(kt, key=lambda k: kt[k][0])
+    ms_tot, launches = kt[dom]
+    avg_ms = ms_tot / max(launches, 1)
+    units, abytes, unit_name = algorithmic_bytes(p, dom)
+    achieved = abytes / (avg_ms * 1e-3) / 1e9
+    traffic = None
+    tfile = os.path.join(ROOT, "profiles", "traffic.json")
+    if os.path.exists(tfile):
+        with open(tfile) as f:
+            tr = json.load(f)
+        key = f"{args.config}:{dom}"
+        if key in tr:
+            traffic = tr[key]
+    kernel_ms = {k: round(v[0] / max(v[1], 1), 4) for k, v in kt.items()}
+
+    model.close()
+    cpu = None
+    if args.cpu_phi > 0:
+        print(f"[bench] cpu baseline: oracle on {args.cpu_phi} interior phi cells", file=sys.stderr, flush=True)
+        v, el_cpu, q = cpu_baseline(p, args.cpu_phi)
+        cpu = {"value": v, "unit": "zone-updates/s", "cores": 1, "kind": "port",
+               "sample": f"one step (after one warm-up step) of the same torus on {q.imax}x{q.jmax}x{q.kmax} "
+                         f"({args.cpu_phi} of the {p.jmax - 2 * p.nm1} interior phi cells), {el_cpu:.1f}s, "
+                         "oracle/raishin_oracle.c single-threaded"}
+
+    line = {
+        "metric": "zone-updates/s on 1 GPU, mixed vs FP64 at L1 err<=1e-5; %HBM peak; p99 latency",
+        "value": zups,
+        "unit": "zone-updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": el / args.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic: seeded Fishbone-Moncrief torus (2% pressure noise, seed 0)",
+        "config": {
+            "workload": f"3D magnetised Kerr-Schild torus {p.imax}x{p.jmax}x{p.kmax} (interior {nint} zones), "
+                        f"{'PPM' if p.irec == 5 else 'MP5' if p.irec == 6 else 'WENO5'}, HLLE, flux-CT, "
+                        f"RK{p.irkt}, FP64",
+            "name": args.config,
+            "interior_zones": nint,
+            "parallelism": "single GPU",
+        },
+        "roofline": {
+            "bound": "hbm",
+            "kernel": dom,
+            "achieved": achieved,
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS,
+            "traffic": traffic,
+            "algorithmic_bytes_per_launch": abytes,
+            "units_per_launch": units,
+            "unit_name": unit_name,
+            "avg_launch_ms": avg_ms,
+        },
+        "kernel_avg_ms": kernel_ms,
+        "cpu_baseline": cpu,
+        "sim_time": sim_time,
+        "counters": counters,
+    }
+    print(json.dumps(line))
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -86,6 +86,21 @@ void raishin_model_sync(raishin_model *m);
 /* Device address of the stream the model's kernels run on (hipStream_t). */
 void *raishin_model_stream(raishin_model *m);
 
+/* Per-kernel timing with HIP events on the model's stream.  Kinds, in order:
+ * 0..2 direction sweeps i,j,k; 3 flux-CT; 4 update (RK + recov1a);
+ * 5 in-order recovery fix-up; 6 ghosts/floors/caluu1; 7 CFL reduction.
+ * set_timing(1) clears the totals and starts recording; get_timing returns
+ * the number of kinds and fills total milliseconds and launch counts. */
+void raishin_model_set_timing(raishin_model *m, int on);
+
+/* One RK stage of a step (stage 1 includes the CFL reduction), exposing its
+ * intermediates in reference layouts when the pointers are non-NULL:
+ * ww(3,9,imax,jmax,kmax) as after ct() (main.f90:845) and the recovered
+ * primitives as after recov() (main.f90:861; interior cells).  Calling
+ * stages 1..irkt in turn is one raishin_model_step. */
+void raishin_model_debug_stage(raishin_model *m, int stage, double *ww, double *wpre);
+int raishin_model_get_timing(raishin_model *m, double *ms, long long *launches, int nkinds);
+
 /* ---- reference-named entry points (argument lists of the reference) ---- */
 /* cdtcfl4: cdtcfl.f90:2-3.  Host arrays in, host scalars out. */
 void raishin_cdtcfl4(const double *uri, const double *gcov, const double *gcon, double *dtcfl,

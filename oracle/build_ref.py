@@ -70,6 +70,13 @@ def configured_pram(p):
     return text
 
 
+def _run(cmd, cwd):
+    r = subprocess.run(cmd, cwd=cwd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
+    if r.returncode != 0:
+        sys.stderr.write(r.stdout)
+        raise RuntimeError("failed: " + " ".join(cmd))
+
+
 def build(name):
     p = ALL[name]
     out = os.path.join(HERE, "_ref", name)
@@ -80,11 +87,10 @@ def build(name):
     for src in SOURCES:
         path = os.path.join(out, "pram.f90") if src == "pram" else os.path.join(REF, src + ".f90")
         obj = os.path.join(out, src + ".o")
-        subprocess.run([FC, "-O2", "-fPIC", "-J", out, "-c", path, "-o", obj],
-                       check=True, cwd=out)
+        _run([FC, "-O2", "-fPIC", "-J", out, "-c", path, "-o", obj], out)
         objs.append(obj)
     lib = os.path.join(out, "libref.so")
-    subprocess.run([FC, "-shared", "-o", lib] + objs, check=True, cwd=out)
+    _run([FC, "-shared", "-o", lib] + objs, out)
     for o in objs:
         os.remove(o)
     return lib
@@ -93,5 +99,8 @@ def build(name):
 if __name__ == "__main__":
     if not os.path.isdir(REF):
         sys.exit(f"reference sources not found at {REF}")
-    for n in sys.argv[1:] or ["uppm", "ump5", "uweno", "t2d", "t3d", "c1"]:
-        print("built", build(n))
+    from concurrent.futures import ThreadPoolExecutor
+    names = sys.argv[1:] or ["uppm", "ump5", "uweno", "t2d", "t3d", "c1"]
+    with ThreadPoolExecutor(max_workers=min(8, len(names))) as ex:
+        for lib in ex.map(build, names):
+            print("built", lib)

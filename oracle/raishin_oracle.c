@@ -15,6 +15,11 @@
 
 static orc_pram P;
 static orc_counters CNT;
+/* optional record of the cells whose initial W is inherited (diagnostics) */
+static long long *DBG_CARRY;
+static long long DBG_CAP, DBG_N;
+void orc_debug_carry(long long *buf, long long cap) { DBG_CARRY = buf; DBG_CAP = cap; DBG_N = 0; }
+long long orc_debug_carry_count(void) { return DBG_N; }
 static int NI, NJ, NK;
 
 #define CELL(i, j, k) ((size_t)((i)-1) + (size_t)NI * ((size_t)((j)-1) + (size_t)NJ * (size_t)((k)-1)))
@@ -881,7 +886,11 @@ void orc_recov1a(double *uu, double *uri, const double *gcov, const double *gcon
         double wgm = (-t1b - sqrt(t1d)) * (1.0 / (2.0 * t1a));
         if (wgp > 0.0) wg = wgp;
         else if (wgm >= 0.0) wg = wgm;
-        else CNT.recov_wg_carry++;
+        else {
+          CNT.recov_wg_carry++;
+          if (DBG_CARRY && DBG_N < DBG_CAP) DBG_CARRY[DBG_N] = (long long)CELL(i, j, k);
+          DBG_N++;
+        }
         /* recov.f90:823-850: "goto 95" adjustment loops */
         for (int guard = 0; guard < 100000; guard++) {
           pp = gam2 * (wg * gf - dro * sqrt(gf));

@@ -97,6 +97,10 @@ U_PPM = Pram("uppm", 24, 12, 22, irec=5, irkt=2, cfl=0.4)
 U_MP5 = Pram("ump5", 20, 12, 18, irec=6, irkt=3, cfl=0.5)
 U_WENO = Pram("uweno", 20, 12, 18, irec=7, irkt=3, cfl=0.5)
 T2D = Pram("t2d", 64, 11, 64, irec=5, irkt=2, cfl=0.4)
+# golden-vector grids (tests/golden)
+G_PPM = Pram("gppm", 14, 11, 12, irec=5, irkt=2, cfl=0.4)
+G_MP5 = Pram("gmp5", 12, 9, 10, irec=6, irkt=3, cfl=0.5)
+G_WENO = Pram("gweno", 12, 9, 10, irec=7, irkt=3, cfl=0.5)
 T3D = Pram("t3d", 32, 16, 32, irec=6, irkt=3, cfl=0.5)
 
-ALL = {p.name: p for p in (C1, C2, C2_PPM, C2_WENO, U_PPM, U_MP5, U_WENO, T2D, T3D)}
+ALL = {p.name: p for p in (C1, C2, C2_PPM, C2_WENO, U_PPM, U_MP5, U_WENO, T2D, T3D, G_PPM, G_MP5, G_WENO)}

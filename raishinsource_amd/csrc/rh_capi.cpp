@@ -82,8 +82,17 @@ bool check_pram(const raishin_pram& p) {
 
 }  // namespace
 
+// Kernel kinds timed with HIP events when timing is on (raishin_model_set_timing).
+enum { K_SWEEP1 = 0, K_SWEEP2, K_SWEEP3, K_FLUXCT, K_UPDATE, K_CARRY, K_POST, K_CFL, K_NKINDS };
+
 struct raishin_model {
   raishin_pram p;
+  // per-kind HIP event pairs recorded around launches; resolved on read
+  bool timing = false;
+  std::vector<std::pair<hipEvent_t, hipEvent_t>> ev[K_NKINDS];
+  size_t ev_used[K_NKINDS] = {0};
+  double ev_ms[K_NKINDS] = {0};
+  long long ev_n[K_NKINDS] = {0};
   Geo G;
   hipStream_t st = nullptr;
   bool need_wo = false;
@@ -98,6 +107,9 @@ struct raishin_model {
   double *r15 = nullptr, *r25 = nullptr;
   int* bc = nullptr;
   unsigned long long *dtbits = nullptr, *cnt = nullptr, *carry = nullptr;
+  unsigned char *cflag = nullptr, *pend = nullptr;
+  long long* cfl_last = nullptr;
+  double* cfl_min = nullptr;
   double* tdt = nullptr;  // [0] time [1] last dt
 };
 
@@ -124,7 +136,13 @@ void dfree_all(raishin_model* m) {
   if (m->dtbits) (void)hipFree(m->dtbits);
   if (m->cnt) (void)hipFree(m->cnt);
   if (m->carry) (void)hipFree(m->carry);
+  if (m->cflag) (void)hipFree(m->cflag);
+  if (m->pend) (void)hipFree(m->pend);
+  if (m->cfl_last) (void)hipFree(m->cfl_last);
+  if (m->cfl_min) (void)hipFree(m->cfl_min);
+  m->cfl_last = nullptr; m->cfl_min = nullptr;
   m->bc = nullptr; m->dtbits = m->cnt = m->carry = nullptr;
+  m->cflag = m->pend = nullptr;
 }
 
 // (i,k) tables of one metric location: planes 0..15 gcov, 16..31 gcon, 32 detg.
@@ -150,6 +168,44 @@ bool metric_tables(const raishin_pram& p, const double* gcov, const double* gcon
         else if (std::memcmp(&d, &out[32 * P2 + q], 8)) { set_error("%s sqrt(-g) depends on j", what); return false; }
       }
   return true;
+}
+
+// Record an event pair around one launch of kind `kind` when timing is on.
+struct TimedLaunch {
+  raishin_model* m;
+  int kind;
+  hipEvent_t b = nullptr, e = nullptr;
+  TimedLaunch(raishin_model* m_, int k) : m(m_), kind(k) {
+    if (!m->timing) return;
+    auto& v = m->ev[kind];
+    if (m->ev_used[kind] == v.size()) {
+      hipEvent_t x, y;
+      if (hipEventCreate(&x) != hipSuccess || hipEventCreate(&y) != hipSuccess) return;
+      v.emplace_back(x, y);
+    }
+    b = v[m->ev_used[kind]].first;
+    e = v[m->ev_used[kind]].second;
+    m->ev_used[kind]++;
+    (void)hipEventRecord(b, m->st);
+  }
+  ~TimedLaunch() {
+    if (b) (void)hipEventRecord(e, m->st);
+  }
+};
+
+// Fold the recorded event pairs into per-kind totals (synchronises).
+void resolve_timing(raishin_model* m) {
+  (void)hipStreamSynchronize(m->st);
+  for (int k = 0; k < K_NKINDS; k++) {
+    for (size_t q = 0; q < m->ev_used[k]; q++) {
+      float ms = 0.f;
+      if (hipEventElapsedTime(&ms, m->ev[k][q].first, m->ev[k][q].second) == hipSuccess) {
+        m->ev_ms[k] += ms;
+        m->ev_n[k] += 1;
+      }
+    }
+    m->ev_used[k] = 0;
+  }
 }
 
 bool upload(double* dst, const double* src, size_t n, hipStream_t st) {
@@ -179,13 +235,21 @@ bool build_model(raishin_model* m, const raishin_pram* p, const double* x1, cons
       !dalloc(&m->Uh, 9 * N) || !dalloc(&m->Us, 9 * N) || !dalloc(&m->Unext, 9 * N) || !dalloc(&m->F, 27 * N) ||
       !dalloc(&m->CT, 6 * N) || !dalloc(&m->wgout, N) || !dalloc(&m->chr, 64 * G.P2) || !dalloc(&m->r15, G.ni) ||
       !dalloc(&m->r25, G.ni) || !dalloc(&m->bc, 54) || !dalloc(&m->dtbits, 4) || !dalloc(&m->cnt, NCOUNT) ||
-      !dalloc(&m->carry, 1 + CARRY_CAP) || !dalloc(&m->tdt, 2))
+      !dalloc(&m->carry, 1 + CARRY_CAP) || !dalloc(&m->tdt, 2) || !dalloc(&m->cflag, N) ||
+      !dalloc(&m->pend, CARRY_CAP) || !dalloc(&m->cfl_last, 4 * cfl_blocks(G)) ||
+      !dalloc(&m->cfl_min, 4 * cfl_blocks(G)))
     return false;
   if (m->need_wo && !dalloc(&m->Wo, 9 * N)) return false;
   HIPCHK(hipMemsetAsync(m->cnt, 0, NCOUNT * 8, m->st));
   HIPCHK(hipMemsetAsync(m->carry, 0, (1 + CARRY_CAP) * 8, m->st));
   HIPCHK(hipMemsetAsync(m->tdt, 0, 16, m->st));
   HIPCHK(hipMemsetAsync(m->W, 0, 9 * N * 8, m->st));
+  // the reference zeroes its flux array once (main.f90:372-381); an interface
+  // whose speeds are unordered (NaN) keeps its previous flux (hll.f90:86-115)
+  HIPCHK(hipMemsetAsync(m->F, 0, 27 * N * 8, m->st));
+  HIPCHK(hipMemsetAsync(m->CT, 0, 6 * N * 8, m->st));
+  HIPCHK(hipMemsetAsync(m->Wpre, 0, 9 * N * 8, m->st));
+  HIPCHK(hipMemsetAsync(m->cflag, 0, N, m->st));
   // metric tables
   const char* names[4] = {"centre", "i-face", "j-face", "k-face"};
   std::vector<double> tab;
@@ -240,34 +304,66 @@ bool build_model(raishin_model* m, const raishin_pram* p, const double* x1, cons
   return true;
 }
 
+struct StageTap {  // optional host copies of a stage's intermediates (reference layouts)
+  double *ww = nullptr, *wpre = nullptr;
+};
+bool copy_planes_out(raishin_model* m, const double* planes, int nplanes, int stride_out, int offset_out,
+                     double* host) {
+  // planes[n*N + c] -> host[offset_out + stride_out*(n + nplanes*c)] via a staging buffer
+  const long long N = m->G.N;
+  std::vector<double> tmp((size_t)nplanes * N);
+  HIPCHK(hipMemcpyAsync(tmp.data(), planes, (size_t)nplanes * N * 8, hipMemcpyDeviceToHost, m->st));
+  HIPCHK(hipStreamSynchronize(m->st));
+  for (long long c = 0; c < N; c++)
+    for (int n = 0; n < nplanes; n++) host[offset_out + (size_t)stride_out * (n + (size_t)nplanes * c)] = tmp[(size_t)n * N + c];
+  return true;
+}
+
 // One RK stage: sweeps -> flux-CT -> update (RK + recov) -> fix-up -> ghosts/floors/conserved.
-bool run_stage(raishin_model* m, int rkstage, const double* Ua, const double* Ub, double* Uout) {
+bool run_stage(raishin_model* m, int rkstage, const double* Ua, const double* Ub, double* Uout,
+               const StageTap* tap = nullptr) {
   const Geo& G = m->G;
   const long long N = G.N;
   double* F1 = m->F;
   double* F2 = m->F + 9 * N;
   double* F3 = m->F + 18 * N;
-  HIPCHK(launch_sweep(G, 1, m->p.irec, m->W, m->met[1], m->dxs[0], m->xs[0], m->p.gam, F1, m->st));
-  HIPCHK(launch_sweep(G, 2, m->p.irec, m->W, m->met[2], m->dxs[1], m->xs[1], m->p.gam, F2, m->st));
-  HIPCHK(launch_sweep(G, 3, m->p.irec, m->W, m->met[3], m->dxs[2], m->xs[2], m->p.gam, F3, m->st));
-  HIPCHK(launch_fluxct(G, F1, F2, F3, m->CT, m->st));
+  { TimedLaunch t(m, K_SWEEP1); HIPCHK(launch_sweep(G, 1, m->p.irec, m->W, m->met[1], m->dxs[0], m->xs[0], m->p.gam, F1, m->st)); }
+  { TimedLaunch t(m, K_SWEEP2); HIPCHK(launch_sweep(G, 2, m->p.irec, m->W, m->met[2], m->dxs[1], m->xs[1], m->p.gam, F2, m->st)); }
+  { TimedLaunch t(m, K_SWEEP3); HIPCHK(launch_sweep(G, 3, m->p.irec, m->W, m->met[3], m->dxs[2], m->xs[2], m->p.gam, F3, m->st)); }
+  { TimedLaunch t(m, K_FLUXCT); HIPCHK(launch_fluxct(G, F1, F2, F3, m->CT, m->st)); }
+  if (tap && tap->ww) {
+    // ww(d,n,i,j,k) as after ct(): CT'd transverse B fluxes inside [nm1, n-nm1]
+    for (int d = 0; d < 3; d++)
+      if (!copy_planes_out(m, m->F + (long long)d * 9 * N, 9, 3, d, tap->ww)) return false;
+    std::vector<double> ct((size_t)6 * N);
+    HIPCHK(hipMemcpy(ct.data(), m->CT, (size_t)6 * N * 8, hipMemcpyDeviceToHost));
+    const int slot[6][2] = {{1, 6}, {2, 6}, {0, 7}, {2, 7}, {0, 8}, {1, 8}};  // (d,n) 0-based of CT planes
+    for (int k = G.nm1; k <= G.nk - G.nm1; k++)
+      for (int j = G.nm1; j <= G.nj - G.nm1; j++)
+        for (int i = G.nm1; i <= G.ni - G.nm1; i++) {
+          const long long c = (i - 1) + (long long)G.ni * ((j - 1) + (long long)G.nj * (k - 1));
+          for (int q = 0; q < 6; q++) tap->ww[slot[q][0] + 3 * (slot[q][1] + 9 * c)] = ct[(size_t)q * N + c];
+        }
+  }
   StageArgs A{};
   A.W = m->W; A.Ua = Ua; A.Ub = Ub; A.F1 = F1; A.F2 = F2; A.F3 = F3; A.CT = m->CT;
   A.metC = m->met[0]; A.chr = m->chr; A.dx1b = m->dxb[0]; A.dx2b = m->dxb[1]; A.dx3b = m->dxb[2];
   A.r15 = m->r15; A.r25 = m->r25; A.dtbits = m->dtbits; A.Wpre = m->Wpre; A.wgout = m->wgout; A.Ukeep = Uout;
-  A.cnt = m->cnt; A.carry = m->carry; A.carry_cap = CARRY_CAP;
+  A.cnt = m->cnt; A.carry = m->carry; A.cflag = m->cflag; A.carry_cap = CARRY_CAP;
   A.gam = m->p.gam; A.cfl = m->p.cfl; A.dmin = m->p.dmin; A.pmin = m->p.pmin; A.iter = m->p.iter;
-  HIPCHK(launch_update(G, rkstage, A, m->st));
+  { TimedLaunch t(m, K_UPDATE); HIPCHK(launch_update(G, rkstage, A, m->st)); }
   FixArgs X{};
   X.W = m->W; X.Ustage = Uout; X.metC = m->met[0]; X.r15 = m->r15; X.r25 = m->r25; X.Wpre = m->Wpre;
-  X.wgout = m->wgout; X.carry = m->carry; X.cnt = m->cnt; X.carry_cap = CARRY_CAP; X.gam = m->p.gam;
+  X.wgout = m->wgout; X.carry = m->carry; X.cnt = m->cnt; X.cflag = m->cflag; X.pend = m->pend;
+  X.carry_cap = CARRY_CAP; X.gam = m->p.gam;
   X.dmin = m->p.dmin; X.pmin = m->p.pmin; X.iter = m->p.iter;
-  HIPCHK(launch_recov_carry(G, X, m->st));
+  { TimedLaunch t(m, K_CARRY); HIPCHK(launch_recov_carry(G, X, m->st)); }
+  if (tap && tap->wpre && !copy_planes_out(m, m->Wpre, 9, 1, 0, tap->wpre)) return false;
   PostArgs P{};
   P.Wpre = m->Wpre; P.Wold = m->W; P.Wo = m->need_wo ? m->Wo : m->W; P.metC = m->met[0]; P.r15 = m->r15;
   P.r25 = m->r25; P.bc = m->bc; P.Wout = m->Wb; P.Uout = Uout; P.cnt = m->cnt; P.gam = m->p.gam;
   P.dmin = m->p.dmin; P.pmin = m->p.pmin; P.pmax = m->p.pmax; P.model = m->p.model;
-  HIPCHK(launch_post(G, P, m->st));
+  { TimedLaunch t(m, K_POST); HIPCHK(launch_post(G, P, m->st)); }
   std::swap(m->W, m->Wb);
   return true;
 }
@@ -275,8 +371,8 @@ bool run_stage(raishin_model* m, int rkstage, const double* Ua, const double* Ub
 bool run_cfl(raishin_model* m) {
   CflArgs C{};
   C.W = m->W; C.metC = m->met[0]; C.dx1b = m->dxb[0]; C.dx2b = m->dxb[1]; C.dx3b = m->dxb[2];
-  C.dtbits = m->dtbits; C.gam = m->p.gam;
-  HIPCHK(launch_cfl(m->G, C, m->st));
+  C.dtbits = m->dtbits; C.blast = m->cfl_last; C.bmin = m->cfl_min; C.gam = m->p.gam;
+  { TimedLaunch t(m, K_CFL); HIPCHK(launch_cfl(m->G, C, m->st)); }
   return true;
 }
 
@@ -341,6 +437,8 @@ raishin_model* raishin_model_create(const raishin_pram* p, const double* x1, con
 void raishin_model_destroy(raishin_model* m) {
   if (!m) return;
   (void)hipStreamSynchronize(m->st);
+  for (int k = 0; k < K_NKINDS; k++)
+    for (auto& pr : m->ev[k]) { (void)hipEventDestroy(pr.first); (void)hipEventDestroy(pr.second); }
   dfree_all(m);
   (void)hipStreamDestroy(m->st);
   delete m;
@@ -442,6 +540,43 @@ void raishin_model_sync(raishin_model* m) {
 }
 
 void* raishin_model_stream(raishin_model* m) { return (void*)m->st; }
+
+void raishin_model_debug_stage(raishin_model* m, int stage, double* ww, double* wpre) {
+  // stage 1 also runs the CFL reduction and time advance of a step
+  StageTap tap;
+  tap.ww = ww;
+  tap.wpre = wpre;
+  auto go = [&]() -> bool {
+    if (stage == 1) {
+      if (!run_cfl(m)) return false;
+      HIPCHK(launch_advance_time(m->dtbits, m->p.cfl, m->tdt, m->st));
+      if (m->need_wo) HIPCHK(hipMemcpyAsync(m->Wo, m->W, 9 * m->G.N * 8, hipMemcpyDeviceToDevice, m->st));
+      return run_stage(m, 1, m->Ucur, nullptr, m->Uh, &tap);
+    }
+    if (stage == 2) {
+      if (m->p.irkt == 2) return run_stage(m, 2, m->Ucur, m->Uh, m->Unext, &tap);
+      return run_stage(m, 3, m->Ucur, m->Uh, m->Us, &tap);
+    }
+    return run_stage(m, 4, m->Ucur, m->Us, m->Unext, &tap);
+  };
+  if (go() && stage == m->p.irkt) std::swap(m->Ucur, m->Unext);
+}
+
+void raishin_model_set_timing(raishin_model* m, int on) {
+  resolve_timing(m);
+  for (int k = 0; k < K_NKINDS; k++) { m->ev_ms[k] = 0.0; m->ev_n[k] = 0; }
+  m->timing = on != 0;
+}
+
+int raishin_model_get_timing(raishin_model* m, double* ms, long long* launches, int nkinds) {
+  resolve_timing(m);
+  const int n = nkinds < K_NKINDS ? nkinds : K_NKINDS;
+  for (int k = 0; k < n; k++) {
+    if (ms) ms[k] = m->ev_ms[k];
+    if (launches) launches[k] = m->ev_n[k];
+  }
+  return K_NKINDS;
+}
 
 void raishin_init(const raishin_pram* p) {
   g_pram = *p;

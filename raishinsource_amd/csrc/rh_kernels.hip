@@ -218,7 +218,7 @@ __global__ void __launch_bounds__(256) k_sweep(Geo G, const double* __restrict__
       const double c1 = fmaxf_(fabs(cmax), fabs(cmin));
       o = 0.5 * (Fl[n] + Fr[n] - c1 * (Ur[n] - Ul[n]));
     } else if (cmax <= 0.0) o = Fr[n];
-    else continue;  // unordered speeds: hlle leaves ww as it was
+    else o = Fo[(long long)n * N];  // unordered (NaN) speeds: hlle keeps ww, then flushes it
     if (fabs(o) < 1.0e-40) o = 0.0;
     Fo[(long long)n * N] = o;
   }
@@ -258,7 +258,7 @@ __global__ void __launch_bounds__(256) k_fluxct(Geo G, const double* __restrict_
 // quadratic has no non-negative root is inherited from the previous cell in
 // the reference's loop order; that cell is flagged in *carry.
 // ------------------------------------------------------------------------
-struct RecovOut { double w[9]; int fallback, fail, negrho, negp, vsqgt1, det0, guard; };
+struct RecovOut { double w[9]; int fallback, fail, negrho, negp, vsqgt1, det0, guard, indep; };
 
 RH_DEV void recov_cell(const double u[9], const double wold[9], const Met& M, double gam, double dmin,
                        double pmin, int iter, double x1pow15, double x1pow25, double wg_in, bool& carry,
@@ -313,6 +313,9 @@ RH_DEV void recov_cell(const double u[9], const double wold[9], const Met& M, do
   const double wgm = (-t1b - sqrt(t1d)) * (1.0 / (2.0 * t1a));
   double wg = wg_in;
   carry = false;
+  // With a NaN among these the result and the final W are NaN whatever W the
+  // cell starts from, so an inherited guess cannot change it.
+  o.indep = isnan(dro) || isnan(qtsq) || isnan(qdotn) || isnan(qdotbsq) || isnan(bsq);
   if (wgp > 0.0) wg = wgp;
   else if (wgm >= 0.0) wg = wgm;
   else carry = true;
@@ -527,7 +530,8 @@ __global__ void __launch_bounds__(256) k_update(Geo G, StageArgs A) {
   RecovOut ro;
   bool carry;
   double wgout;
-  recov_cell(u, w, M, gam, A.dmin, A.pmin, A.iter, A.r15[i - 1], A.r25[i - 1], 0.0, carry, ro, wgout);
+  recov_cell(u, w, M, gam, A.dmin, A.pmin, A.iter, A.r15[i - 1], A.r25[i - 1], __builtin_nan(""), carry, ro, wgout);
+  const bool dep = carry && !ro.indep;
 #pragma unroll
   for (int n = 0; n < 9; n++) A.Wpre[(long long)n * N + c] = ro.w[n];
   A.wgout[c] = wgout;
@@ -538,11 +542,12 @@ __global__ void __launch_bounds__(256) k_update(Geo G, StageArgs A) {
   count_wave(A.cnt, C_VSQGT1, ro.vsqgt1);
   count_wave(A.cnt, C_DET0, ro.det0);
   count_wave(A.cnt, C_WGCARRY, carry);
-  count_wave(A.cnt, C_GUARD, ro.guard);
-  if (carry) {
-    // remember the cell and its RK result for the in-order fix-up pass
+  count_wave(A.cnt, C_GUARD, ro.guard && !dep);
+  if (dep) {
+    // remember the cell and its RK result for the in-order resolution
 #pragma unroll
     for (int n = 0; n < 9; n++) A.Ukeep[(long long)n * N + c] = u[n];
+    A.cflag[c] = 1;
     const unsigned long long slot = atomicAdd(&A.carry[0], 1ULL);
     if (slot < (unsigned long long)A.carry_cap) A.carry[1 + slot] = (unsigned long long)c;
   }
@@ -752,25 +757,42 @@ __global__ void __launch_bounds__(256) k_post(Geo G, PostArgs A) {
 }
 
 // ------------------------------------------------------------------------
-// CFL: cdtcfl4 (cdtcfl.f90:2-129) with calcha4 centre speeds; the minimum is
-// taken per wavefront with shuffles, then per block, then with one 64-bit
-// atomicMin on the bit pattern (positive doubles order like their bits).
+// CFL: cdtcfl4 (cdtcfl.f90:2-129) with calcha4 centre speeds.
+// The reference folds dtcfl = MIN(dtcfl, x) over the cells in (k,j,i) order,
+// and MIN(a,b) = a<b?a:b: a NaN element sets the running value to NaN and
+// the next element replaces it.  So the result is the minimum over the
+// elements after the last NaN (NaN if a NaN is the last element; 1.0 and all
+// elements if there is none).  Each block reduces its contiguous range to
+// (position of its last NaN, minimum after it) with wavefront shuffles; one
+// block then combines the block results in order.
+// Accumulators: 0 dtcfl (three elements per cell: dtx1a, dtx2a, dtx3a),
+// 1..3 dtx1..dtx3 (one element per cell).
 // ------------------------------------------------------------------------
 RH_DEV double wave_min(double v) {
 #pragma unroll
   for (int off = 32; off >= 1; off >>= 1) {
     const double o = __shfl_xor(v, off, 64);
-    v = fminf_(v, o);
+    v = (o < v) ? o : v;  // operands are never NaN here
+  }
+  return v;
+}
+RH_DEV long long wave_max_ll(long long v) {
+#pragma unroll
+  for (int off = 32; off >= 1; off >>= 1) {
+    const long long o = __shfl_xor(v, off, 64);
+    v = o > v ? o : v;
   }
   return v;
 }
 
 __global__ void __launch_bounds__(256) k_cfl(Geo G, CflArgs A) {
-  __shared__ double red[4][4];
+  __shared__ long long s_last[4][4];
+  __shared__ double s_min[4][4];
   const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   int i, j, k;
-  double d[4] = {1.0, 1.0, 1.0, 1.0};  // dtcfl, dtx1, dtx2, dtx3 identities (cdtcfl.f90:55-58)
-  if (box_index(t, 2, G.ni - 1, 2, G.nj - 1, 2, G.nk - 1, i, j, k)) {
+  double e[4] = {__builtin_inf(), __builtin_inf(), __builtin_inf(), 0.0};  // dtx1a, dtx2a, dtx3a
+  bool valid = box_index(t, 2, G.ni - 1, 2, G.nj - 1, 2, G.nk - 1, i, j, k);
+  if (valid) {
     const long long N = G.N, c = cidx(i, j, k, G);
     const Met M{A.metC, G.P2, q2(i, k, G)};
     double w[9];
@@ -783,39 +805,109 @@ __global__ void __launch_bounds__(256) k_cfl(Geo G, CflArgs A) {
     const int nmax = G.nm1 * 2 + (G.nm1 - 1);
     const int nd[4] = {0, G.ni, G.nj, G.nk};
     const double dxv[4] = {0.0, A.dx1b[i - 1], A.dx2b[j - 1], A.dx3b[k - 1]};
-    double dta[4];
 #pragma unroll
     for (int q = 1; q <= 3; q++) {
       double vf = fmaxf_(fmaxf_(0.0, fabs(vp[q])), fabs(vm[q]));
       if (nd[q] > nmax && vf > 0.0) {
         if (vf > 1.0) vf = 1.0;
-        dta[q] = fabs(dxv[q] / (vf + 1.0e-12));
+        e[q - 1] = fabs(dxv[q] / (vf + 1.0e-12));
       } else {
-        dta[q] = 1.0e10;
+        e[q - 1] = 1.0e10;
       }
     }
-    d[0] = fminf_(fminf_(fminf_(1.0, dta[1]), dta[2]), dta[3]);
-    d[1] = fminf_(1.0, dta[1]);
-    d[2] = fminf_(1.0, dta[2]);
-    d[3] = fminf_(1.0, dta[3]);
+  }
+  // per accumulator: this thread's last NaN position and minimum after it
+  long long last[4];
+  double mn[4];
+  {
+    long long l0 = -1;
+    double m0 = __builtin_inf();
+    if (valid)
+#pragma unroll
+      for (int q = 0; q < 3; q++) {
+        if (isnan(e[q])) { l0 = 3 * t + q; m0 = __builtin_inf(); }
+        else m0 = (e[q] < m0) ? e[q] : m0;
+      }
+    last[0] = l0; mn[0] = m0;
+#pragma unroll
+    for (int q = 0; q < 3; q++) {
+      const bool nan = valid && isnan(e[q]);
+      last[q + 1] = nan ? t : -1;
+      mn[q + 1] = (valid && !nan) ? e[q] : __builtin_inf();
+    }
   }
   const int lane = threadIdx.x % 64, wid = threadIdx.x / 64;
+  long long L[4];
 #pragma unroll
-  for (int q = 0; q < 4; q++) {
-    const double v = wave_min(d[q]);
-    if (lane == 0) red[wid][q] = v;
+  for (int a = 0; a < 4; a++) {
+    const long long v = wave_max_ll(last[a]);
+    if (lane == 0) s_last[wid][a] = v;
+  }
+  __syncthreads();
+#pragma unroll
+  for (int a = 0; a < 4; a++) {
+    long long v = s_last[0][a];
+    for (int w = 1; w < (int)(blockDim.x / 64); w++) v = s_last[w][a] > v ? s_last[w][a] : v;
+    L[a] = v;
+  }
+  // contribute the minimum over own elements after the block's last NaN
+#pragma unroll
+  for (int a = 0; a < 4; a++) {
+    double m = __builtin_inf();
+    if (valid) {
+      if (a == 0) {
+#pragma unroll
+        for (int q = 0; q < 3; q++)
+          if (3 * t + q > L[0] && !isnan(e[q])) m = (e[q] < m) ? e[q] : m;
+      } else if (t > L[a] && !isnan(e[a - 1])) {
+        m = e[a - 1];
+      }
+    }
+    const double v = wave_min(m);
+    if (lane == 0) s_min[wid][a] = v;
   }
   __syncthreads();
   if (threadIdx.x < 4) {
-    double v = red[0][threadIdx.x];
-    for (int w = 1; w < (int)(blockDim.x / 64); w++) v = fminf_(v, red[w][threadIdx.x]);
-    atomicMin(&A.dtbits[threadIdx.x], (unsigned long long)__double_as_longlong(v));
+    const int a = threadIdx.x;
+    double v = s_min[0][a];
+    for (int w = 1; w < (int)(blockDim.x / 64); w++) v = (s_min[w][a] < v) ? s_min[w][a] : v;
+    A.blast[4 * blockIdx.x + a] = L[a];
+    A.bmin[4 * blockIdx.x + a] = v;
   }
 }
 
-// Reset the CFL accumulators and advance nothing: stream-ordered, graph-safe.
-__global__ void k_cfl_reset(unsigned long long* dtbits) {
-  if (threadIdx.x < 4) dtbits[threadIdx.x] = (unsigned long long)__double_as_longlong(1.0);
+// Combine the per-block results in block order (one workgroup): the last NaN
+// lies in the last block that has one; that block's minimum after it and the
+// full minima of every later block make the result.
+__global__ void __launch_bounds__(1024) k_cfl_finish(CflArgs A, int nblocks) {
+  __shared__ long long s_l[16];
+  __shared__ double s_m[16];
+  const int lane = threadIdx.x % 64, wid = threadIdx.x / 64, nw = blockDim.x / 64;
+  for (int a = 0; a < 4; a++) {
+    long long bg = -1;
+    for (int b = threadIdx.x; b < nblocks; b += blockDim.x)
+      if (A.blast[4 * b + a] >= 0) bg = b > bg ? b : bg;
+    bg = wave_max_ll(bg);
+    if (lane == 0) s_l[wid] = bg;
+    __syncthreads();
+    long long B = s_l[0];
+    for (int w = 1; w < nw; w++) B = s_l[w] > B ? s_l[w] : B;
+    double m = __builtin_inf();
+    for (int b = threadIdx.x; b < nblocks; b += blockDim.x)
+      if (b >= B) { const double v = A.bmin[4 * b + a]; m = (v < m) ? v : m; }
+    m = wave_min(m);
+    if (lane == 0) s_m[wid] = m;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      double M = s_m[0];
+      for (int w = 1; w < nw; w++) M = (s_m[w] < M) ? s_m[w] : M;
+      double r;
+      if (B < 0) r = fminf_(1.0, M);                              // no NaN: MIN over 1.0 and all
+      else r = (M == __builtin_inf()) ? __builtin_nan("") : M;  // a trailing NaN stays
+      A.dtbits[a] = (unsigned long long)__double_as_longlong(r);
+    }
+    __syncthreads();
+  }
 }
 
 // time <- time + cfl*dtcfl ; last dt kept (main.f90:760-781)
@@ -827,44 +919,64 @@ __global__ void k_advance_time(const unsigned long long* dtbits, double cfl, dou
   }
 }
 
-// In-order fix-up of cells whose initial W is inherited from the previous cell
-// (recov.f90:815-817 leaves wg unchanged).  One thread walks the flagged cells
-// in the reference's (k,j,i) order; each is recomputed with the final W of its
-// predecessor, which the update kernel stored in wgout.
-__global__ void k_recov_carry(Geo G, FixArgs A) {
-  if (blockIdx.x != 0 || threadIdx.x != 0) return;
+// Resolution of cells whose initial W is inherited from the previous cell in
+// the reference's (k,j,i) loop order (recov.f90:815-817 leaves wg unchanged,
+// recov.f90:632 starts it at 0).  A flagged cell is computed once its
+// predecessor's final W is known: passes over the flagged list, each taking
+// the cells whose predecessor is resolved, until none is left.  Chains of
+// consecutive flagged cells take one pass per link.  One workgroup.
+__global__ void __launch_bounds__(1024) k_recov_carry(Geo G, FixArgs A) {
+  __shared__ int s_progress, s_left;
   unsigned long long n = A.carry[0];
   if (n == 0) return;
-  if (n > (unsigned long long)A.carry_cap) { A.cnt[C_GUARD] += 1; n = (unsigned long long)A.carry_cap; }
-  // insertion sort by cell index (flagged cells are rare)
-  unsigned long long* list = A.carry + 1;
-  for (unsigned long long a = 1; a < n; a++) {
-    const unsigned long long v = list[a];
-    long long b = (long long)a - 1;
-    while (b >= 0 && list[b] > v) { list[b + 1] = list[b]; b--; }
-    list[b + 1] = v;
+  if (n > (unsigned long long)A.carry_cap) {
+    if (threadIdx.x == 0) atomicAdd(&A.cnt[C_GUARD], 1ULL);
+    n = (unsigned long long)A.carry_cap;
   }
+  const unsigned long long* list = A.carry + 1;
+  unsigned char* pend = A.pend;
+  for (unsigned long long e = threadIdx.x; e < n; e += blockDim.x) pend[e] = 1;
+  __syncthreads();
   const long long N = G.N;
   const long long sj = G.ni, sk = (long long)G.ni * G.nj;
-  for (unsigned long long e = 0; e < n; e++) {
-    const long long c = (long long)list[e];
-    const int i = (int)(c % G.ni) + 1, j = (int)((c / sj) % G.nj) + 1, k = (int)(c / sk) + 1;
-    // predecessor in loop order over the interior box
-    int pi = i - 1, pj = j, pk = k;
-    if (pi < G.nm1 + 1) { pi = G.ni - G.nm1; pj = j - 1; if (pj < G.nm1 + 1) { pj = G.nj - G.nm1; pk = k - 1; } }
-    double wg_prev = 0.0;  // recov.f90:632 initialises wg once per call
-    if (pk >= G.nm1 + 1) wg_prev = A.wgout[cidx(pi, pj, pk, G)];
-    const Met M{A.metC, G.P2, q2(i, k, G)};
-    double u[9], w[9];
-    for (int m = 0; m < 9; m++) { u[m] = A.Ustage[(long long)m * N + c]; w[m] = A.W[(long long)m * N + c]; }
-    RecovOut ro;
-    bool carry;
-    double wgout;
-    recov_cell(u, w, M, A.gam, A.dmin, A.pmin, A.iter, A.r15[i - 1], A.r25[i - 1], wg_prev, carry, ro, wgout);
-    for (int m = 0; m < 9; m++) A.Wpre[(long long)m * N + c] = ro.w[m];
-    A.wgout[c] = wgout;
+  for (int pass = 0; pass < (1 << 22); pass++) {
+    if (threadIdx.x == 0) { s_progress = 0; s_left = 0; }
+    __syncthreads();
+    for (unsigned long long e = threadIdx.x; e < n; e += blockDim.x) {
+      if (pend[e] != 1) continue;
+      const long long c = (long long)list[e];
+      const int i = (int)(c % G.ni) + 1, j = (int)((c / sj) % G.nj) + 1, k = (int)(c / sk) + 1;
+      int pi = i - 1, pj = j, pk = k;
+      if (pi < G.nm1 + 1) { pi = G.ni - G.nm1; pj = j - 1; if (pj < G.nm1 + 1) { pj = G.nj - G.nm1; pk = k - 1; } }
+      double wg_prev = 0.0;
+      if (pk >= G.nm1 + 1) {
+        const long long pc = cidx(pi, pj, pk, G);
+        if (A.cflag[pc]) { s_left = 1; continue; }  // predecessor not resolved yet
+        wg_prev = A.wgout[pc];
+      }
+      const Met M{A.metC, G.P2, q2(i, k, G)};
+      double u[9], w[9];
+      for (int m = 0; m < 9; m++) { u[m] = A.Ustage[(long long)m * N + c]; w[m] = A.W[(long long)m * N + c]; }
+      RecovOut ro;
+      bool carry;
+      double wgout;
+      recov_cell(u, w, M, A.gam, A.dmin, A.pmin, A.iter, A.r15[i - 1], A.r25[i - 1], wg_prev, carry, ro, wgout);
+      for (int m = 0; m < 9; m++) A.Wpre[(long long)m * N + c] = ro.w[m];
+      A.wgout[c] = wgout;
+      if (ro.guard) atomicAdd(&A.cnt[C_GUARD], 1ULL);
+      pend[e] = 2;
+      s_progress = 1;
+    }
+    __syncthreads();
+    for (unsigned long long e = threadIdx.x; e < n; e += blockDim.x)
+      if (pend[e] == 2) { pend[e] = 0; A.cflag[list[e]] = 0; }
+    __syncthreads();
+    const bool stop = !s_left || !s_progress;  // no progress: nothing can resolve further
+    __syncthreads();                            // all have read before the next reset
+    if (stop) break;
   }
-  A.carry[0] = 0;
+  __syncthreads();
+  if (threadIdx.x == 0) A.carry[0] = 0;
 }
 
 // caluu1 (caluu.f90:2-126) on every cell: conserved variables from primitives.
@@ -939,7 +1051,7 @@ hipError_t launch_update(const Geo& G, int stage, const StageArgs& A, hipStream_
 }
 
 hipError_t launch_recov_carry(const Geo& G, const FixArgs& A, hipStream_t st) {
-  hipLaunchKernelGGL(k_recov_carry, dim3(1), dim3(64), 0, st, G, A);
+  hipLaunchKernelGGL(k_recov_carry, dim3(1), dim3(1024), 0, st, G, A);
   return hipGetLastError();
 }
 
@@ -948,10 +1060,15 @@ hipError_t launch_post(const Geo& G, const PostArgs& A, hipStream_t st) {
   return hipGetLastError();
 }
 
-hipError_t launch_cfl(const Geo& G, const CflArgs& A, hipStream_t st) {
-  hipLaunchKernelGGL(k_cfl_reset, dim3(1), dim3(64), 0, st, A.dtbits);
+long long cfl_blocks(const Geo& G) {
   const long long n = box(2, G.ni - 1) * box(2, G.nj - 1) * box(2, G.nk - 1);
-  if (n > 0) hipLaunchKernelGGL(k_cfl, dim3(nblk(n, 256)), dim3(256), 0, st, G, A);
+  return n > 0 ? (long long)nblk(n, 256) : 1;
+}
+
+hipError_t launch_cfl(const Geo& G, const CflArgs& A, hipStream_t st) {
+  const unsigned nb = (unsigned)cfl_blocks(G);
+  hipLaunchKernelGGL(k_cfl, dim3(nb), dim3(256), 0, st, G, A);
+  hipLaunchKernelGGL(k_cfl_finish, dim3(1), dim3(1024), 0, st, A, (int)nb);
   return hipGetLastError();
 }
 

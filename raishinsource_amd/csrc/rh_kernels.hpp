@@ -19,6 +19,7 @@ struct StageArgs {
   double *Ukeep;                   // RK result of flagged cells (for the fix-up)
   unsigned long long *cnt;
   unsigned long long *carry;       // [0] count, [1..cap] flagged cells
+  unsigned char *cflag;            // per cell: 1 while its inherited W is unresolved
   long long carry_cap;
   double gam, cfl, dmin, pmin;
   int iter;
@@ -36,7 +37,9 @@ struct PostArgs {
 
 struct CflArgs {
   const double *W, *metC, *dx1b, *dx2b, *dx3b;
-  unsigned long long *dtbits;      // [0] dtcfl [1..3] dt1..dt3 as ordered bit patterns
+  unsigned long long *dtbits;      // [0] dtcfl [1..3] dt1..dt3 (bit patterns of the doubles)
+  long long *blast;                // per block and accumulator: position of the last NaN
+  double *bmin;                    // per block and accumulator: minimum after it
   double gam;
 };
 
@@ -44,6 +47,7 @@ struct FixArgs {
   const double *W, *Ustage, *metC, *r15, *r25;
   double *Wpre, *wgout;
   unsigned long long *carry, *cnt;
+  unsigned char *cflag, *pend;
   long long carry_cap;
   double gam, dmin, pmin;
   int iter;
@@ -56,6 +60,7 @@ hipError_t launch_update(const Geo& G, int stage, const StageArgs& A, hipStream_
 hipError_t launch_recov_carry(const Geo& G, const FixArgs& A, hipStream_t st);
 hipError_t launch_post(const Geo& G, const PostArgs& A, hipStream_t st);
 hipError_t launch_cfl(const Geo& G, const CflArgs& A, hipStream_t st);
+long long cfl_blocks(const Geo& G);
 hipError_t launch_advance_time(const unsigned long long* dtbits, double cfl, double* tdt, hipStream_t st);
 hipError_t launch_caluu1(const Geo& G, const double* W, const double* metC, double gam, double* U, hipStream_t st);
 hipError_t launch_aos_to_soa(const double* aos, double* soa, long long N, hipStream_t st);

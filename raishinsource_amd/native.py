@@ -48,8 +48,11 @@ EXPORTS = (
     "raishin_model_set_state", "raishin_model_get_state", "raishin_model_step", "raishin_model_cdtcfl4",
     "raishin_model_counters", "raishin_model_reset_counters", "raishin_model_sync", "raishin_model_stream",
     "raishin_cdtcfl4", "raishin_init", "raishin_grid", "raishin_coord1", "raishin_calchrist",
-    "raishin_mdtorus1",
+    "raishin_mdtorus1", "raishin_model_set_timing", "raishin_model_get_timing",
+    "raishin_model_debug_stage",
 )
+
+KERNEL_KINDS = ("sweep_i", "sweep_j", "sweep_k", "fluxct", "update", "recov_fixup", "post", "cfl")
 
 
 class RaishinError(RuntimeError):
@@ -83,6 +86,10 @@ def lib():
         L.raishin_calchrist.argtypes = [C.POINTER(PramS)] + [vp] * 4
         L.raishin_mdtorus1.argtypes = [C.POINTER(PramS), C.POINTER(TorusS)] + [vp] * 7
         L.raishin_mdtorus1.restype = C.c_int
+        L.raishin_model_set_timing.argtypes = [vp, C.c_int]
+        L.raishin_model_get_timing.argtypes = [vp, C.POINTER(C.c_double), C.POINTER(C.c_longlong), C.c_int]
+        L.raishin_model_get_timing.restype = C.c_int
+        L.raishin_model_debug_stage.argtypes = [vp, C.c_int, vp, vp]
         _lib = L
     return _lib
 
@@ -220,6 +227,28 @@ class Model:
     def sync(self):
         lib().raishin_model_sync(self.h)
         check()
+
+    def stage(self, st, want_ww=False, want_wpre=False):
+        """One RK stage (1..irkt); returns (ww after ct, recovered primitives)."""
+        p = self.p
+        ww = fzeros(3, 9, p.imax, p.jmax, p.kmax) if want_ww else None
+        wp = fzeros(9, p.imax, p.jmax, p.kmax) if want_wpre else None
+        lib().raishin_model_debug_stage(self.h, st, _p(ww), _p(wp))
+        check()
+        return ww, wp
+
+    def set_timing(self, on=True):
+        lib().raishin_model_set_timing(self.h, 1 if on else 0)
+        check()
+
+    def get_timing(self):
+        """{kernel kind: (total ms, launches)} since set_timing(True)."""
+        n = len(KERNEL_KINDS)
+        ms = (C.c_double * n)()
+        cnt = (C.c_longlong * n)()
+        lib().raishin_model_get_timing(self.h, ms, cnt, n)
+        check()
+        return {k: (ms[q], int(cnt[q])) for q, k in enumerate(KERNEL_KINDS)}
 
     @property
     def stream(self):

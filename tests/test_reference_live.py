@@ -1,0 +1,78 @@
+"""Oracle and package set-up against the reference Fortran built in this
+container (oracle/_ref); skipped where the reference build is absent."""
+import numpy as np
+import pytest
+
+from harness import Oracle, RefStepper, fzeros, grid_and_metric, have_ref, synthetic_state
+from raishinsource_amd import configs, native
+
+
+def bits_equal(a, b):
+    return np.array_equal(np.ascontiguousarray(a).view(np.uint64), np.ascontiguousarray(b).view(np.uint64))
+
+
+def need(name):
+    if not have_ref(name):
+        pytest.skip(f"reference build oracle/_ref/{name} absent (python oracle/build_ref.py {name})")
+
+
+@pytest.mark.parametrize("name", ["uppm", "ump5", "t2d"])
+def test_setup_matches_reference(name):
+    """grid.f90:2, coord.f90:5 (+caldetg), coord.f90:1089 against raishin_grid/coord1/calchrist."""
+    need(name)
+    r, g, m = grid_and_metric(name)
+    p = r.p
+    g2 = native.setup_grid(p)
+    for k in ("x1", "x2", "x3", "dx1", "dx2", "dx3", "x1a", "x2a", "x3a", "x1b", "x2b", "x3b"):
+        assert bits_equal(g[k], g2[k]), k
+    for ax, n in (("1", p.imax), ("2", p.jmax), ("3", p.kmax)):
+        assert bits_equal(g["dx%sb" % ax][:n - 1], g2["dx%sb" % ax][:n - 1])
+    m2 = native.setup_metric(p, g2)
+    for k in m:
+        assert bits_equal(m[k], m2[k]), k
+
+
+@pytest.mark.parametrize("name", ["uppm", "ump5", "uweno"])
+def test_routines_match_reference(name):
+    need(name)
+    r, g, m = grid_and_metric(name)
+    p = r.p
+    o = Oracle(p)
+    uri = synthetic_state(p, 3)
+    B = r.bounds()
+    ni, nj, nk = p.imax, p.jmax, p.kmax
+    uu_r, uu_o = fzeros(9, ni, nj, nk), fzeros(9, ni, nj, nk)
+    r.call("caluu1", uri, uu_r, m["gcov"], m["gcon"], m["detg"], g["x1"], *B)
+    o.call("caluu1", uri, uu_o, m["gcov"], m["gcon"], m["detg"])
+    assert bits_equal(uu_r, uu_o)
+    uh = uu_r.copy(order="F")
+    n1 = p.nm1
+    uh[:, n1:ni - n1, n1:nj - n1, n1:nk - n1] *= 1.0 + 2e-3
+    wr, wo, ur, uo = uri.copy(order="F"), uri.copy(order="F"), uh.copy(order="F"), uh.copy(order="F")
+    r.call("recov", ur, wr, m["gcov"], m["gcon"], m["detg"], g["x1"], g["x2"], g["x3"], n1, *B)
+    o.call("recov1a", uo, wo, m["gcov"], m["gcon"], m["detg"], g["x1"], n1)
+    assert bits_equal(wr, wo) and bits_equal(ur, uo)
+
+
+@pytest.mark.parametrize("name,steps", [("t2d", 3), ("t3d", 2)])
+def test_step_matches_reference(name, steps):
+    """Composed step: reference routines in main.f90's order (bnd4/infchk from the
+    oracle, bnd.f90 is not buildable here) against orc_step; NaN-aware."""
+    need(name)
+    r, g, m = grid_and_metric(name)
+    p = r.p
+    uri = native.setup_torus(p, g, m)
+    o = Oracle(p)
+    uu = fzeros(9, p.imax, p.jmax, p.kmax)
+    o.call("caluu1", uri, uu, m["gcov"], m["gcon"], m["detg"])
+    A = [uu.copy(order="F"), uri.copy(order="F")]
+    Bs = [uu.copy(order="F"), uri.copy(order="F")]
+    rs = RefStepper(name, g, m)
+    t1 = t2 = 0.0
+    for _ in range(steps):
+        d1, t1 = rs.step(A[0], A[1], t1)
+        d2, t2 = o.step(Bs[0], Bs[1], g, m, t2)
+        assert d1 == d2
+    for q in (0, 1):
+        same = (A[q] == Bs[q]) | (np.isnan(A[q]) & np.isnan(Bs[q]))
+        assert same.all()
