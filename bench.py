@@ -91,7 +91,7 @@ def main():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--config", default="c2ppm")
-    ap.add_argument("--cpu-phi", type=int, default=22,
+    ap.add_argument("--cpu-phi", type=int, default=64,
                     help="interior phi cells of the CPU-baseline sample (0: skip the CPU baseline)")
     args = ap.parse_args()
     if args.gpus != 1:

@@ -892,7 +892,8 @@ void orc_recov1a(double *uu, double *uri, const double *gcov, const double *gcon
           DBG_N++;
         }
         /* recov.f90:823-850: "goto 95" adjustment loops */
-        for (int guard = 0; guard < 100000; guard++) {
+        int guard = 0;
+        for (; guard < 100000; guard++) {
           pp = gam2 * (wg * gf - dro * sqrt(gf));
           double t3a = (wg * wg * wg) * (wg + 2.0 * bsq) - qdotbsq * (2.0 * wg + bsq);
           double t3b = (wg * wg) * (qtsq - bsq * bsq);
@@ -901,6 +902,7 @@ void orc_recov1a(double *uu, double *uri, const double *gcov, const double *gcon
           if (t3c > 0.0) { wg = fabs((double)0.9f * wg); continue; }
           break;
         }
+        if (guard >= 100000) CNT.guard++;
         double wsq = wg * wg;
         double xsq = (bsq + wg) * (bsq + wg);
         vsq = fabs((wsq * qtsq + qdotbsq * (bsq + 2.0 * wg)) * (1.0 / (wsq * xsq)));
@@ -1213,13 +1215,15 @@ void orc_pminmax2b(double *uri, const double *gcov, const double *gcon, const do
         lower(bcon1, bcov1, g);
         double bsq = bcon1[1] * bcov1[1] + bcon1[2] * bcov1[2] + bcon1[3] * bcov1[3];
         int scaled = 0;
-        for (int guard = 0; guard < 100000; guard++) {
+        int guard = 0;
+        for (; guard < 100000; guard++) {
           double roh = de + (gam / (gam - 1.0)) * pr;
           double t1a = roh * gfl * gfl - pr;
           double t1b = fabs((double)0.05f * bsq);
           if (t1a <= t1b) { de = (double)1.05f * de; pr = (double)1.05f * pr; scaled = 1; continue; }
           break;
         }
+        if (guard >= 100000) CNT.guard++;
         if (scaled) CNT.pmm_scale++;
         if (de != w[0]) w[0] = de;
         else if (pr != w[4]) w[4] = pr;

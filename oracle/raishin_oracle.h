@@ -39,6 +39,8 @@ typedef struct {
   long long atm_floor;        /* atmosphere1a resets */
   long long pmm_scale;        /* pminmax2b 1.05 rescale loops entered */
   long long pmm_floor;        /* pminmax2b min/max/ratio clamps applied */
+  long long guard;            /* unbounded goto loops (recov.f90:823-850,
+                                 correction.f90 1.05 rescale) cut at 100000 trips */
 } orc_counters;
 
 void orc_set_pram(const orc_pram *p);

@@ -41,7 +41,6 @@ void set_error(const char* fmt, ...) {
   } while (0)
 
 constexpr int NCOUNT = 11;
-constexpr long long CARRY_CAP = 1 << 16;
 
 int ghost_nm0(int irec) {
   // main.f90:179-211
@@ -99,6 +98,7 @@ struct raishin_model {
   // state (9 planes each)
   double *W = nullptr, *Wb = nullptr, *Wpre = nullptr, *Wo = nullptr;
   double *Ucur = nullptr, *Uh = nullptr, *Us = nullptr, *Unext = nullptr;
+  // F: ww(3,9,...) (27 planes); CT: the six transverse B flux copies ct.f90 averages
   double *F = nullptr, *CT = nullptr, *wgout = nullptr, *stage_aos = nullptr;
   // geometry
   double *met[4] = {nullptr, nullptr, nullptr, nullptr};  // centre, i-face, j-face, k-face
@@ -108,6 +108,7 @@ struct raishin_model {
   int* bc = nullptr;
   unsigned long long *dtbits = nullptr, *cnt = nullptr, *carry = nullptr;
   unsigned char *cflag = nullptr, *pend = nullptr;
+  long long carry_cap = 0;  // room for every interior cell: the flagged list cannot overflow
   long long* cfl_last = nullptr;
   double* cfl_min = nullptr;
   double* tdt = nullptr;  // [0] time [1] last dt
@@ -225,6 +226,7 @@ bool build_model(raishin_model* m, const raishin_pram* p, const double* x1, cons
   G.P2 = (long long)G.ni * G.nk;
   const long long N = G.N;
   HIPCHK(hipStreamCreateWithFlags(&m->st, hipStreamNonBlocking));
+  m->carry_cap = (long long)(G.ni - 2 * G.nm1) * (G.nj - 2 * G.nm1) * (G.nk - 2 * G.nm1);
   for (int a = 0; a < 6; a++)
     for (int q = 0; q < 9; q++) {
       const int* b = a == 0 ? p->iboux1in : a == 1 ? p->iboux1ot : a == 2 ? p->iboux2in : a == 3 ? p->iboux2ot
@@ -235,13 +237,13 @@ bool build_model(raishin_model* m, const raishin_pram* p, const double* x1, cons
       !dalloc(&m->Uh, 9 * N) || !dalloc(&m->Us, 9 * N) || !dalloc(&m->Unext, 9 * N) || !dalloc(&m->F, 27 * N) ||
       !dalloc(&m->CT, 6 * N) || !dalloc(&m->wgout, N) || !dalloc(&m->chr, 64 * G.P2) || !dalloc(&m->r15, G.ni) ||
       !dalloc(&m->r25, G.ni) || !dalloc(&m->bc, 54) || !dalloc(&m->dtbits, 4) || !dalloc(&m->cnt, NCOUNT) ||
-      !dalloc(&m->carry, 1 + CARRY_CAP) || !dalloc(&m->tdt, 2) || !dalloc(&m->cflag, N) ||
-      !dalloc(&m->pend, CARRY_CAP) || !dalloc(&m->cfl_last, 4 * cfl_blocks(G)) ||
+      !dalloc(&m->carry, 1 + m->carry_cap) || !dalloc(&m->tdt, 2) || !dalloc(&m->cflag, N) ||
+      !dalloc(&m->pend, m->carry_cap) || !dalloc(&m->cfl_last, 4 * cfl_blocks(G)) ||
       !dalloc(&m->cfl_min, 4 * cfl_blocks(G)))
     return false;
   if (m->need_wo && !dalloc(&m->Wo, 9 * N)) return false;
   HIPCHK(hipMemsetAsync(m->cnt, 0, NCOUNT * 8, m->st));
-  HIPCHK(hipMemsetAsync(m->carry, 0, (1 + CARRY_CAP) * 8, m->st));
+  HIPCHK(hipMemsetAsync(m->carry, 0, (1 + m->carry_cap) * 8, m->st));
   HIPCHK(hipMemsetAsync(m->tdt, 0, 16, m->st));
   HIPCHK(hipMemsetAsync(m->W, 0, 9 * N * 8, m->st));
   // the reference zeroes its flux array once (main.f90:372-381); an interface
@@ -327,35 +329,26 @@ bool run_stage(raishin_model* m, int rkstage, const double* Ua, const double* Ub
   double* F1 = m->F;
   double* F2 = m->F + 9 * N;
   double* F3 = m->F + 18 * N;
-  { TimedLaunch t(m, K_SWEEP1); HIPCHK(launch_sweep(G, 1, m->p.irec, m->W, m->met[1], m->dxs[0], m->xs[0], m->p.gam, F1, m->st)); }
-  { TimedLaunch t(m, K_SWEEP2); HIPCHK(launch_sweep(G, 2, m->p.irec, m->W, m->met[2], m->dxs[1], m->xs[1], m->p.gam, F2, m->st)); }
-  { TimedLaunch t(m, K_SWEEP3); HIPCHK(launch_sweep(G, 3, m->p.irec, m->W, m->met[3], m->dxs[2], m->xs[2], m->p.gam, F3, m->st)); }
-  { TimedLaunch t(m, K_FLUXCT); HIPCHK(launch_fluxct(G, F1, F2, F3, m->CT, m->st)); }
+  { TimedLaunch t(m, K_SWEEP1); HIPCHK(launch_sweep(G, 1, m->p.irec, m->W, m->met[1], m->dxs[0], m->xs[0], m->p.gam, F1, m->CT, m->st)); }
+  { TimedLaunch t(m, K_SWEEP2); HIPCHK(launch_sweep(G, 2, m->p.irec, m->W, m->met[2], m->dxs[1], m->xs[1], m->p.gam, F2, m->CT, m->st)); }
+  { TimedLaunch t(m, K_SWEEP3); HIPCHK(launch_sweep(G, 3, m->p.irec, m->W, m->met[3], m->dxs[2], m->xs[2], m->p.gam, F3, m->CT, m->st)); }
+  { TimedLaunch t(m, K_FLUXCT); HIPCHK(launch_fluxct(G, m->CT, F1, F2, F3, m->st)); }
   if (tap && tap->ww) {
-    // ww(d,n,i,j,k) as after ct(): CT'd transverse B fluxes inside [nm1, n-nm1]
+    // ww(d,n,i,j,k) as after ct()
     for (int d = 0; d < 3; d++)
       if (!copy_planes_out(m, m->F + (long long)d * 9 * N, 9, 3, d, tap->ww)) return false;
-    std::vector<double> ct((size_t)6 * N);
-    HIPCHK(hipMemcpy(ct.data(), m->CT, (size_t)6 * N * 8, hipMemcpyDeviceToHost));
-    const int slot[6][2] = {{1, 6}, {2, 6}, {0, 7}, {2, 7}, {0, 8}, {1, 8}};  // (d,n) 0-based of CT planes
-    for (int k = G.nm1; k <= G.nk - G.nm1; k++)
-      for (int j = G.nm1; j <= G.nj - G.nm1; j++)
-        for (int i = G.nm1; i <= G.ni - G.nm1; i++) {
-          const long long c = (i - 1) + (long long)G.ni * ((j - 1) + (long long)G.nj * (k - 1));
-          for (int q = 0; q < 6; q++) tap->ww[slot[q][0] + 3 * (slot[q][1] + 9 * c)] = ct[(size_t)q * N + c];
-        }
   }
   StageArgs A{};
-  A.W = m->W; A.Ua = Ua; A.Ub = Ub; A.F1 = F1; A.F2 = F2; A.F3 = F3; A.CT = m->CT;
+  A.W = m->W; A.Ua = Ua; A.Ub = Ub; A.F1 = F1; A.F2 = F2; A.F3 = F3;
   A.metC = m->met[0]; A.chr = m->chr; A.dx1b = m->dxb[0]; A.dx2b = m->dxb[1]; A.dx3b = m->dxb[2];
   A.r15 = m->r15; A.r25 = m->r25; A.dtbits = m->dtbits; A.Wpre = m->Wpre; A.wgout = m->wgout; A.Ukeep = Uout;
-  A.cnt = m->cnt; A.carry = m->carry; A.cflag = m->cflag; A.carry_cap = CARRY_CAP;
+  A.cnt = m->cnt; A.carry = m->carry; A.cflag = m->cflag; A.carry_cap = m->carry_cap;
   A.gam = m->p.gam; A.cfl = m->p.cfl; A.dmin = m->p.dmin; A.pmin = m->p.pmin; A.iter = m->p.iter;
   { TimedLaunch t(m, K_UPDATE); HIPCHK(launch_update(G, rkstage, A, m->st)); }
   FixArgs X{};
   X.W = m->W; X.Ustage = Uout; X.metC = m->met[0]; X.r15 = m->r15; X.r25 = m->r25; X.Wpre = m->Wpre;
   X.wgout = m->wgout; X.carry = m->carry; X.cnt = m->cnt; X.cflag = m->cflag; X.pend = m->pend;
-  X.carry_cap = CARRY_CAP; X.gam = m->p.gam;
+  X.carry_cap = m->carry_cap; X.gam = m->p.gam;
   X.dmin = m->p.dmin; X.pmin = m->p.pmin; X.iter = m->p.iter;
   { TimedLaunch t(m, K_CARRY); HIPCHK(launch_recov_carry(G, X, m->st)); }
   if (tap && tap->wpre && !copy_planes_out(m, m->Wpre, 9, 1, 0, tap->wpre)) return false;
@@ -458,6 +451,9 @@ void raishin_model_set_state(raishin_model* m, const double* uri, const double* 
     }
     double tdt[2] = {time, 0.0};
     HIPCHK(hipMemcpyAsync(m->tdt, tdt, 16, hipMemcpyHostToDevice, m->st));
+    // a new run: ww starts from zero (main.f90:374-376); hlle keeps its old
+    // value wherever the wave speeds are NaN
+    HIPCHK(hipMemsetAsync(m->F, 0, (size_t)27 * N * 8, m->st));
     HIPCHK(hipStreamSynchronize(m->st));
     return true;
   };

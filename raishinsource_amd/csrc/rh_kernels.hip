@@ -152,10 +152,14 @@ RH_DEV void ppm_cell(const double* __restrict__ W, long long N, long long c, lon
 // rec2 (rec.f90:2-142) -> caluu3/calflx3* -> calcha (calcha.f90:2-126) -> hlle
 // (hll.f90:47-192), fused per interface.  One thread per interface c+1/2.
 // ------------------------------------------------------------------------
+// F holds ww(D,:,:,:,:) exactly as the reference's array: the last value that
+// hlle or ct wrote to each slot.  The two transverse B fluxes of direction D
+// also go to their plane of CTsrc (the copies ct.f90:54-67 takes before
+// averaging); inside the CT box fluxct then writes F itself.
 template <int D, int IREC>
 __global__ void __launch_bounds__(256) k_sweep(Geo G, const double* __restrict__ W, const double* __restrict__ metF,
                                                const double* __restrict__ dxs, const double* __restrict__ xs,
-                                               double gam, double* __restrict__ F) {
+                                               double gam, double* __restrict__ F, double* __restrict__ CTsrc) {
   int i, j, k;
   const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (!box_index(t, G.nm0, G.ni - G.nm0, G.nm0, G.nj - G.nm0, G.nm0, G.nk - G.nm0, i, j, k)) return;
@@ -210,6 +214,12 @@ __global__ void __launch_bounds__(256) k_sweep(Geo G, const double* __restrict__
   const double cmin = (D == 3) ? fminf_(fminf_(0.0, vpr), vml)   // calcha.f90:107
                                : fminf_(fminf_(0.0, vmr), vml);
   double* Fo = F + c;
+  // transverse B fluxes of this direction and their CTsrc planes
+  // (0 ww(2,7) 1 ww(3,7) 2 ww(1,8) 3 ww(3,8) 4 ww(1,9) 5 ww(2,9))
+  constexpr int nA = (D == 1) ? 7 : 6, nB = (D == 3) ? 7 : 8;
+  constexpr int pA = (D == 1) ? 2 : (D == 2) ? 0 : 1, pB = (D == 1) ? 4 : (D == 2) ? 5 : 3;
+  const bool in_ct = i >= G.nm1 && i <= G.ni - G.nm1 && j >= G.nm1 && j <= G.nj - G.nm1 && k >= G.nm1 &&
+                     k <= G.nk - G.nm1;
 #pragma unroll
   for (int n = 0; n < 9; n++) {
     double o;
@@ -220,35 +230,41 @@ __global__ void __launch_bounds__(256) k_sweep(Geo G, const double* __restrict__
     } else if (cmax <= 0.0) o = Fr[n];
     else o = Fo[(long long)n * N];  // unordered (NaN) speeds: hlle keeps ww, then flushes it
     if (fabs(o) < 1.0e-40) o = 0.0;
-    Fo[(long long)n * N] = o;
+    if (n == nA || n == nB) {
+      CTsrc[(long long)(n == nA ? pA : pB) * N + c] = o;
+      if (!in_ct) Fo[(long long)n * N] = o;  // inside the CT box fluxct writes this slot
+    } else {
+      Fo[(long long)n * N] = o;
+    }
   }
 }
 
 // ------------------------------------------------------------------------
-// Flux-CT (Toth 2000): fluxct, ct.f90:30-118.  Writes the six averaged
-// transverse B fluxes to CT planes: 0 ww(2,7) 1 ww(3,7) 2 ww(1,8) 3 ww(3,8)
-// 4 ww(1,9) 5 ww(2,9).
+// Flux-CT (Toth 2000): fluxct, ct.f90:30-118.  Reads the six transverse B
+// fluxes from CTsrc (planes 0 ww(2,7) 1 ww(3,7) 2 ww(1,8) 3 ww(3,8) 4 ww(1,9)
+// 5 ww(2,9), as the sweeps left them) and writes the averaged values into
+// the flux array in place, like ct.f90:98-109.
 // ------------------------------------------------------------------------
-__global__ void __launch_bounds__(256) k_fluxct(Geo G, const double* __restrict__ F1, const double* __restrict__ F2,
-                                                const double* __restrict__ F3, double* __restrict__ CT) {
+__global__ void __launch_bounds__(256) k_fluxct(Geo G, const double* __restrict__ CTsrc, double* __restrict__ F1,
+                                                double* __restrict__ F2, double* __restrict__ F3) {
   int i, j, k;
   const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (!box_index(t, G.nm1, G.ni - G.nm1, G.nm1, G.nj - G.nm1, G.nm1, G.nk - G.nm1, i, j, k)) return;
   const long long N = G.N, c = cidx(i, j, k, G);
   const long long si = 1, sj = G.ni, sk = (long long)G.ni * G.nj;
-  const double* fyx = F2 + 6 * N;  // ww(2,7)
-  const double* fzx = F3 + 6 * N;  // ww(3,7)
-  const double* fxy = F1 + 7 * N;  // ww(1,8)
-  const double* fzy = F3 + 7 * N;  // ww(3,8)
-  const double* fxz = F1 + 8 * N;  // ww(1,9)
-  const double* fyz = F2 + 8 * N;  // ww(2,9)
+  const double* fyx = CTsrc + 0 * N;  // ww(2,7)
+  const double* fzx = CTsrc + 1 * N;  // ww(3,7)
+  const double* fxy = CTsrc + 2 * N;  // ww(1,8)
+  const double* fzy = CTsrc + 3 * N;  // ww(3,8)
+  const double* fxz = CTsrc + 4 * N;  // ww(1,9)
+  const double* fyz = CTsrc + 5 * N;  // ww(2,9)
   const double c8 = 1.0 / 8.0;
-  CT[0 * N + c] = c8 * (2.0 * fyx[c] + fyx[c + si] + fyx[c - si] - fxy[c] - fxy[c + sj] - fxy[c - si] - fxy[c - si + sj]);
-  CT[1 * N + c] = c8 * (2.0 * fzx[c] + fzx[c + si] + fzx[c - si] - fxz[c] - fxz[c + sk] - fxz[c - si] - fxz[c - si + sk]);
-  CT[2 * N + c] = c8 * (2.0 * fxy[c] + fxy[c + sj] + fxy[c - sj] - fyx[c] - fyx[c + si] - fyx[c - sj] - fyx[c + si - sj]);
-  CT[3 * N + c] = c8 * (2.0 * fzy[c] + fzy[c + sj] + fzy[c - sj] - fyz[c] - fyz[c + sk] - fyz[c - sj] - fyz[c - sj + sk]);
-  CT[4 * N + c] = c8 * (2.0 * fxz[c] + fxz[c + sk] + fxz[c - sk] - fzx[c] - fzx[c + si] - fzx[c - sk] - fzx[c + si - sk]);
-  CT[5 * N + c] = c8 * (2.0 * fyz[c] + fyz[c + sk] + fyz[c - sk] - fzy[c] - fzy[c + sj] - fzy[c - sk] - fzy[c + sj - sk]);
+  F2[6 * N + c] = c8 * (2.0 * fyx[c] + fyx[c + si] + fyx[c - si] - fxy[c] - fxy[c + sj] - fxy[c - si] - fxy[c - si + sj]);
+  F3[6 * N + c] = c8 * (2.0 * fzx[c] + fzx[c + si] + fzx[c - si] - fxz[c] - fxz[c + sk] - fxz[c - si] - fxz[c - si + sk]);
+  F1[7 * N + c] = c8 * (2.0 * fxy[c] + fxy[c + sj] + fxy[c - sj] - fyx[c] - fyx[c + si] - fyx[c - sj] - fyx[c + si - sj]);
+  F3[7 * N + c] = c8 * (2.0 * fzy[c] + fzy[c + sj] + fzy[c - sj] - fyz[c] - fyz[c + sk] - fyz[c - sj] - fyz[c - sj + sk]);
+  F1[8 * N + c] = c8 * (2.0 * fxz[c] + fxz[c + sk] + fxz[c - sk] - fzx[c] - fzx[c + si] - fzx[c - sk] - fzx[c + si - sk]);
+  F2[8 * N + c] = c8 * (2.0 * fyz[c] + fyz[c + sk] + fyz[c - sk] - fzy[c] - fzy[c + sj] - fzy[c - sk] - fzy[c + sj - sk]);
 }
 
 // ------------------------------------------------------------------------
@@ -500,13 +516,7 @@ __global__ void __launch_bounds__(256) k_update(Geo G, StageArgs A) {
 #pragma unroll
   for (int m = 0; m < 9; m++) {
     const long long o = (long long)m * N + c;
-    double f1c = A.F1[o], f1m = A.F1[o - 1];
-    double f2c = A.F2[o], f2m = A.F2[o - sj];
-    double f3c = A.F3[o], f3m = A.F3[o - sk];
-    if (m == 7) { f1c = A.CT[2 * N + c]; f1m = A.CT[2 * N + c - 1]; f3c = A.CT[3 * N + c]; f3m = A.CT[3 * N + c - sk]; }
-    if (m == 8) { f1c = A.CT[4 * N + c]; f1m = A.CT[4 * N + c - 1]; f2c = A.CT[5 * N + c]; f2m = A.CT[5 * N + c - sj]; }
-    if (m == 6) { f2c = A.CT[0 * N + c]; f2m = A.CT[0 * N + c - sj]; f3c = A.CT[1 * N + c]; f3m = A.CT[1 * N + c - sk]; }
-    const double d1 = f1c - f1m, d2 = f2c - f2m, d3 = f3c - f3m;
+    const double d1 = A.F1[o] - A.F1[o - 1], d2 = A.F2[o] - A.F2[o - sj], d3 = A.F3[o] - A.F3[o - sk];
     double v;
     if (STAGE == 1) {
       v = A.Ua[o] - a1 * d1 - a2 * d2 - a3 * d3;
@@ -535,12 +545,13 @@ __global__ void __launch_bounds__(256) k_update(Geo G, StageArgs A) {
 #pragma unroll
   for (int n = 0; n < 9; n++) A.Wpre[(long long)n * N + c] = ro.w[n];
   A.wgout[c] = wgout;
-  count_wave(A.cnt, C_FALLBACK, ro.fallback);
-  count_wave(A.cnt, C_FAIL, ro.fail);
-  count_wave(A.cnt, C_NEGRHO, ro.negrho);
-  count_wave(A.cnt, C_NEGP, ro.negp);
-  count_wave(A.cnt, C_VSQGT1, ro.vsqgt1);
-  count_wave(A.cnt, C_DET0, ro.det0);
+  // a flagged cell's outcome is counted when the fix-up computes it
+  count_wave(A.cnt, C_FALLBACK, ro.fallback && !dep);
+  count_wave(A.cnt, C_FAIL, ro.fail && !dep);
+  count_wave(A.cnt, C_NEGRHO, ro.negrho && !dep);
+  count_wave(A.cnt, C_NEGP, ro.negp && !dep);
+  count_wave(A.cnt, C_VSQGT1, ro.vsqgt1 && !dep);
+  count_wave(A.cnt, C_DET0, ro.det0 && !dep);
   count_wave(A.cnt, C_WGCARRY, carry);
   count_wave(A.cnt, C_GUARD, ro.guard && !dep);
   if (dep) {
@@ -963,6 +974,12 @@ __global__ void __launch_bounds__(1024) k_recov_carry(Geo G, FixArgs A) {
       recov_cell(u, w, M, A.gam, A.dmin, A.pmin, A.iter, A.r15[i - 1], A.r25[i - 1], wg_prev, carry, ro, wgout);
       for (int m = 0; m < 9; m++) A.Wpre[(long long)m * N + c] = ro.w[m];
       A.wgout[c] = wgout;
+      if (ro.fallback) atomicAdd(&A.cnt[C_FALLBACK], 1ULL);
+      if (ro.fail) atomicAdd(&A.cnt[C_FAIL], 1ULL);
+      if (ro.negrho) atomicAdd(&A.cnt[C_NEGRHO], 1ULL);
+      if (ro.negp) atomicAdd(&A.cnt[C_NEGP], 1ULL);
+      if (ro.vsqgt1) atomicAdd(&A.cnt[C_VSQGT1], 1ULL);
+      if (ro.det0) atomicAdd(&A.cnt[C_DET0], 1ULL);
       if (ro.guard) atomicAdd(&A.cnt[C_GUARD], 1ULL);
       pend[e] = 2;
       s_progress = 1;
@@ -1020,11 +1037,11 @@ static inline unsigned nblk(long long n, int b) { return (unsigned)((n + b - 1) 
 static inline long long box(int lo, int hi) { return hi >= lo ? (long long)(hi - lo + 1) : 0; }
 
 hipError_t launch_sweep(const Geo& G, int dir, int irec, const double* W, const double* metF, const double* dxs,
-                        const double* xs, double gam, double* F, hipStream_t st) {
+                        const double* xs, double gam, double* F, double* CTsrc, hipStream_t st) {
   const long long n = box(G.nm0, G.ni - G.nm0) * box(G.nm0, G.nj - G.nm0) * box(G.nm0, G.nk - G.nm0);
   if (n == 0) return hipSuccess;
   const unsigned nb = nblk(n, 256);
-#define L(Dd, R) hipLaunchKernelGGL((k_sweep<Dd, R>), dim3(nb), dim3(256), 0, st, G, W, metF, dxs, xs, gam, F)
+#define L(Dd, R) hipLaunchKernelGGL((k_sweep<Dd, R>), dim3(nb), dim3(256), 0, st, G, W, metF, dxs, xs, gam, F, CTsrc)
   if (irec == 5) { if (dir == 1) L(1, 5); else if (dir == 2) L(2, 5); else L(3, 5); }
   else if (irec == 6) { if (dir == 1) L(1, 6); else if (dir == 2) L(2, 6); else L(3, 6); }
   else { if (dir == 1) L(1, 7); else if (dir == 2) L(2, 7); else L(3, 7); }
@@ -1032,10 +1049,10 @@ hipError_t launch_sweep(const Geo& G, int dir, int irec, const double* W, const 
   return hipGetLastError();
 }
 
-hipError_t launch_fluxct(const Geo& G, const double* F1, const double* F2, const double* F3, double* CT, hipStream_t st) {
+hipError_t launch_fluxct(const Geo& G, const double* CTsrc, double* F1, double* F2, double* F3, hipStream_t st) {
   const long long n = box(G.nm1, G.ni - G.nm1) * box(G.nm1, G.nj - G.nm1) * box(G.nm1, G.nk - G.nm1);
   if (n == 0) return hipSuccess;
-  hipLaunchKernelGGL(k_fluxct, dim3(nblk(n, 256)), dim3(256), 0, st, G, F1, F2, F3, CT);
+  hipLaunchKernelGGL(k_fluxct, dim3(nblk(n, 256)), dim3(256), 0, st, G, CTsrc, F1, F2, F3);
   return hipGetLastError();
 }
 

@@ -8,8 +8,7 @@ namespace rh {
 struct StageArgs {
   const double *W;                 // primitives at stage start (9 planes)
   const double *Ua, *Ub;           // RK inputs: stage1 uu; stage2 uo,uh; stage3 uo,us
-  const double *F1, *F2, *F3;      // HLLE fluxes per direction (9 planes each)
-  const double *CT;                // six CT'd transverse B fluxes
+  const double *F1, *F2, *F3;      // ww(1..3,:,...) after ct (9 planes each)
   const double *metC, *chr;        // centre metric tables, Christoffel tables
   const double *dx1b, *dx2b, *dx3b;
   const double *r15, *r25;         // x1(i)**(-3/2), x1(i)**(-5/2) from the host libm
@@ -54,8 +53,8 @@ struct FixArgs {
 };
 
 hipError_t launch_sweep(const Geo& G, int dir, int irec, const double* W, const double* metF, const double* dxs,
-                        const double* xs, double gam, double* F, hipStream_t st);
-hipError_t launch_fluxct(const Geo& G, const double* F1, const double* F2, const double* F3, double* CT, hipStream_t st);
+                        const double* xs, double gam, double* F, double* CTsrc, hipStream_t st);
+hipError_t launch_fluxct(const Geo& G, const double* CTsrc, double* F1, double* F2, double* F3, hipStream_t st);
 hipError_t launch_update(const Geo& G, int stage, const StageArgs& A, hipStream_t st);
 hipError_t launch_recov_carry(const Geo& G, const FixArgs& A, hipStream_t st);
 hipError_t launch_post(const Geo& G, const PostArgs& A, hipStream_t st);

@@ -104,7 +104,7 @@ class Oracle:
     class Counters(C.Structure):
         _fields_ = [(n, C.c_longlong) for n in (
             "recov_fallback", "recov_fail", "recov_neg_rho", "recov_neg_p", "recov_vsq_gt1",
-            "recov_det0", "recov_wg_carry", "atm_floor", "pmm_scale", "pmm_floor")]
+            "recov_det0", "recov_wg_carry", "atm_floor", "pmm_scale", "pmm_floor", "guard")]
 
     class State(C.Structure):
         _fields_ = [(n, C.c_void_p) for n in (
@@ -114,6 +114,10 @@ class Oracle:
     def __init__(self, p):
         self.lib = C.CDLL(ORACLE_SO)
         self.lib.orc_step.restype = C.c_double
+        # a new run: the step's persistent arrays (ww above all, which hlle
+        # keeps where the wave speeds are NaN) start from zero, as
+        # main.f90:374-376 sets them
+        self.lib.orc_free_work()
         self.set_pram(p)
 
     def set_pram(self, p):
