@@ -29,11 +29,17 @@ HBM_PEAK_GBS = 8000.0  # MI355X HBM3E peak, GB/s
 def algorithmic_bytes(p, kind):
     """Compulsory HBM bytes of one launch of a kernel kind (FP64 state).
 
-    sweep_*: read the 9 primitive planes once, write 9 flux planes, per
-             interface, plus the face metric table (33 doubles per (i,k)).
-    update:  read W, U (one RK input at stage 1, two after), 3x9 fluxes,
-             6 CT fluxes; write 9 recovered planes and W per cell; centre
-             metric + Christoffel tables (97 doubles per (i,k)).
+    sweep_*: per interface, read the two face states (PPM: 18 edge values
+             from ppm_edges; MP5/WENO5: the 9 primitive planes once) and
+             write 9 flux planes, plus the face metric table (33 doubles per
+             (i,k)).
+    ppm_edges: per cell and direction, read the 9 primitive planes, write
+             18 edge planes (all three directions per launch kind).
+    update:  read W, U (one RK input at stage 1, two after), 3x9 fluxes;
+             write the 9 RK-result planes; centre metric + Christoffel tables
+             (97 doubles per (i,k)).
+    recov:   read the RK result and W, write 9 recovered planes and the
+             final W of the Newton iteration; centre metric table.
     post:    read 9 recovered planes, write 9 primitive + 9 conserved
              planes per cell, centre metric table.
     """
@@ -41,11 +47,18 @@ def algorithmic_bytes(p, kind):
     P2 = ni * nk
     if kind.startswith("sweep"):
         units = (ni - 2 * nm0 + 1) * (nj - 2 * nm0 + 1) * (nk - 2 * nm0 + 1)
-        return units, 144.0 * units + 264.0 * P2, "interfaces"
+        per = 216.0 if p.irec == 5 else 144.0
+        return units, per * units + 264.0 * P2, "interfaces"
+    if kind == "ppm_edges":
+        units = (ni - 2 * nm0 + 2) * (nj - 2 * nm0 + 1) * (nk - 2 * nm0 + 1)  # one direction's cells
+        return units, 216.0 * units, "cells"
     if kind == "update":
         units = (ni - 2 * nm1) * (nj - 2 * nm1) * (nk - 2 * nm1)
-        per = 72 + (72 if p.irkt == 2 else (72 + 2 * 144) / 3.0) + 216 + 48 + 72 + 8
+        per = 72 + (72 if p.irkt == 2 else (72 + 2 * 144) / 3.0) + 216 + 72
         return units, per * units + 776.0 * P2, "interior cells"
+    if kind == "recov":
+        units = (ni - 2 * nm1) * (nj - 2 * nm1) * (nk - 2 * nm1)
+        return units, (72 + 72 + 72 + 8) * units + 264.0 * P2, "interior cells"
     if kind == "post":
         units = ni * nj * nk
         return units, 216.0 * units + 264.0 * P2, "cells"

@@ -87,8 +87,9 @@ void raishin_model_sync(raishin_model *m);
 void *raishin_model_stream(raishin_model *m);
 
 /* Per-kernel timing with HIP events on the model's stream.  Kinds, in order:
- * 0..2 direction sweeps i,j,k; 3 flux-CT; 4 update (RK + recov1a);
- * 5 in-order recovery fix-up; 6 ghosts/floors/caluu1; 7 CFL reduction.
+ * 0..2 direction sweeps i,j,k; 3 flux-CT; 4 RK update + sources;
+ * 5 in-order recovery fix-up; 6 ghosts/floors/caluu1; 7 CFL reduction;
+ * 8 PPM face values (irec=5, all three directions); 9 recov1a.
  * set_timing(1) clears the totals and starts recording; get_timing returns
  * the number of kinds and fills total milliseconds and launch counts. */
 void raishin_model_set_timing(raishin_model *m, int on);
@@ -100,15 +101,6 @@ void raishin_model_set_timing(raishin_model *m, int on);
  * stages 1..irkt in turn is one raishin_model_step. */
 void raishin_model_debug_stage(raishin_model *m, int stage, double *ww, double *wpre);
 int raishin_model_get_timing(raishin_model *m, double *ms, long long *launches, int nkinds);
-
-/* ---- reference-named entry points (argument lists of the reference) ---- */
-/* cdtcfl4: cdtcfl.f90:2-3.  Host arrays in, host scalars out. */
-void raishin_cdtcfl4(const double *uri, const double *gcov, const double *gcon, double *dtcfl,
-                     const double *dx1, const double *dx2, const double *dx3, double *dt1,
-                     double *dt2, double *dt3, const int *nm1, const int *is1, const int *ie1,
-                     const int *js1, const int *je1, const int *ks1, const int *ke1);
-/* Set the `module pram` values once (pram.f90), before any reference-named call. */
-void raishin_init(const raishin_pram *p);
 
 #ifdef __cplusplus
 }

@@ -21,7 +21,12 @@ HIP_FLAGS = ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}", "-ffp-contr
              "-Wno-unused-command-line-argument"]
 HOST_FLAGS = ["-O2", "-std=c++17", "-fPIC", "-ffp-contract=off", "-fno-fast-math"]
 
-UNITS = [("rh_kernels.hip", "hip"), ("rh_capi.cpp", "hip"), ("rh_setup.cpp", "host")]
+FC = os.environ.get("FC_FLANG", "/opt/rocm/lib/llvm/bin/flang")
+FORTRAN = ["raishin_c.f90", "raishin_wrappers.f90", "raishin_main.f90"]
+FORTRAN_MAIN = os.path.join(HERE, "fortran", "raishin_main")
+
+UNITS = [("rh_kernels.hip", "hip"), ("rh_model.cpp", "hip"), ("rh_capi.cpp", "hip"), ("rh_ref.cpp", "hip"),
+         ("rh_setup.cpp", "host")]
 
 
 def _run(cmd):
@@ -47,7 +52,23 @@ def build(verbose=False):
             print(out)
         objs.append(o)
     _run([HIPCC, "-shared", "-fPIC", f"--offload-arch={ARCH}", "-o", LIB] + objs)
+    build_fortran(verbose)
     return LIB
+
+
+def build_fortran(verbose=False):
+    """The iso_c_binding module, the reference-named wrappers and the Fortran
+    main loop, linked against the library (found next to it at run time)."""
+    fdir = os.path.join(HERE, "fortran")
+    fobjs = []
+    for src in FORTRAN:
+        o = os.path.join(OUT, src + ".o")
+        out = _run([FC, "-O2", "-J", OUT, "-c", os.path.join(fdir, src), "-o", o])
+        if verbose and out:
+            print(out)
+        fobjs.append(o)
+    _run([FC, "-o", FORTRAN_MAIN] + fobjs + ["-L" + HERE, "-lraishin_hip", "-Wl,-rpath,$ORIGIN/.."])
+    return FORTRAN_MAIN
 
 
 if __name__ == "__main__":

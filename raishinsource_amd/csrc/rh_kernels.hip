@@ -28,6 +28,27 @@ RH_DEV bool box_index(long long t, int ilo, int ihi, int jlo, int jhi, int klo, 
   return true;
 }
 
+// Workgroup b is dispatched to XCD b % 8.  Renumbering the workgroups so that
+// each XCD takes one contiguous eighth of the launch keeps neighbouring rows
+// of a stencil in the same L2.  A bijection on [0, gridDim.x).
+RH_DEV long long xcd_block() {
+  const unsigned nb = gridDim.x, b = blockIdx.x, x = b % 8u, idx = b / 8u, q = nb / 8u, r = nb % 8u;
+  return x < r ? (long long)x * (q + 1) + idx : (long long)r * (q + 1) + (long long)(x - r) * q + idx;
+}
+RH_DEV long long xcd_thread() { return xcd_block() * blockDim.x + threadIdx.x; }
+
+// As box_index with the j and k axes swapped in the ordering (i fastest, then
+// k, then j): consecutive workgroups step along k, the k sweep's stencil axis.
+RH_DEV bool box_index_ikj(long long t, int ilo, int ihi, int jlo, int jhi, int klo, int khi, int& i, int& j, int& k) {
+  const long long ci = ihi - ilo + 1, cj = jhi - jlo + 1, ck = khi - klo + 1;
+  if (ci <= 0 || cj <= 0 || ck <= 0 || t >= ci * cj * ck) return false;
+  i = ilo + (int)(t % ci);
+  const long long r = t / ci;
+  k = klo + (int)(r % ck);
+  j = jlo + (int)(r / ck);
+  return true;
+}
+
 // Counter slots (raishin_counters field order).
 enum { C_FALLBACK = 0, C_FAIL, C_NEGRHO, C_NEGP, C_VSQGT1, C_DET0, C_WGCARRY, C_ATM, C_PMMSCALE, C_PMMFLOOR,
        C_GUARD, C_NCOUNT };
@@ -41,20 +62,21 @@ RH_DEV void count_wave(unsigned long long* cnt, int slot, bool flag) {
 
 // ------------------------------------------------------------------------
 // PPM reconstruction of one cell (rec.f90:1261-2056), direction D.
-// Returns the cell's left edge L (uri*r(c-1)) and right edge R (uri*l(c)).
+// Hands each variable's left edge l (uri*r(c-1)) and right edge r (uri*l(c))
+// to store(n, l, r), one variable at a time.
 // p is the 1-based index of the cell along D; the loads are along stride s.
 // dxs[d] holds the dummy dx*b(d) of rec2 (= actual dx*b(d-1), rec.f90:58).
 // ------------------------------------------------------------------------
-template <int D>
+template <int D, class Store>
 RH_DEV void ppm_cell(const double* __restrict__ W, long long N, long long c, long long s, int p,
-                     const double* __restrict__ dxs, const double* __restrict__ xs, double gam,
-                     double L[9], double R[9]) {
+                     const double* __restrict__ dxs, const double* __restrict__ xs, const double* __restrict__ pc,
+                     long long npc, double gam, Store store) {
   const double dm2 = dxs[p - 2], dm1 = dxs[p - 1], d0 = dxs[p], dp1 = dxs[p + 1], dp2 = dxs[p + 2];
-  // interp coefficients for dma at q = p-1, p, p+1 (rec.f90:1353-1355)
-  auto dma_of = [&](double um, double u0, double up, double dl, double dc, double dr) -> double {
-    const double t1a = dc / (dl + dc + dr);
-    const double t1b = (2.0 * dl + dc) / (dr + dc);
-    const double t1c = (dc + 2.0 * dr) / (dl + dc);
+  // interp's width-only coefficients come from the table pc (ppm_coefficients
+  // on the host, same expressions and order): planes 0..2 of dma at q
+  // (rec.f90:1353-1355), planes 3..6 of equation (51) at q+1/2 (rec.f90:1427-1436)
+  auto dma_of = [&](double um, double u0, double up, int q) -> double {
+    const double t1a = pc[q], t1b = pc[npc + q], t1c = pc[2 * npc + q];
     const double sp1 = (up - u0);
     const double sm1 = (u0 - um);
     const double spm1 = sp1 * sm1;
@@ -65,16 +87,11 @@ RH_DEV void ppm_cell(const double* __restrict__ W, long long N, long long c, lon
     if (D == 3) return t1 * fminf_(fabs(da), t1d);   // rec.f90:1414
     return t1 * fminf_(fabs(da), 2.0 * t1d);
   };
-  // equation (51) at index q: value at q+1/2 (rec.f90:1427-1436)
-  auto eq51 = [&](double uq, double uq1, double dmq, double dmq1, double dA, double dB, double dC, double dD) -> double {
-    // dA..dD = dx(q-1), dx(q), dx(q+1), dx(q+2)
-    const double t2 = (dB / (dB + dC)) * (uq1 - uq);
-    const double t3a = 1.0 / (dA + dB + dC + dD);
-    const double t3b = (2.0 * dC * dB) / (dB + dC);
-    const double t3c = ((dA + dB) / (2.0 * dB + dC)) - ((dD + dC) / (2.0 * dC + dB));
-    const double t3 = t3a * t3b * t3c * (uq1 - uq);
-    const double t4 = t3a * dB * ((dA + dB) / (2.0 * dB + dC));
-    const double t5 = t3a * dC * ((dC + dD) / (dB + 2.0 * dC));
+  auto eq51 = [&](double uq, double uq1, double dmq, double dmq1, int q) -> double {
+    const double t2 = pc[3 * npc + q] * (uq1 - uq);
+    const double t3 = pc[4 * npc + q] * (uq1 - uq);
+    const double t4 = pc[5 * npc + q];
+    const double t5 = pc[6 * npc + q];
     return uq + t2 + t3 - t4 * dmq1 + t5 * dmq;
   };
 
@@ -123,15 +140,15 @@ RH_DEV void ppm_cell(const double* __restrict__ W, long long N, long long c, lon
     if (gam * 1.0 * t3 < t4) eta = 0.0;
   }
 
-#pragma unroll
+#pragma unroll 1
   for (int n = 0; n < 9; n++) {
     const double* Pn = W + (long long)n * N + c;
     const double um2 = Pn[-2 * s], um1 = Pn[-s], u0 = Pn[0], up1 = Pn[s], up2 = Pn[2 * s];
-    const double dmm1 = dma_of(um2, um1, u0, dxs[p - 2], dm1, d0);
-    const double dm0 = dma_of(um1, u0, up1, dm1, d0, dp1);
-    const double dmp1 = dma_of(u0, up1, up2, d0, dp1, dp2);
-    double r = eq51(u0, up1, dm0, dmp1, dm1, d0, dp1, dp2);     // value at p+1/2
-    double l = eq51(um1, u0, dmm1, dm0, dm2, dm1, d0, dp1);     // value at p-1/2
+    const double dmm1 = dma_of(um2, um1, u0, p - 1);
+    const double dm0 = dma_of(um1, u0, up1, p);
+    const double dmp1 = dma_of(u0, up1, up2, p + 1);
+    double r = eq51(u0, up1, dm0, dmp1, p);          // value at p+1/2
+    double l = eq51(um1, u0, dmm1, dm0, p - 1);      // value at p-1/2
     if (n == 0) {
       r = r * (1.0 - eta) + (up1 - 0.5 * dmp1) * eta;
       l = l * (1.0 - eta) + (um1 + 0.5 * dmm1) * eta;
@@ -143,8 +160,34 @@ RH_DEV void ppm_cell(const double* __restrict__ W, long long N, long long c, lon
     const double t4 = (r - l) * (r - l);
     if (t3 > t4) l = 3.0 * u0 - 2.0 * r;
     if (-t3 > t4) r = 3.0 * u0 - 2.0 * l;
-    L[n] = l; R[n] = r;
+    store(n, l, r);
   }
+}
+
+// ------------------------------------------------------------------------
+// PPM face values of every cell once (rec.f90:1261-2056 along D): planes 0..8
+// the left edge (uri*r(c-1)), 9..17 the right edge (uri*l(c)).  Cells
+// nm0..n-nm0+1 along D, the sweep's box across.
+// ------------------------------------------------------------------------
+template <int D>
+__global__ void __launch_bounds__(256) k_ppm_edges(Geo G, const double* __restrict__ W, const double* __restrict__ dxs,
+                                                   const double* __restrict__ xs, const double* __restrict__ pc,
+                                                   double gam, double* __restrict__ E) {
+  int i, j, k;
+  const long long t = xcd_thread();
+  const int hi = G.ni - G.nm0 + (D == 1), hj = G.nj - G.nm0 + (D == 2), hk = G.nk - G.nm0 + (D == 3);
+  const bool in = (D == 3) ? box_index_ikj(t, G.nm0, hi, G.nm0, hj, G.nm0, hk, i, j, k)
+                           : box_index(t, G.nm0, hi, G.nm0, hj, G.nm0, hk, i, j, k);
+  if (!in) return;
+  const long long N = G.N;
+  const long long s = (D == 1) ? 1LL : (D == 2) ? (long long)G.ni : (long long)G.ni * G.nj;
+  const long long c = cidx(i, j, k, G);
+  const int p = (D == 1) ? i : (D == 2) ? j : k;
+  const long long npc = ((D == 1) ? G.ni : (D == 2) ? G.nj : G.nk) + 2;
+  ppm_cell<D>(W, N, c, s, p, dxs, xs, pc, npc, gam, [&](int n, double l, double r) {
+    E[(long long)n * N + c] = l;
+    E[(long long)(9 + n) * N + c] = r;
+  });
 }
 
 // ------------------------------------------------------------------------
@@ -156,24 +199,25 @@ RH_DEV void ppm_cell(const double* __restrict__ W, long long N, long long c, lon
 // hlle or ct wrote to each slot.  The two transverse B fluxes of direction D
 // also go to their plane of CTsrc (the copies ct.f90:54-67 takes before
 // averaging); inside the CT box fluxct then writes F itself.
+//
+// PPM's face values come from k_ppm_edges (E); MP5 and WENO5 reconstruct here.
 template <int D, int IREC>
 __global__ void __launch_bounds__(256) k_sweep(Geo G, const double* __restrict__ W, const double* __restrict__ metF,
-                                               const double* __restrict__ dxs, const double* __restrict__ xs,
-                                               double gam, double* __restrict__ F, double* __restrict__ CTsrc) {
+                                               const double* __restrict__ E, double gam, double* __restrict__ F,
+                                               double* __restrict__ CTsrc) {
   int i, j, k;
-  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (!box_index(t, G.nm0, G.ni - G.nm0, G.nm0, G.nj - G.nm0, G.nm0, G.nk - G.nm0, i, j, k)) return;
+  const long long t = xcd_thread();
+  const bool in = (D == 3) ? box_index_ikj(t, G.nm0, G.ni - G.nm0, G.nm0, G.nj - G.nm0, G.nm0, G.nk - G.nm0, i, j, k)
+                           : box_index(t, G.nm0, G.ni - G.nm0, G.nm0, G.nj - G.nm0, G.nm0, G.nk - G.nm0, i, j, k);
+  if (!in) return;
   const long long N = G.N;
   const long long s = (D == 1) ? 1LL : (D == 2) ? (long long)G.ni : (long long)G.ni * G.nj;
   const long long c = cidx(i, j, k, G);
-  const int p = (D == 1) ? i : (D == 2) ? j : k;
   double wl[9], wr[9];
   if (IREC == 5) {
-    double Lc[9], Rc[9], Ln[9], Rn[9];
-    ppm_cell<D>(W, N, c, s, p, dxs, xs, gam, Lc, Rc);
-    ppm_cell<D>(W, N, c + s, s, p + 1, dxs, xs, gam, Ln, Rn);
+    // right edge of cell c (uri*l(c)) and left edge of cell c+1 (uri*r(c))
 #pragma unroll
-    for (int n = 0; n < 9; n++) { wl[n] = Rc[n]; wr[n] = Ln[n]; }
+    for (int n = 0; n < 9; n++) { wl[n] = E[(long long)(9 + n) * N + c]; wr[n] = E[(long long)n * N + c + s]; }
   } else {
 #pragma unroll
     for (int n = 0; n < 9; n++) {
@@ -248,7 +292,7 @@ __global__ void __launch_bounds__(256) k_sweep(Geo G, const double* __restrict__
 __global__ void __launch_bounds__(256) k_fluxct(Geo G, const double* __restrict__ CTsrc, double* __restrict__ F1,
                                                 double* __restrict__ F2, double* __restrict__ F3) {
   int i, j, k;
-  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long long t = xcd_thread();
   if (!box_index(t, G.nm1, G.ni - G.nm1, G.nm1, G.nj - G.nm1, G.nm1, G.nk - G.nm1, i, j, k)) return;
   const long long N = G.N, c = cidx(i, j, k, G);
   const long long si = 1, sj = G.ni, sk = (long long)G.ni * G.nj;
@@ -473,17 +517,23 @@ RH_DEV void recov_cell(const double u[9], const double wold[9], const Met& M, do
 // STAGE: 1 rk2fst, 2 rk2snd, 3 rk3snd, 4 rk3trd.
 // ------------------------------------------------------------------------
 template <int STAGE>
-__global__ void __launch_bounds__(256) k_update(Geo G, StageArgs A) {
+__global__ void __launch_bounds__(256) k_rk(Geo G, StageArgs A) {
   int i, j, k;
-  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long long t = xcd_thread();
   if (!box_index(t, G.nm1 + 1, G.ni - G.nm1, G.nm1 + 1, G.nj - G.nm1, G.nm1 + 1, G.nk - G.nm1, i, j, k)) return;
   const long long N = G.N, c = cidx(i, j, k, G);
   const long long sj = G.ni, sk = (long long)G.ni * G.nj;
   const double gam = A.gam;
   // dt and dt/dx (main.f90:760-780), recomputed per thread from the CFL minimum
-  const double dtcfl = __longlong_as_double((long long)A.dtbits[0]);
-  const double dt = A.cfl * dtcfl;
-  const double a1 = dt / A.dx1b[i - 1], a2 = dt / A.dx2b[j - 1], a3 = dt / A.dx3b[k - 1];
+  double dt, a1, a2, a3;
+  if (A.ak1) {  // akap*b and dt handed in by the caller (main.f90:760-780 on the host)
+    dt = A.dt_host;
+    a1 = A.ak1[i - 1]; a2 = A.ak2[j - 1]; a3 = A.ak3[k - 1];
+  } else {
+    const double dtcfl = __longlong_as_double((long long)A.dtbits[0]);
+    dt = A.cfl * dtcfl;
+    a1 = dt / A.dx1b[i - 1]; a2 = dt / A.dx2b[j - 1]; a3 = dt / A.dx3b[k - 1];
+  }
   const Met M{A.metC, G.P2, q2(i, k, G)};
 
   double w[9];
@@ -512,7 +562,6 @@ __global__ void __launch_bounds__(256) k_update(Geo G, StageArgs A) {
     sf[0] = dg * ta; sf[1] = dg * tb; sf[2] = dg * tc; sf[3] = dg * td;
   }
 
-  double u[9];
 #pragma unroll
   for (int m = 0; m < 9; m++) {
     const long long o = (long long)m * N + c;
@@ -534,13 +583,28 @@ __global__ void __launch_bounds__(256) k_update(Geo G, StageArgs A) {
       else if (STAGE == 3) v = v + sdt * 0.25;
       else v = v + sdt * (double)(2.0f / 3.0f);
     }
-    u[m] = v;
+    A.Ukeep[o] = v;
   }
+}
 
+// recov1a (recov.f90:578-1228) on every interior cell from the stage's RK
+// result; cells whose initial W is inherited are flagged for k_recov_carry.
+__global__ void __launch_bounds__(256) k_recov(Geo G, StageArgs A) {
+  int i, j, k;
+  const long long t = xcd_thread();
+  if (!box_index(t, G.nm1 + 1, G.ni - G.nm1, G.nm1 + 1, G.nj - G.nm1, G.nm1 + 1, G.nk - G.nm1, i, j, k)) return;
+  const long long N = G.N, c = cidx(i, j, k, G);
+  const Met M{A.metC, G.P2, q2(i, k, G)};
+  double w[9], u[9];
+#pragma unroll
+  for (int n = 0; n < 9; n++) {
+    w[n] = A.W[(long long)n * N + c];
+    u[n] = A.Ukeep[(long long)n * N + c];
+  }
   RecovOut ro;
   bool carry;
   double wgout;
-  recov_cell(u, w, M, gam, A.dmin, A.pmin, A.iter, A.r15[i - 1], A.r25[i - 1], __builtin_nan(""), carry, ro, wgout);
+  recov_cell(u, w, M, A.gam, A.dmin, A.pmin, A.iter, A.r15[i - 1], A.r25[i - 1], __builtin_nan(""), carry, ro, wgout);
   const bool dep = carry && !ro.indep;
 #pragma unroll
   for (int n = 0; n < 9; n++) A.Wpre[(long long)n * N + c] = ro.w[n];
@@ -555,9 +619,7 @@ __global__ void __launch_bounds__(256) k_update(Geo G, StageArgs A) {
   count_wave(A.cnt, C_WGCARRY, carry);
   count_wave(A.cnt, C_GUARD, ro.guard && !dep);
   if (dep) {
-    // remember the cell and its RK result for the in-order resolution
-#pragma unroll
-    for (int n = 0; n < 9; n++) A.Ukeep[(long long)n * N + c] = u[n];
+    // remember the cell for the in-order resolution
     A.cflag[c] = 1;
     const unsigned long long slot = atomicAdd(&A.carry[0], 1ULL);
     if (slot < (unsigned long long)A.carry_cap) A.carry[1 + slot] = (unsigned long long)c;
@@ -643,23 +705,65 @@ RH_DEV double bnd_value(const BndCtx& B, int m, int i, int j, int k) {
   return sgn * bnd_s0(B, m, c[1], c[2], c[3]);
 }
 
+// 1-based (i,j,k) of the t-th ghost cell: the two theta slabs (all i, j),
+// then the two phi slabs (interior k), then the two r slabs (interior j, k).
+RH_DEV bool ghost_index(long long t, const Geo& G, int& i, int& j, int& k) {
+  const long long ni = G.ni, nj = G.nj, nk = G.nk, g = G.nm1;
+  const long long nkS = 2 * g * ni * nj, njS = 2 * g * ni * (nk - 2 * g), niS = 2 * g * (nj - 2 * g) * (nk - 2 * g);
+  if (t < 0) return false;
+  if (t < nkS) {
+    i = (int)(t % ni) + 1;
+    const long long r = t / ni;
+    j = (int)(r % nj) + 1;
+    const long long kk = r / nj;
+    k = (int)(kk < g ? kk : nk - 2 * g + kk) + 1;
+    return true;
+  }
+  t -= nkS;
+  if (t < njS) {
+    i = (int)(t % ni) + 1;
+    const long long r = t / ni;
+    const long long jj = r % (2 * g);
+    j = (int)(jj < g ? jj : nj - 2 * g + jj) + 1;
+    k = (int)(g + r / (2 * g)) + 1;
+    return true;
+  }
+  t -= njS;
+  if (t < niS) {
+    const long long ii = t % (2 * g);
+    i = (int)(ii < g ? ii : ni - 2 * g + ii) + 1;
+    const long long r = t / (2 * g);
+    j = (int)(g + r % (nj - 2 * g)) + 1;
+    k = (int)(g + r / (nj - 2 * g)) + 1;
+    return true;
+  }
+  return false;
+}
+
+// Ghost fill of the stage: bnd4 composed per ghost cell, written into the
+// ghost slots of the recovered primitives.  bnd_value reads only interior
+// cells of that array (and the stage-start values of other ghosts), so the
+// writes cannot race with the reads.
+__global__ void __launch_bounds__(256) k_bnd(Geo G, PostArgs A) {
+  int i, j, k;
+  if (!ghost_index((long long)blockIdx.x * blockDim.x + threadIdx.x, G, i, j, k)) return;
+  const long long N = G.N, c = cidx(i, j, k, G);
+  const BndCtx B{&G, A.Wpre, A.Wold, A.Wo, A.bc};
+#pragma unroll 1
+  for (int n = 0; n < 9; n++) A.Wghost[(long long)n * N + c] = bnd_value(B, n, i, j, k);
+}
+
+// infchk + atmosphere1a + pminmax2b on every cell, after k_bnd; caluu1 follows (k_caluu1).
 __global__ void __launch_bounds__(256) k_post(Geo G, PostArgs A) {
-  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long long t = xcd_thread();
   int i, j, k;
   if (!box_index(t, 1, G.ni, 1, G.nj, 1, G.nk, i, j, k)) return;
   const long long N = G.N, c = cidx(i, j, k, G);
   const double gam = A.gam;
   const Met M{A.metC, G.P2, q2(i, k, G)};
   double w[9];
-  const bool ghost = is_ghost(i, G.ni, G.nm1) || is_ghost(j, G.nj, G.nm1) || is_ghost(k, G.nk, G.nm1);
-  if (!ghost) {
 #pragma unroll
-    for (int n = 0; n < 9; n++) w[n] = A.Wpre[(long long)n * N + c];
-  } else {
-    const BndCtx B{&G, A.Wpre, A.Wold, A.Wo, A.bc};
-#pragma unroll 1
-    for (int n = 0; n < 9; n++) w[n] = bnd_value(B, n, i, j, k);
-  }
+  for (int n = 0; n < 9; n++) w[n] = A.Wpre[(long long)n * N + c];
   // infchk, outer radial ghosts (bnd.f90:1740-1819); model 7..20
   if (A.model >= 7 && A.model <= 20 && i > G.ni - G.nm1) {
     const double gfl = calgfl(w[1], w[2], w[3], M);
@@ -756,15 +860,6 @@ __global__ void __launch_bounds__(256) k_post(Geo G, PostArgs A) {
   count_wave(A.cnt, C_PMMFLOOR, hit);
 #pragma unroll
   for (int n = 0; n < 9; n++) A.Wout[(long long)n * N + c] = w[n];
-  // caluu1: conserved variables of every cell
-  {
-    Prel s;
-    prelude(w, gam, M, s);
-    double u[9];
-    tmunu_row(s, 0, M.detg(), pow(s.de, gam), u);
-#pragma unroll
-    for (int n = 0; n < 9; n++) A.Uout[(long long)n * N + c] = u[n];
-  }
 }
 
 // ------------------------------------------------------------------------
@@ -1030,18 +1125,48 @@ __global__ void k_soa_to_aos(const double* __restrict__ soa, double* __restrict_
   aos[t] = soa[n * N + c];
 }
 
+// ww(3,9,i,j,k) of the reference (d fastest, then n) <-> planes d*9+n.
+__global__ void k_ww_to_planes(const double* __restrict__ ww, double* __restrict__ F, long long N) {
+  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= N) return;
+#pragma unroll
+  for (int d = 0; d < 3; d++)
+#pragma unroll
+    for (int n = 0; n < 9; n++) F[(long long)(d * 9 + n) * N + t] = ww[27 * t + d + 3 * n];
+}
+__global__ void k_planes_to_ww(const double* __restrict__ F, double* __restrict__ ww, long long N) {
+  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= N) return;
+#pragma unroll
+  for (int d = 0; d < 3; d++)
+#pragma unroll
+    for (int n = 0; n < 9; n++) ww[27 * t + d + 3 * n] = F[(long long)(d * 9 + n) * N + t];
+}
+
 // ------------------------------------------------------------------------
 // Launch helpers (host side)
 // ------------------------------------------------------------------------
 static inline unsigned nblk(long long n, int b) { return (unsigned)((n + b - 1) / b); }
 static inline long long box(int lo, int hi) { return hi >= lo ? (long long)(hi - lo + 1) : 0; }
 
-hipError_t launch_sweep(const Geo& G, int dir, int irec, const double* W, const double* metF, const double* dxs,
-                        const double* xs, double gam, double* F, double* CTsrc, hipStream_t st) {
+hipError_t launch_ppm_edges(const Geo& G, int dir, const double* W, const double* dxs, const double* xs,
+                            const double* pc, double gam, double* E, hipStream_t st) {
+  const long long n = box(G.nm0, G.ni - G.nm0 + (dir == 1)) * box(G.nm0, G.nj - G.nm0 + (dir == 2)) *
+                      box(G.nm0, G.nk - G.nm0 + (dir == 3));
+  if (n == 0) return hipSuccess;
+  const unsigned nb = nblk(n, 256);
+  if (dir == 1) hipLaunchKernelGGL((k_ppm_edges<1>), dim3(nb), dim3(256), 0, st, G, W, dxs, xs, pc, gam, E);
+  else if (dir == 2) hipLaunchKernelGGL((k_ppm_edges<2>), dim3(nb), dim3(256), 0, st, G, W, dxs, xs, pc, gam, E);
+  else hipLaunchKernelGGL((k_ppm_edges<3>), dim3(nb), dim3(256), 0, st, G, W, dxs, xs, pc, gam, E);
+  return hipGetLastError();
+}
+
+hipError_t launch_sweep(const Geo& G, int dir, int irec, const double* W, const double* metF, const double* E,
+                        double gam, double* F, double* CTsrc, hipStream_t st) {
   const long long n = box(G.nm0, G.ni - G.nm0) * box(G.nm0, G.nj - G.nm0) * box(G.nm0, G.nk - G.nm0);
   if (n == 0) return hipSuccess;
   const unsigned nb = nblk(n, 256);
-#define L(Dd, R) hipLaunchKernelGGL((k_sweep<Dd, R>), dim3(nb), dim3(256), 0, st, G, W, metF, dxs, xs, gam, F, CTsrc)
+#define L(Dd, R) hipLaunchKernelGGL((k_sweep<Dd, R>), dim3(nb), dim3(256), 0, st, G, W, metF, E, gam, F, CTsrc)
   if (irec == 5) { if (dir == 1) L(1, 5); else if (dir == 2) L(2, 5); else L(3, 5); }
   else if (irec == 6) { if (dir == 1) L(1, 6); else if (dir == 2) L(2, 6); else L(3, 6); }
   else { if (dir == 1) L(1, 7); else if (dir == 2) L(2, 7); else L(3, 7); }
@@ -1060,10 +1185,17 @@ hipError_t launch_update(const Geo& G, int stage, const StageArgs& A, hipStream_
   const long long n = box(G.nm1 + 1, G.ni - G.nm1) * box(G.nm1 + 1, G.nj - G.nm1) * box(G.nm1 + 1, G.nk - G.nm1);
   if (n == 0) return hipSuccess;
   const unsigned nb = nblk(n, 256);
-  if (stage == 1) hipLaunchKernelGGL(k_update<1>, dim3(nb), dim3(256), 0, st, G, A);
-  else if (stage == 2) hipLaunchKernelGGL(k_update<2>, dim3(nb), dim3(256), 0, st, G, A);
-  else if (stage == 3) hipLaunchKernelGGL(k_update<3>, dim3(nb), dim3(256), 0, st, G, A);
-  else hipLaunchKernelGGL(k_update<4>, dim3(nb), dim3(256), 0, st, G, A);
+  if (stage == 1) hipLaunchKernelGGL(k_rk<1>, dim3(nb), dim3(256), 0, st, G, A);
+  else if (stage == 2) hipLaunchKernelGGL(k_rk<2>, dim3(nb), dim3(256), 0, st, G, A);
+  else if (stage == 3) hipLaunchKernelGGL(k_rk<3>, dim3(nb), dim3(256), 0, st, G, A);
+  else hipLaunchKernelGGL(k_rk<4>, dim3(nb), dim3(256), 0, st, G, A);
+  return hipGetLastError();
+}
+
+hipError_t launch_recov(const Geo& G, const StageArgs& A, hipStream_t st) {
+  const long long n = box(G.nm1 + 1, G.ni - G.nm1) * box(G.nm1 + 1, G.nj - G.nm1) * box(G.nm1 + 1, G.nk - G.nm1);
+  if (n == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_recov, dim3(nblk(n, 256)), dim3(256), 0, st, G, A);
   return hipGetLastError();
 }
 
@@ -1073,7 +1205,11 @@ hipError_t launch_recov_carry(const Geo& G, const FixArgs& A, hipStream_t st) {
 }
 
 hipError_t launch_post(const Geo& G, const PostArgs& A, hipStream_t st) {
+  const long long g = G.nm1;
+  const long long ng = G.N - (long long)(G.ni - 2 * g) * (G.nj - 2 * g) * (G.nk - 2 * g);
+  if (ng > 0) hipLaunchKernelGGL(k_bnd, dim3(nblk(ng, 256)), dim3(256), 0, st, G, A);
   hipLaunchKernelGGL(k_post, dim3(nblk(G.N, 256)), dim3(256), 0, st, G, A);
+  hipLaunchKernelGGL(k_caluu1, dim3(nblk(G.N, 256)), dim3(256), 0, st, G, (const double*)A.Wout, A.metC, A.gam, A.Uout);
   return hipGetLastError();
 }
 
@@ -1105,6 +1241,17 @@ hipError_t launch_aos_to_soa(const double* aos, double* soa, long long N, hipStr
 }
 hipError_t launch_soa_to_aos(const double* soa, double* aos, long long N, hipStream_t st) {
   hipLaunchKernelGGL(k_soa_to_aos, dim3(nblk(N * 9, 256)), dim3(256), 0, st, soa, aos, N);
+  return hipGetLastError();
+}
+
+hipError_t launch_ww_to_planes(const double* ww, double* F, long long N, hipStream_t st) {
+  if (N == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_ww_to_planes, dim3(nblk(N, 256)), dim3(256), 0, st, ww, F, N);
+  return hipGetLastError();
+}
+hipError_t launch_planes_to_ww(const double* F, double* ww, long long N, hipStream_t st) {
+  if (N == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_planes_to_ww, dim3(nblk(N, 256)), dim3(256), 0, st, F, ww, N);
   return hipGetLastError();
 }
 

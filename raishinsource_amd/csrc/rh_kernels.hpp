@@ -13,9 +13,11 @@ struct StageArgs {
   const double *dx1b, *dx2b, *dx3b;
   const double *r15, *r25;         // x1(i)**(-3/2), x1(i)**(-5/2) from the host libm
   const unsigned long long *dtbits;
+  const double *ak1, *ak2, *ak3;   // dt/dx*b from the caller (else from dtbits)
+  double dt_host;
   double *Wpre;                    // recovered primitives (interior)
   double *wgout;                   // final W per cell (inherited initial guesses)
-  double *Ukeep;                   // RK result of flagged cells (for the fix-up)
+  double *Ukeep;                   // the stage's RK result (interior cells)
   unsigned long long *cnt;
   unsigned long long *carry;       // [0] count, [1..cap] flagged cells
   unsigned char *cflag;            // per cell: 1 while its inherited W is unresolved
@@ -26,6 +28,7 @@ struct StageArgs {
 
 struct PostArgs {
   const double *Wpre, *Wold, *Wo;
+  double *Wghost;                  // = Wpre: k_bnd writes its ghost slots
   const double *metC, *r15, *r25;
   const int *bc;                   // boundary codes [axis*2+side][9]
   double *Wout, *Uout;
@@ -52,10 +55,17 @@ struct FixArgs {
   int iter;
 };
 
-hipError_t launch_sweep(const Geo& G, int dir, int irec, const double* W, const double* metF, const double* dxs,
-                        const double* xs, double gam, double* F, double* CTsrc, hipStream_t st);
+// PPM face values of every cell along dir into E (18 planes: left edges, right edges).
+// pc: the width-only interp coefficients of the direction (7 planes of n+2, ppm_coefficients).
+hipError_t launch_ppm_edges(const Geo& G, int dir, const double* W, const double* dxs, const double* xs,
+                            const double* pc, double gam, double* E, hipStream_t st);
+// Fluxes at the interfaces along dir; E = the PPM edges (irec 5), unused otherwise.
+hipError_t launch_sweep(const Geo& G, int dir, int irec, const double* W, const double* metF, const double* E,
+                        double gam, double* F, double* CTsrc, hipStream_t st);
 hipError_t launch_fluxct(const Geo& G, const double* CTsrc, double* F1, double* F2, double* F3, hipStream_t st);
+// RK update + sources into A.Ukeep (every interior cell), then recov1a into A.Wpre.
 hipError_t launch_update(const Geo& G, int stage, const StageArgs& A, hipStream_t st);
+hipError_t launch_recov(const Geo& G, const StageArgs& A, hipStream_t st);
 hipError_t launch_recov_carry(const Geo& G, const FixArgs& A, hipStream_t st);
 hipError_t launch_post(const Geo& G, const PostArgs& A, hipStream_t st);
 hipError_t launch_cfl(const Geo& G, const CflArgs& A, hipStream_t st);
@@ -64,5 +74,7 @@ hipError_t launch_advance_time(const unsigned long long* dtbits, double cfl, dou
 hipError_t launch_caluu1(const Geo& G, const double* W, const double* metC, double gam, double* U, hipStream_t st);
 hipError_t launch_aos_to_soa(const double* aos, double* soa, long long N, hipStream_t st);
 hipError_t launch_soa_to_aos(const double* soa, double* aos, long long N, hipStream_t st);
+hipError_t launch_ww_to_planes(const double* ww, double* F, long long N, hipStream_t st);
+hipError_t launch_planes_to_ww(const double* F, double* ww, long long N, hipStream_t st);
 
 }  // namespace rh

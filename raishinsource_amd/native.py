@@ -52,7 +52,19 @@ EXPORTS = (
     "raishin_model_debug_stage",
 )
 
-KERNEL_KINDS = ("sweep_i", "sweep_j", "sweep_k", "fluxct", "update", "recov_fixup", "post", "cfl")
+# reference-named entry points (include/raishin_ref.h)
+REF_EXPORTS = (
+    "raishin_init", "raishin_geometry", "raishin_finalize", "raishin_push", "raishin_pull", "raishin_sync",
+    "raishin_get_counters", "raishin_reset_counters", "raishin_ident", "raishin_cdtcfl4", "raishin_rec2",
+    "raishin_calcha", "raishin_hll", "raishin_ct", "raishin_caltenr", "raishin_calsf", "raishin_rk2fst",
+    "raishin_rk2snd", "raishin_rk3snd", "raishin_rk3trd", "raishin_rk2adsff", "raishin_rk2adsfs",
+    "raishin_rk3adsfs", "raishin_rk3adsft", "raishin_recov", "raishin_bnd4", "raishin_infchk",
+    "raishin_atmosphere1a", "raishin_pminmax2b", "raishin_caluu1",
+)
+
+FORTRAN_MAIN = os.path.join(HERE, "fortran", "raishin_main")
+
+KERNEL_KINDS = ("sweep_i", "sweep_j", "sweep_k", "fluxct", "update", "recov_fixup", "post", "cfl", "ppm_edges", "recov")
 
 
 class RaishinError(RuntimeError):

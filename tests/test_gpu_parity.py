@@ -144,16 +144,23 @@ def test_cdtcfl4_bitwise(name, state, oracle_lib):
     got = np.array(mod.cdtcfl4())
     mod.close()
     assert bitdiff(got, want) == 0, (got, want)
-    # the stateless entry point with the reference's argument list
+    # the reference-named entry point (cdtcfl.f90:2 argument list)
     L = native.lib()
     ps = native.pram_struct(p)
+    vp = lambda a: C.c_void_p(a.ctypes.data)  # noqa: E731
     L.raishin_init(C.byref(ps))
+    L.raishin_geometry(*[vp(a) for a in (g["x1"], g["x2"], g["x3"], g["dx1b"], g["dx2b"], g["dx3b"], m["gcov"],
+                                         m["gcon"], m["gcovi"], m["gconi"], m["gcovj"], m["gconj"], m["gcovk"],
+                                         m["gconk"], m["detg"], m["detgi"], m["detgj"], m["detgk"], m["christ"])])
+    native.check()
     out = [C.c_double(0) for _ in range(4)]
     ints = [C.c_int(v) for v in (p.nm1, 1, p.imax, 1, p.jmax, 1, p.kmax)]
-    vp = lambda a: C.c_void_p(a.ctypes.data)  # noqa: E731
-    L.raishin_cdtcfl4(vp(uri), vp(m["gcov"]), vp(m["gcon"]), C.byref(out[0]), vp(g["dx1b"]), vp(g["dx2b"]),
-                      vp(g["dx3b"]), C.byref(out[1]), C.byref(out[2]), C.byref(out[3]), *[C.byref(x) for x in ints])
-    native.check()
+    try:
+        L.raishin_cdtcfl4(vp(uri), vp(m["gcov"]), vp(m["gcon"]), C.byref(out[0]), vp(g["dx1b"]), vp(g["dx2b"]),
+                          vp(g["dx3b"]), C.byref(out[1]), C.byref(out[2]), C.byref(out[3]), *[C.byref(x) for x in ints])
+        native.check()
+    finally:
+        L.raishin_finalize()
     assert bitdiff(np.array([x.value for x in out]), want) == 0
 
 
