@@ -132,7 +132,8 @@ void raishin_model_cdtcfl4(raishin_model* m, double* dtcfl, double* dt1, double*
 
 void raishin_model_counters(raishin_model* m, raishin_counters* c) {
   unsigned long long h[NCOUNT];
-  if (hipMemcpyAsync(h, m->cnt, sizeof h, hipMemcpyDeviceToHost, m->st) != hipSuccess ||
+  if (launch_sum_counters(m->cpart, m->crows, m->cnt, m->st) != hipSuccess ||
+      hipMemcpyAsync(h, m->cnt, sizeof h, hipMemcpyDeviceToHost, m->st) != hipSuccess ||
       hipStreamSynchronize(m->st) != hipSuccess) {
     set_error("raishin_model_counters: copy failed");
     return;
@@ -142,7 +143,9 @@ void raishin_model_counters(raishin_model* m, raishin_counters* c) {
 }
 
 void raishin_model_reset_counters(raishin_model* m) {
-  if (hipMemsetAsync(m->cnt, 0, NCOUNT * 8, m->st) != hipSuccess) set_error("reset counters failed");
+  if (hipMemsetAsync(m->cnt, 0, NCOUNT * 8, m->st) != hipSuccess ||
+      hipMemsetAsync(m->cpart, 0, (size_t)m->crows * NCOUNT * 8, m->st) != hipSuccess)
+    set_error("reset counters failed");
 }
 
 void raishin_model_sync(raishin_model* m) {

@@ -53,11 +53,47 @@ RH_DEV bool box_index_ikj(long long t, int ilo, int ihi, int jlo, int jhi, int k
 enum { C_FALLBACK = 0, C_FAIL, C_NEGRHO, C_NEGP, C_VSQGT1, C_DET0, C_WGCARRY, C_ATM, C_PMMSCALE, C_PMMFLOOR,
        C_GUARD, C_NCOUNT };
 
-RH_DEV void count_wave(unsigned long long* cnt, int slot, bool flag) {
-  // one atomic per wavefront: ballot the event, lane 0 adds the popcount
+// Event counts without atomics: each wavefront of a launch owns the row
+// cpart[w*C_NCOUNT ..] (w = its index in the launch); the lowest flagged lane
+// adds the wave's popcount.  Launches are stream-ordered, so a row has one
+// writer at a time; k_sum_counters folds the rows when counters are read.
+// (Device-scope atomics on a few addresses from every wave serialise across
+// the XCDs: they cost more than the kernels that raise the events.)
+RH_DEV void count_wave(unsigned long long* cpart, int slot, bool flag) {
   const unsigned long long b = __ballot(flag);
-  if (b && (threadIdx.x % warpSize) == (unsigned)(__ffsll((long long)b) - 1))
-    atomicAdd(&cnt[slot], (unsigned long long)__popcll(b));
+  if (b && (threadIdx.x % warpSize) == (unsigned)(__ffsll((long long)b) - 1)) {
+    const long long w = ((long long)blockIdx.x * blockDim.x + threadIdx.x) / warpSize;
+    cpart[w * C_NCOUNT + slot] += (unsigned long long)__popcll(b);
+  }
+}
+
+// cnt[s] += sum over rows of cpart[.][s], then the rows are cleared.  One
+// workgroup of 1024.
+__global__ void __launch_bounds__(1024) k_sum_counters(unsigned long long* __restrict__ cpart, long long rows,
+                                                       unsigned long long* __restrict__ cnt) {
+  __shared__ unsigned long long part[C_NCOUNT][1024 / 64];
+  unsigned long long acc[C_NCOUNT];
+#pragma unroll
+  for (int s = 0; s < C_NCOUNT; s++) acc[s] = 0;
+  for (long long r = threadIdx.x; r < rows; r += blockDim.x)
+#pragma unroll
+    for (int s = 0; s < C_NCOUNT; s++) {
+      acc[s] += cpart[r * C_NCOUNT + s];
+      cpart[r * C_NCOUNT + s] = 0;
+    }
+  const int lane = threadIdx.x % 64, wv = threadIdx.x / 64;
+#pragma unroll
+  for (int s = 0; s < C_NCOUNT; s++) {
+    unsigned long long v = acc[s];
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off);
+    if (lane == 0) part[s][wv] = v;
+  }
+  __syncthreads();
+  if (threadIdx.x < C_NCOUNT) {
+    unsigned long long t = 0;
+    for (int q = 0; q < (int)(blockDim.x / 64); q++) t += part[threadIdx.x][q];
+    cnt[threadIdx.x] += t;
+  }
 }
 
 // ------------------------------------------------------------------------
@@ -610,14 +646,14 @@ __global__ void __launch_bounds__(256) k_recov(Geo G, StageArgs A) {
   for (int n = 0; n < 9; n++) A.Wpre[(long long)n * N + c] = ro.w[n];
   A.wgout[c] = wgout;
   // a flagged cell's outcome is counted when the fix-up computes it
-  count_wave(A.cnt, C_FALLBACK, ro.fallback && !dep);
-  count_wave(A.cnt, C_FAIL, ro.fail && !dep);
-  count_wave(A.cnt, C_NEGRHO, ro.negrho && !dep);
-  count_wave(A.cnt, C_NEGP, ro.negp && !dep);
-  count_wave(A.cnt, C_VSQGT1, ro.vsqgt1 && !dep);
-  count_wave(A.cnt, C_DET0, ro.det0 && !dep);
-  count_wave(A.cnt, C_WGCARRY, carry);
-  count_wave(A.cnt, C_GUARD, ro.guard && !dep);
+  count_wave(A.cpart, C_FALLBACK, ro.fallback && !dep);
+  count_wave(A.cpart, C_FAIL, ro.fail && !dep);
+  count_wave(A.cpart, C_NEGRHO, ro.negrho && !dep);
+  count_wave(A.cpart, C_NEGP, ro.negp && !dep);
+  count_wave(A.cpart, C_VSQGT1, ro.vsqgt1 && !dep);
+  count_wave(A.cpart, C_DET0, ro.det0 && !dep);
+  count_wave(A.cpart, C_WGCARRY, carry);
+  count_wave(A.cpart, C_GUARD, ro.guard && !dep);
   if (dep) {
     // remember the cell for the in-order resolution
     A.cflag[c] = 1;
@@ -808,7 +844,7 @@ __global__ void __launch_bounds__(256) k_post(Geo G, PostArgs A) {
     }
     w[5] = 0.0;
   }
-  count_wave(A.cnt, C_ATM, atm);
+  count_wave(A.cpart, C_ATM, atm);
   // pminmax2b: correction.f90:808-961
   bool scaled = false, hit = false;
   {
@@ -856,8 +892,8 @@ __global__ void __launch_bounds__(256) k_post(Geo G, PostArgs A) {
     if (bbro > 50.0) { w[0] = s.bbsq / 50.0; hit = true; }
     if (bbpr > 2.0e3) { w[4] = s.bbsq / 2.0e3; hit = true; }
   }
-  count_wave(A.cnt, C_PMMSCALE, scaled);
-  count_wave(A.cnt, C_PMMFLOOR, hit);
+  count_wave(A.cpart, C_PMMSCALE, scaled);
+  count_wave(A.cpart, C_PMMFLOOR, hit);
 #pragma unroll
   for (int n = 0; n < 9; n++) A.Wout[(long long)n * N + c] = w[n];
 }
@@ -1210,6 +1246,11 @@ hipError_t launch_post(const Geo& G, const PostArgs& A, hipStream_t st) {
   if (ng > 0) hipLaunchKernelGGL(k_bnd, dim3(nblk(ng, 256)), dim3(256), 0, st, G, A);
   hipLaunchKernelGGL(k_post, dim3(nblk(G.N, 256)), dim3(256), 0, st, G, A);
   hipLaunchKernelGGL(k_caluu1, dim3(nblk(G.N, 256)), dim3(256), 0, st, G, (const double*)A.Wout, A.metC, A.gam, A.Uout);
+  return hipGetLastError();
+}
+
+hipError_t launch_sum_counters(unsigned long long* cpart, long long rows, unsigned long long* cnt, hipStream_t st) {
+  hipLaunchKernelGGL(k_sum_counters, dim3(1), dim3(1024), 0, st, cpart, rows, cnt);
   return hipGetLastError();
 }
 

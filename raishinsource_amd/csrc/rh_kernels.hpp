@@ -18,7 +18,7 @@ struct StageArgs {
   double *Wpre;                    // recovered primitives (interior)
   double *wgout;                   // final W per cell (inherited initial guesses)
   double *Ukeep;                   // the stage's RK result (interior cells)
-  unsigned long long *cnt;
+  unsigned long long *cnt, *cpart; // cpart: per-wavefront event rows (count_wave)
   unsigned long long *carry;       // [0] count, [1..cap] flagged cells
   unsigned char *cflag;            // per cell: 1 while its inherited W is unresolved
   long long carry_cap;
@@ -32,7 +32,7 @@ struct PostArgs {
   const double *metC, *r15, *r25;
   const int *bc;                   // boundary codes [axis*2+side][9]
   double *Wout, *Uout;
-  unsigned long long *cnt;
+  unsigned long long *cnt, *cpart;
   double gam, dmin, pmin, pmax;
   int model;
 };
@@ -70,6 +70,8 @@ hipError_t launch_recov_carry(const Geo& G, const FixArgs& A, hipStream_t st);
 hipError_t launch_post(const Geo& G, const PostArgs& A, hipStream_t st);
 hipError_t launch_cfl(const Geo& G, const CflArgs& A, hipStream_t st);
 long long cfl_blocks(const Geo& G);
+// Fold the per-wavefront event rows into cnt (and clear them).
+hipError_t launch_sum_counters(unsigned long long* cpart, long long rows, unsigned long long* cnt, hipStream_t st);
 hipError_t launch_advance_time(const unsigned long long* dtbits, double cfl, double* tdt, hipStream_t st);
 hipError_t launch_caluu1(const Geo& G, const double* W, const double* metC, double gam, double* U, hipStream_t st);
 hipError_t launch_aos_to_soa(const double* aos, double* soa, long long N, hipStream_t st);

@@ -82,6 +82,8 @@ void dfree_all(raishin_model* m) {
   if (m->bc) (void)hipFree(m->bc);
   if (m->dtbits) (void)hipFree(m->dtbits);
   if (m->cnt) (void)hipFree(m->cnt);
+  if (m->cpart) (void)hipFree(m->cpart);
+  m->cpart = nullptr;
   if (m->carry) (void)hipFree(m->carry);
   if (m->cflag) (void)hipFree(m->cflag);
   if (m->pend) (void)hipFree(m->pend);
@@ -216,6 +218,9 @@ bool build_model(raishin_model* m, const raishin_pram* p, const double* x1, cons
     return false;
   if (m->need_wo && !dalloc(&m->Wo, 9 * N)) return false;
   if (p->irec == 5 && !dalloc(&m->E, 18 * N)) return false;  // PPM face values
+  m->crows = ((N + 255) / 256) * 4;  // wavefronts of the largest launch (every cell, 256 per workgroup)
+  if (!dalloc(&m->cpart, (size_t)m->crows * NCOUNT)) return false;
+  HIPCHK(hipMemsetAsync(m->cpart, 0, (size_t)m->crows * NCOUNT * 8, m->st));
   HIPCHK(hipMemsetAsync(m->cnt, 0, NCOUNT * 8, m->st));
   HIPCHK(hipMemsetAsync(m->carry, 0, (1 + m->carry_cap) * 8, m->st));
   HIPCHK(hipMemsetAsync(m->tdt, 0, 16, m->st));
@@ -328,7 +333,7 @@ bool stage_update(raishin_model* m, int rkstage, const double* W, const double* 
   A.W = W; A.Ua = Ua; A.Ub = Ub; A.F1 = F; A.F2 = F + 9 * N; A.F3 = F + 18 * N;
   A.metC = m->met[0]; A.chr = m->chr; A.dx1b = m->dxb[0]; A.dx2b = m->dxb[1]; A.dx3b = m->dxb[2];
   A.r15 = m->r15; A.r25 = m->r25; A.dtbits = m->dtbits; A.Wpre = m->Wpre; A.wgout = m->wgout; A.Ukeep = Uout;
-  A.cnt = m->cnt; A.carry = m->carry; A.cflag = m->cflag; A.carry_cap = m->carry_cap;
+  A.cnt = m->cnt; A.cpart = m->cpart; A.carry = m->carry; A.cflag = m->cflag; A.carry_cap = m->carry_cap;
   A.gam = m->p.gam; A.cfl = m->p.cfl; A.dmin = m->p.dmin; A.pmin = m->p.pmin; A.iter = m->p.iter;
   if (hd) { A.ak1 = hd->ak1; A.ak2 = hd->ak2; A.ak3 = hd->ak3; A.dt_host = hd->dt; }
   { TimedLaunch t(m, K_UPDATE); HIPCHK(launch_update(G, rkstage, A, m->st)); }
@@ -345,7 +350,7 @@ bool stage_update(raishin_model* m, int rkstage, const double* W, const double* 
 bool stage_post(raishin_model* m, const double* Wold, const double* Wo, double* Wout, double* Uout) {
   PostArgs P{};
   P.Wpre = m->Wpre; P.Wghost = m->Wpre; P.Wold = Wold; P.Wo = Wo; P.metC = m->met[0]; P.r15 = m->r15;
-  P.r25 = m->r25; P.bc = m->bc; P.Wout = Wout; P.Uout = Uout; P.cnt = m->cnt; P.gam = m->p.gam;
+  P.r25 = m->r25; P.bc = m->bc; P.Wout = Wout; P.Uout = Uout; P.cnt = m->cnt; P.cpart = m->cpart; P.gam = m->p.gam;
   P.dmin = m->p.dmin; P.pmin = m->p.pmin; P.pmax = m->p.pmax; P.model = m->p.model;
   { TimedLaunch t(m, K_POST); HIPCHK(launch_post(m->G, P, m->st)); }
   return true;

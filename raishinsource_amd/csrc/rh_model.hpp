@@ -46,6 +46,8 @@ struct raishin_model {
   double *r15 = nullptr, *r25 = nullptr;
   int* bc = nullptr;
   unsigned long long *dtbits = nullptr, *cnt = nullptr, *carry = nullptr;
+  unsigned long long* cpart = nullptr;  // per-wavefront event rows, folded into cnt on read
+  long long crows = 0;
   unsigned char *cflag = nullptr, *pend = nullptr;
   long long carry_cap = 0;  // room for every interior cell: the flagged list cannot overflow
   long long* cfl_last = nullptr;
