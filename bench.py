@@ -106,6 +106,8 @@ def main():
     ap.add_argument("--config", default="c2ppm")
     ap.add_argument("--cpu-phi", type=int, default=64,
                     help="interior phi cells of the CPU-baseline sample (0: skip the CPU baseline)")
+    ap.add_argument("--mixed-steps", type=int, default=100,
+                    help="steps of the FP64-vs-mixed error comparison (0: skip the mixed-precision run)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {args.gpus}): every box here has one MI355X and "
@@ -208,8 +210,64 @@ def main():
         "sim_time": sim_time,
         "counters": counters,
     }
+    if args.mixed_steps > 0:
+        line["mixed"] = mixed_precision(p, g, m, uri, args, el / args.steps)
     print(json.dumps(line))
     return 0
+
+
+def mixed_precision(p, g, m, uri, args, s_per_step_fp64):
+    """The mixed form (reconstruction and face fluxes in FP32): its rate on
+    the same timed steps, and its per-field errors against the FP64 run after
+    args.mixed_steps steps from the same initial state."""
+    import numpy as np
+
+    from raishinsource_amd import native
+
+    model = native.Model(p, g, m)
+    model.set_state(uri)
+    model.set_precision(True)
+    for _ in range(args.warmup):
+        model.step(1, sync=False)
+    model.sync()
+    t0 = time.perf_counter()
+    model.step(args.steps, sync=False)
+    model.sync()
+    el = time.perf_counter() - t0
+    model.close()
+    nint = (p.imax - 2 * p.nm1) * (p.jmax - 2 * p.nm1) * (p.kmax - 2 * p.nm1)
+    print(f"[bench] mixed: timed {args.steps} steps: {el:.3f}s", file=sys.stderr, flush=True)
+    runs = {}
+    for mixed in (False, True):
+        mod = native.Model(p, g, m)
+        mod.set_state(uri)
+        mod.set_precision(mixed)
+        mod.step(args.mixed_steps, sync=False)
+        w, u, t = mod.get_state()
+        runs[mixed] = (w, mod.counters(), t)
+        mod.close()
+        print(f"[bench] {'mixed' if mixed else 'fp64'} {args.mixed_steps} steps done", file=sys.stderr, flush=True)
+    w64, c64, t64 = runs[False]
+    w32, c32, t32 = runs[True]
+    n1 = p.nm1
+    inner = (slice(None), slice(n1, p.imax - n1), slice(n1, p.jmax - n1), slice(n1, p.kmax - n1))
+    a, b = w32[inner], w64[inner]
+    names = ["rho", "u1", "u2", "u3", "p", "s", "B1", "B2", "B3"]
+    l1 = {nm: float(np.abs(a[v] - b[v]).sum() / max(np.abs(b[v]).sum(), 1e-300)) for v, nm in enumerate(names)}
+    linf = {nm: float(np.abs(a[v] - b[v]).max() / max(np.abs(b[v]).max(), 1e-300)) for v, nm in enumerate(names)}
+    return {
+        "value": nint * args.steps / el,
+        "unit": "zone-updates/s",
+        "ms_per_step": el / args.steps * 1e3,
+        "speedup_vs_fp64": (s_per_step_fp64 * args.steps) / el,
+        "dtype": "f32 reconstruction + face fluxes, f64 elsewhere",
+        "error_steps": args.mixed_steps,
+        "rel_l1_vs_fp64": l1,
+        "rel_linf_vs_fp64": linf,
+        "sim_time": {"fp64": t64, "mixed": t32},
+        "recov_fallback": {"fp64": c64["recov_fallback"], "mixed": c32["recov_fallback"]},
+        "recov_fail": {"fp64": c64["recov_fail"], "mixed": c32["recov_fail"]},
+    }
 
 
 if __name__ == "__main__":

@@ -86,6 +86,14 @@ void raishin_model_sync(raishin_model *m);
 /* Device address of the stream the model's kernels run on (hipStream_t). */
 void *raishin_model_stream(raishin_model *m);
 
+/* Arithmetic of the reconstruction and of the face fluxes (wave speeds,
+ * HLLE): 0 = FP64, the reference's arithmetic (default); 1 = mixed: those
+ * two in FP32 on the vector ALUs, while the metric tables, flux-CT, the RK
+ * accumulation, the sources, recov1a, the floors and the state stay FP64.
+ * Not part of the reference; its error against FP64 is measured by bench.py
+ * --precision mixed.  Takes effect from the next stage. */
+void raishin_model_set_precision(raishin_model *m, int mode);
+
 /* Per-kernel timing with HIP events on the model's stream.  Kinds, in order:
  * 0..2 direction sweeps i,j,k; 3 flux-CT; 4 RK update + sources;
  * 5 in-order recovery fix-up; 6 ghosts/floors/caluu1; 7 CFL reduction;

@@ -175,6 +175,11 @@ void raishin_model_debug_stage(raishin_model* m, int stage, double* ww, double* 
   if (go() && stage == m->p.irkt) std::swap(m->Ucur, m->Unext);
 }
 
+void raishin_model_set_precision(raishin_model* m, int mode) {
+  if (mode != 0 && mode != 1) { set_error("raishin_model_set_precision: mode %d (0 FP64, 1 mixed)", mode); return; }
+  m->mixed = mode == 1;
+}
+
 void raishin_model_set_timing(raishin_model* m, int on) {
   resolve_timing(m);
   for (int k = 0; k < K_NKINDS; k++) { m->ev_ms[k] = 0.0; m->ev_n[k] = 0; }

@@ -1,4 +1,4 @@
-// Device-side pointwise physics of the RAISHIN GRMHD step (gfx950, FP64).
+// Device-side pointwise physics of the RAISHIN GRMHD step (gfx950).
 //
 // Every function reproduces the reference arithmetic operation for operation
 // (same association, same single-precision literals) so that, compiled with
@@ -13,9 +13,17 @@
 
 namespace rh {
 
-RH_DEV double fmaxf_(double a, double b) { return a > b ? a : b; }
-RH_DEV double fminf_(double a, double b) { return a < b ? a : b; }
+template <class R> RH_DEV R fmaxf_(R a, R b) { return a > b ? a : b; }
+template <class R> RH_DEV R fminf_(R a, R b) { return a < b ? a : b; }
+// libm by arithmetic type (the float forms serve the mixed-precision kernels)
+RH_DEV double absr(double x) { return fabs(x); }
+RH_DEV float absr(float x) { return fabsf(x); }
+RH_DEV double sqrtr(double x) { return sqrt(x); }
+RH_DEV float sqrtr(float x) { return sqrtf(x); }
+RH_DEV double powr(double x, double y) { return pow(x, y); }
+RH_DEV float powr(float x, float y) { return powf(x, y); }
 RH_DEV double fsign1(double x) { return copysign(1.0, x); }
+RH_DEV float fsign1(float x) { return copysignf(1.0f, x); }
 
 // Geometry of one model on the device.
 struct Geo {
@@ -36,107 +44,119 @@ struct Met {
 };
 
 // physfunc.f90 helpers -------------------------------------------------
+// Templated on the arithmetic type R: double reproduces the reference; float
+// is the mixed-precision face arithmetic (metric values rounded to float).
 
 // calgfl: physfunc.f90:2-41
-RH_DEV double calgfl(const double u1, const double u2, const double u3, const Met& M) {
-  const double ut[4] = {0.0, u1, u2, u3};
-  double utsq = 0.0;
+template <class R>
+RH_DEV R calgfl(const R u1, const R u2, const R u3, const Met& M) {
+  const R ut[4] = {R(0), u1, u2, u3};
+  R utsq = R(0);
 #pragma unroll
   for (int m = 1; m <= 3; m++)
 #pragma unroll
-    for (int n = 1; n <= 3; n++) utsq = utsq + M.g(m, n) * ut[m] * ut[n];
-  if (utsq < 0.0) {
-    if (fabs(utsq) <= 1.0e-10) utsq = fabs(utsq);
-    else utsq = 1.0e-10;
+    for (int n = 1; n <= 3; n++) utsq = utsq + R(M.g(m, n)) * ut[m] * ut[n];
+  if (utsq < R(0)) {
+    if (absr(utsq) <= R(1.0e-10)) utsq = absr(utsq);
+    else utsq = R(1.0e-10);
   }
-  if (fabs(utsq) > 5.0e2) utsq = 5.0e2;
-  return sqrt(1.0 + utsq);
+  if (absr(utsq) > R(5.0e2)) utsq = R(5.0e2);
+  return sqrtr(R(1) + utsq);
 }
 
 // calucon: physfunc.f90:44-74
-RH_DEV void calucon(const double ut[4], double ucon[4], double gfl, const Met& M) {
-  const double alpha1 = 1.0 / sqrt(-M.gi(0, 0));
-  const double b1 = alpha1 * alpha1 * M.gi(0, 1);
-  const double b2 = alpha1 * alpha1 * M.gi(0, 2);
-  const double b3 = alpha1 * alpha1 * M.gi(0, 3);
-  ucon[0] = gfl * (1.0 / alpha1);
-  ucon[1] = ut[1] - gfl * b1 * (1.0 / alpha1);
-  ucon[2] = ut[2] - gfl * b2 * (1.0 / alpha1);
-  ucon[3] = ut[3] - gfl * b3 * (1.0 / alpha1);
+template <class R>
+RH_DEV void calucon(const R ut[4], R ucon[4], R gfl, const Met& M) {
+  const R alpha1 = R(1) / sqrtr(-R(M.gi(0, 0)));
+  const R b1 = alpha1 * alpha1 * R(M.gi(0, 1));
+  const R b2 = alpha1 * alpha1 * R(M.gi(0, 2));
+  const R b3 = alpha1 * alpha1 * R(M.gi(0, 3));
+  ucon[0] = gfl * (R(1) / alpha1);
+  ucon[1] = ut[1] - gfl * b1 * (R(1) / alpha1);
+  ucon[2] = ut[2] - gfl * b2 * (R(1) / alpha1);
+  ucon[3] = ut[3] - gfl * b3 * (R(1) / alpha1);
 }
 
 // lower: physfunc.f90:158-192
-RH_DEV void lower(const double v[4], double w[4], const Met& M) {
+template <class R>
+RH_DEV void lower(const R v[4], R w[4], const Met& M) {
 #pragma unroll
   for (int m = 0; m < 4; m++)
-    w[m] = M.g(m, 0) * v[0] + M.g(m, 1) * v[1] + M.g(m, 2) * v[2] + M.g(m, 3) * v[3];
+    w[m] = R(M.g(m, 0)) * v[0] + R(M.g(m, 1)) * v[1] + R(M.g(m, 2)) * v[2] + R(M.g(m, 3)) * v[3];
 }
 // upper: physfunc.f90:195-230
-RH_DEV void upper(const double v[4], double w[4], const Met& M) {
+template <class R>
+RH_DEV void upper(const R v[4], R w[4], const Met& M) {
 #pragma unroll
   for (int m = 0; m < 4; m++)
-    w[m] = M.gi(m, 0) * v[0] + M.gi(m, 1) * v[1] + M.gi(m, 2) * v[2] + M.gi(m, 3) * v[3];
+    w[m] = R(M.gi(m, 0)) * v[0] + R(M.gi(m, 1)) * v[1] + R(M.gi(m, 2)) * v[2] + R(M.gi(m, 3)) * v[3];
 }
 
 // calbbcon: physfunc.f90:77-95
-RH_DEV void calbbcon(const double bcon[4], double bb[4], const double ucov[4], const double ucon[4]) {
+template <class R>
+RH_DEV void calbbcon(const R bcon[4], R bb[4], const R ucov[4], const R ucon[4]) {
   bb[0] = bcon[1] * ucov[1] + bcon[2] * ucov[2] + bcon[3] * ucov[3];
-  bb[1] = (bcon[1] + bb[0] * ucon[1]) * (1.0 / ucon[0]);
-  bb[2] = (bcon[2] + bb[0] * ucon[2]) * (1.0 / ucon[0]);
-  bb[3] = (bcon[3] + bb[0] * ucon[3]) * (1.0 / ucon[0]);
+  bb[1] = (bcon[1] + bb[0] * ucon[1]) * (R(1) / ucon[0]);
+  bb[2] = (bcon[2] + bb[0] * ucon[2]) * (R(1) / ucon[0]);
+  bb[3] = (bcon[3] + bb[0] * ucon[3]) * (R(1) / ucon[0]);
 }
 
 // cal3met: coord.f90:970-1013; g3/gi3 hold gcov3a/gcon3a(m,n) at (m-1)+3(n-1)
-RH_DEV void cal3met(const Met& M, double g3[9], double gi3[9]) {
-  const double alpha1 = 1.0 / sqrt(-M.gi(0, 0));
-  double beta1[4];
-  beta1[1] = alpha1 * alpha1 * M.gi(0, 1);
-  beta1[2] = alpha1 * alpha1 * M.gi(0, 2);
-  beta1[3] = alpha1 * alpha1 * M.gi(0, 3);
-  double oncon[4], oncov[4];
-  oncon[0] = 1.0 / alpha1;
+template <class R>
+RH_DEV void cal3met(const Met& M, R g3[9], R gi3[9]) {
+  const R alpha1 = R(1) / sqrtr(-R(M.gi(0, 0)));
+  R beta1[4];
+  beta1[1] = alpha1 * alpha1 * R(M.gi(0, 1));
+  beta1[2] = alpha1 * alpha1 * R(M.gi(0, 2));
+  beta1[3] = alpha1 * alpha1 * R(M.gi(0, 3));
+  R oncon[4], oncov[4];
+  oncon[0] = R(1) / alpha1;
   oncon[1] = beta1[1] / alpha1;
   oncon[2] = beta1[2] / alpha1;
   oncon[3] = beta1[3] / alpha1;
-  oncov[0] = -alpha1; oncov[1] = 0.0; oncov[2] = 0.0; oncov[3] = 0.0;
+  oncov[0] = -alpha1; oncov[1] = R(0); oncov[2] = R(0); oncov[3] = R(0);
 #pragma unroll
   for (int m = 1; m <= 3; m++)
 #pragma unroll
     for (int n = 1; n <= 3; n++) {
-      g3[(m - 1) + 3 * (n - 1)] = M.g(m, n) + (oncov[m] * oncov[n]);
-      gi3[(m - 1) + 3 * (n - 1)] = M.gi(m, n) + (oncon[m] * oncon[n]);
+      g3[(m - 1) + 3 * (n - 1)] = R(M.g(m, n)) + (oncov[m] * oncov[n]);
+      gi3[(m - 1) + 3 * (n - 1)] = R(M.gi(m, n)) + (oncon[m] * oncon[n]);
     }
 }
 
 // Shared prelude of caluu1/caluu3/calflx3*/caltenr/calcha4*:
 // primitives -> enthalpy, Lorentz factor, u^mu, u_mu, b^mu, b_mu, b^2.
-struct Prel {
-  double de, pr, roh, gfl, bbsq;
-  double ut[4], bcon[4], ucon[4], ucov[4], bb[4], bbcov[4];
+template <class R>
+struct PrelT {
+  R de, pr, roh, gfl, bbsq;
+  R ut[4], bcon[4], ucon[4], ucov[4], bb[4], bbcov[4];
 };
+using Prel = PrelT<double>;
 
-RH_DEV void prelude(const double w[9], double gam, const Met& M, Prel& s) {
+template <class R>
+RH_DEV void prelude(const R w[9], R gam, const Met& M, PrelT<R>& s) {
   s.de = w[0];
-  s.ut[0] = 0.0; s.ut[1] = w[1]; s.ut[2] = w[2]; s.ut[3] = w[3];
+  s.ut[0] = R(0); s.ut[1] = w[1]; s.ut[2] = w[2]; s.ut[3] = w[3];
   s.pr = w[4];
-  s.bcon[0] = 0.0; s.bcon[1] = w[6]; s.bcon[2] = w[7]; s.bcon[3] = w[8];
-  s.roh = s.de + (gam / (gam - 1.0)) * s.pr;   // caluu.f90:52 (ieos=0)
-  s.gfl = calgfl(w[1], w[2], w[3], M);
-  calucon(s.ut, s.ucon, s.gfl, M);
-  lower(s.ucon, s.ucov, M);
-  calbbcon(s.bcon, s.bb, s.ucov, s.ucon);
-  lower(s.bb, s.bbcov, M);
+  s.bcon[0] = R(0); s.bcon[1] = w[6]; s.bcon[2] = w[7]; s.bcon[3] = w[8];
+  s.roh = s.de + (gam / (gam - R(1))) * s.pr;   // caluu.f90:52 (ieos=0)
+  s.gfl = calgfl<R>(w[1], w[2], w[3], M);
+  calucon<R>(s.ut, s.ucon, s.gfl, M);
+  lower<R>(s.ucon, s.ucov, M);
+  calbbcon<R>(s.bcon, s.bb, s.ucov, s.ucon);
+  lower<R>(s.bb, s.bbcov, M);
   s.bbsq = s.bb[0] * s.bbcov[0] + s.bb[1] * s.bbcov[1] + s.bb[2] * s.bbcov[2] + s.bb[3] * s.bbcov[3];
 }
 
-// Row d of sqrt(-g) * (rho u^d, T^d_i, T^d_t + rho u^d, rho s u^d, induction):
+// Row d of sqrtr(-g) * (rho u^d, T^d_i, T^d_t + rho u^d, rho s u^d, induction):
 // d=0 is caluu1/caluu3 (caluu.f90:85-103), d=1..3 calflx3i/j/k (calflx.f90:468-507,
 // 608-646, 744-782). powdg = de**gam computed once by the caller.
-RH_DEV void tmunu_row(const Prel& s, int d, double detg, double powdg, double out[9]) {
-  double delta[4] = {0.0, 0.0, 0.0, 0.0};
-  delta[d] = 1.0;
-  const double rb = s.roh + s.bbsq, pb = s.pr + 0.5 * s.bbsq;
-  const double ud = s.ucon[d];
+template <class R>
+RH_DEV void tmunu_row(const PrelT<R>& s, int d, R detg, R powdg, R out[9]) {
+  R delta[4] = {R(0), R(0), R(0), R(0)};
+  delta[d] = R(1);
+  const R rb = s.roh + s.bbsq, pb = s.pr + R(0.5) * s.bbsq;
+  const R ud = s.ucon[d];
   out[0] = s.de * ud;
   out[1] = rb * ud * s.ucov[1] + pb * delta[1] - s.bb[d] * s.bbcov[1];
   out[2] = rb * ud * s.ucov[2] + pb * delta[2] - s.bb[d] * s.bbcov[2];
@@ -146,10 +166,10 @@ RH_DEV void tmunu_row(const Prel& s, int d, double detg, double powdg, double ou
   if (d == 0) {
     out[6] = s.bcon[1]; out[7] = s.bcon[2]; out[8] = s.bcon[3];
   } else {
-    const double r0 = 1.0 / s.ucon[0];
+    const R r0 = R(1) / s.ucon[0];
 #pragma unroll
     for (int c = 1; c <= 3; c++)
-      out[5 + c] = (c == d) ? 0.0 : (ud * r0) * s.bcon[c] - (s.ucon[c] * r0) * s.bcon[d];
+      out[5 + c] = (c == d) ? R(0) : (ud * r0) * s.bcon[c] - (s.ucon[c] * r0) * s.bcon[d];
   }
 #pragma unroll
   for (int n = 0; n < 9; n++) out[n] = detg * out[n];
@@ -158,41 +178,42 @@ RH_DEV void tmunu_row(const Prel& s, int d, double detg, double powdg, double ou
 // Fast magnetosonic speeds (icha=2, Del Zanna et al. 2007) along direction d
 // for a face state: calcha4i/j/k (calcha.f90:370-906).  The three directions
 // write cs^2 and vA^2 differently (calcha.f90:480,648,826 and 486,654,832).
-RH_DEV void face_speed(const Prel& s, int d, double gam, const Met& M, double& vfp, double& vfm) {
-  const double alpha1 = 1.0 / sqrt(-M.gi(0, 0));
-  const double beta_d = alpha1 * alpha1 * M.gi(0, d);
-  double g3[9], gi3[9];
-  cal3met(M, g3, gi3);
-  double vcon[4];
-  vcon[1] = s.ut[1] * (1.0 / s.gfl);
-  vcon[2] = s.ut[2] * (1.0 / s.gfl);
-  vcon[3] = s.ut[3] * (1.0 / s.gfl);
-  double vsq = 0.0;
+template <class R>
+RH_DEV void face_speed(const PrelT<R>& s, int d, R gam, const Met& M, R& vfp, R& vfm) {
+  const R alpha1 = R(1) / sqrtr(-R(M.gi(0, 0)));
+  const R beta_d = alpha1 * alpha1 * R(M.gi(0, d));
+  R g3[9], gi3[9];
+  cal3met<R>(M, g3, gi3);
+  R vcon[4];
+  vcon[1] = s.ut[1] * (R(1) / s.gfl);
+  vcon[2] = s.ut[2] * (R(1) / s.gfl);
+  vcon[3] = s.ut[3] * (R(1) / s.gfl);
+  R vsq = R(0);
 #pragma unroll
   for (int m = 1; m <= 3; m++)
 #pragma unroll
     for (int n = 1; n <= 3; n++) vsq = vsq + g3[(m - 1) + 3 * (n - 1)] * vcon[m] * vcon[n];
-  double cssq, vasq;
-  if (d == 3) cssq = gam * s.pr * (1.0 / s.roh);
+  R cssq, vasq;
+  if (d == 3) cssq = gam * s.pr * (R(1) / s.roh);
   else cssq = gam * s.pr / s.roh;
   if (d == 2) vasq = s.bbsq / (s.roh + s.bbsq);
-  else vasq = s.bbsq * (1.0 / (s.roh + s.bbsq));
-  const double omsq = vasq + cssq * (1.0 - vasq);
-  const double f1 = vcon[d] * (1.0 - omsq);
-  const double f2 = 1.0 - vsq * omsq;
-  const double f3 = omsq * (1.0 - vsq) * ((1.0 - vsq * omsq) * gi3[(d - 1) * 4] - (1.0 - omsq) * vcon[d] * vcon[d]);
-  double p = (f1 + sqrt(f3)) * (1.0 / f2);
-  double m = (f1 - sqrt(f3)) * (1.0 / f2);
+  else vasq = s.bbsq * (R(1) / (s.roh + s.bbsq));
+  const R omsq = vasq + cssq * (R(1) - vasq);
+  const R f1 = vcon[d] * (R(1) - omsq);
+  const R f2 = R(1) - vsq * omsq;
+  const R f3 = omsq * (R(1) - vsq) * ((R(1) - vsq * omsq) * gi3[(d - 1) * 4] - (R(1) - omsq) * vcon[d] * vcon[d]);
+  R p = (f1 + sqrtr(f3)) * (R(1) / f2);
+  R m = (f1 - sqrtr(f3)) * (R(1) / f2);
   p = alpha1 * p - beta_d;
   m = alpha1 * m - beta_d;
-  if (fabs(p) > 1.0) p = 1.0;
-  if (fabs(m) > 1.0) m = -1.0;
+  if (absr(p) > R(1)) p = R(1);
+  if (absr(m) > R(1)) m = R(-1);
   vfp = p; vfm = m;
 }
 
 // Centre speeds for the CFL condition: calcha4 (calcha.f90:129-367).
 RH_DEV void centre_speeds(const Prel& s, double gam, const Met& M, double vp[4], double vm[4]) {
-  const double alpha1 = 1.0 / sqrt(-M.gi(0, 0));
+  const double alpha1 = 1.0 / sqrtr(-M.gi(0, 0));
   double beta1[4];
   beta1[1] = alpha1 * alpha1 * M.gi(0, 1);
   beta1[2] = alpha1 * alpha1 * M.gi(0, 2);
@@ -216,88 +237,93 @@ RH_DEV void centre_speeds(const Prel& s, double gam, const Met& M, double vp[4],
   for (int d = 1; d <= 3; d++) {
     const double f1 = vcon[d] * (1.0 - omsq);
     const double f3 = omsq * (1.0 - vsq) * ((1.0 - vsq * omsq) * gi3[(d - 1) * 4] - (1.0 - omsq) * vcon[d] * vcon[d]);
-    vp[d] = (f1 + sqrt(f3)) * (1.0 / f2);
-    vm[d] = (f1 - sqrt(f3)) * (1.0 / f2);
+    vp[d] = (f1 + sqrtr(f3)) * (1.0 / f2);
+    vm[d] = (f1 - sqrtr(f3)) * (1.0 / f2);
   }
 #pragma unroll
   for (int d = 1; d <= 3; d++) {
     vp[d] = alpha1 * vp[d] - beta1[d];
     vm[d] = alpha1 * vm[d] - beta1[d];
-    if (fabs(vp[d]) > 1.0) vp[d] = 1.0;
-    if (fabs(vm[d]) > 1.0) vm[d] = -1.0;
+    if (absr(vp[d]) > 1.0) vp[d] = 1.0;
+    if (absr(vm[d]) > 1.0) vm[d] = -1.0;
   }
 }
 
 // Reconstruction ---------------------------------------------------------
 
-RH_DEV double dmm(double x, double y) {
-  return 0.5 * (fsign1(x) + fsign1(y)) * fminf_(fabs(x), fabs(y));
+template <class R>
+RH_DEV R dmm(R x, R y) {
+  return R(0.5) * (fsign1(x) + fsign1(y)) * fminf_(absr(x), absr(y));
 }
-RH_DEV double dm4(double w, double x, double y, double z) {
-  return 0.125 * (fsign1(w) + fsign1(x)) * fabs(fsign1(w) + fsign1(y)) * (fsign1(w) + fsign1(z)) *
-         fminf_(fminf_(fminf_(fabs(w), fabs(x)), fabs(y)), fabs(z));
+template <class R>
+RH_DEV R dm4(R w, R x, R y, R z) {
+  return R(0.125) * (fsign1(w) + fsign1(x)) * absr(fsign1(w) + fsign1(y)) * (fsign1(w) + fsign1(z)) *
+         fminf_(fminf_(fminf_(absr(w), absr(x)), absr(y)), absr(z));
 }
 
 // MP5 (Suresh & Huynh), rec.f90:2059-2343.  s[0..4] = u(c-2..c+2).
 // Right edge of cell c (uri*l(c)): rec.f90:2113-2145.
-RH_DEV double mp5_right(const double s[5]) {
-  const double tmp1 = 1.0 / 60.0, tmp2 = 4.0 / 3.0, alpha = 4.0, eps1 = 1.0e-10;
-  const double u1 = tmp1 * (2.0 * s[0] - 13.0 * s[1] + 47.0 * s[2] + 27.0 * s[3] - 3.0 * s[4]);
-  const double ump = s[2] + dmm(s[3] - s[2], alpha * (s[2] - s[1]));
+template <class R>
+RH_DEV R mp5_right(const R s[5]) {
+  const R tmp1 = R(1.0) / R(60.0), tmp2 = R(4.0) / R(3.0), alpha = R(4.0), eps1 = R(1.0e-10);
+  const R u1 = tmp1 * (R(2.0) * s[0] - R(13.0) * s[1] + R(47.0) * s[2] + R(27.0) * s[3] - R(3.0) * s[4]);
+  const R ump = s[2] + dmm(s[3] - s[2], alpha * (s[2] - s[1]));
   if ((u1 - s[2]) * (u1 - ump) <= eps1) return u1;
-  const double dim1 = s[0] - 2.0 * s[1] + s[2];
-  const double di1 = s[1] - 2.0 * s[2] + s[3];
-  const double dip1 = s[2] - 2.0 * s[3] + s[4];
-  const double dph = dm4(4.0 * di1 - dip1, 4.0 * dip1 - di1, di1, dip1);
-  const double dmh = dm4(4.0 * di1 - dim1, 4.0 * dim1 - di1, di1, dim1);
-  const double ul = s[2] + alpha * (s[2] - s[1]);
-  const double av = 0.5 * (s[2] + s[3]);
-  const double md = av - 0.5 * dph;
-  const double lc = s[2] + 0.5 * (s[2] - s[1]) + tmp2 * dmh;
-  const double umax = fmaxf_(fminf_(fminf_(s[2], s[3]), md), fminf_(fminf_(s[2], ul), lc));
-  const double umin = fminf_(fmaxf_(fmaxf_(s[2], s[3]), md), fmaxf_(fmaxf_(s[2], ul), lc));
+  const R dim1 = s[0] - R(2.0) * s[1] + s[2];
+  const R di1 = s[1] - R(2.0) * s[2] + s[3];
+  const R dip1 = s[2] - R(2.0) * s[3] + s[4];
+  const R dph = dm4(R(4.0) * di1 - dip1, R(4.0) * dip1 - di1, di1, dip1);
+  const R dmh = dm4(R(4.0) * di1 - dim1, R(4.0) * dim1 - di1, di1, dim1);
+  const R ul = s[2] + alpha * (s[2] - s[1]);
+  const R av = R(0.5) * (s[2] + s[3]);
+  const R md = av - R(0.5) * dph;
+  const R lc = s[2] + R(0.5) * (s[2] - s[1]) + tmp2 * dmh;
+  const R umax = fmaxf_(fminf_(fminf_(s[2], s[3]), md), fminf_(fminf_(s[2], ul), lc));
+  const R umin = fminf_(fmaxf_(fmaxf_(s[2], s[3]), md), fmaxf_(fmaxf_(s[2], ul), lc));
   return u1 + dmm(umin - u1, umax - u1);
 }
 // Left edge of cell c (uri*r(c-1)): rec.f90:2149-2186.
-RH_DEV double mp5_left(const double s[5]) {
-  const double tmp1 = 1.0 / 60.0, tmp2 = 4.0 / 3.0, alpha = 4.0, eps1 = 1.0e-10;
-  const double u1 = tmp1 * (2.0 * s[4] - 13.0 * s[3] + 47.0 * s[2] + 27.0 * s[1] - 3.0 * s[0]);
-  const double ump = s[2] + dmm(s[1] - s[2], alpha * (s[2] - s[3]));
+template <class R>
+RH_DEV R mp5_left(const R s[5]) {
+  const R tmp1 = R(1.0) / R(60.0), tmp2 = R(4.0) / R(3.0), alpha = R(4.0), eps1 = R(1.0e-10);
+  const R u1 = tmp1 * (R(2.0) * s[4] - R(13.0) * s[3] + R(47.0) * s[2] + R(27.0) * s[1] - R(3.0) * s[0]);
+  const R ump = s[2] + dmm(s[1] - s[2], alpha * (s[2] - s[3]));
   if ((u1 - s[2]) * (u1 - ump) <= eps1) return u1;
-  const double dim1 = s[0] - 2.0 * s[1] + s[2];
-  const double di1 = s[1] - 2.0 * s[2] + s[3];
-  const double dip1 = s[2] - 2.0 * s[3] + s[4];
-  const double dph = dm4(4.0 * di1 - dip1, 4.0 * dip1 - di1, di1, dip1);
-  const double dmh = dm4(4.0 * di1 - dim1, 4.0 * dim1 - di1, di1, dim1);
-  const double ul = s[2] + alpha * (s[2] - s[3]);
-  const double av = 0.5 * (s[2] + s[1]);
-  const double md = av + 0.5 * dmh;
-  const double lc = s[2] + 0.5 * (s[2] - s[3]) - tmp2 * dph;
-  const double umax = fmaxf_(fminf_(fminf_(s[2], s[1]), md), fminf_(fminf_(s[2], ul), lc));
-  const double umin = fminf_(fmaxf_(fmaxf_(s[2], s[1]), md), fmaxf_(fmaxf_(s[2], ul), lc));
+  const R dim1 = s[0] - R(2.0) * s[1] + s[2];
+  const R di1 = s[1] - R(2.0) * s[2] + s[3];
+  const R dip1 = s[2] - R(2.0) * s[3] + s[4];
+  const R dph = dm4(R(4.0) * di1 - dip1, R(4.0) * dip1 - di1, di1, dip1);
+  const R dmh = dm4(R(4.0) * di1 - dim1, R(4.0) * dim1 - di1, di1, dim1);
+  const R ul = s[2] + alpha * (s[2] - s[3]);
+  const R av = R(0.5) * (s[2] + s[1]);
+  const R md = av + R(0.5) * dmh;
+  const R lc = s[2] + R(0.5) * (s[2] - s[3]) - tmp2 * dph;
+  const R umax = fmaxf_(fminf_(fminf_(s[2], s[1]), md), fminf_(fminf_(s[2], ul), lc));
+  const R umin = fminf_(fmaxf_(fmaxf_(s[2], s[1]), md), fmaxf_(fmaxf_(s[2], ul), lc));
   return u1 + dmm(umin - u1, umax - u1);
 }
 
 // WENO5 (Jiang & Shu), rec.f90:2373-2642; 13./12. is a single-precision literal.
-RH_DEV double weno_edge(double a0, double a1, double a2, double a3, double a4) {
+template <class R>
+RH_DEV R weno_edge(R a0, R a1, R a2, R a3, R a4) {
   // a0..a4 = stencil ordered from the far upwind side to the far downwind side
-  const double c1312 = (double)(13.0f / 12.0f), c14 = 0.25, epsi = 1.0e-6;
-  const double cd0 = 1.0 / 10.0, cd1 = 6.0 / 10.0, cd2 = 3.0 / 10.0;
-  const double c00 = 2.0 / 6.0, c01 = -7.0 / 6.0, c02 = 11.0 / 6.0, c10 = -1.0 / 6.0, c11 = 5.0 / 6.0,
-               c12 = 2.0 / 6.0, c20 = 2.0 / 6.0, c21 = 5.0 / 6.0, c22 = -1.0 / 6.0;
-  double a, b;
-  a = a0 - 2.0 * a1 + a2; b = a0 - 4.0 * a1 + 3.0 * a2;
-  const double b0 = c1312 * (a * a) + c14 * (b * b) + epsi;
-  a = a1 - 2.0 * a2 + a3; b = a1 - a3;
-  const double b1 = c1312 * (a * a) + c14 * (b * b) + epsi;
-  a = a2 - 2.0 * a3 + a4; b = 3.0 * a2 - 4.0 * a3 + a4;
-  const double b2 = c1312 * (a * a) + c14 * (b * b) + epsi;
-  const double w0 = cd0 / (b0 * b0), w1 = cd1 / (b1 * b1), w2 = cd2 / (b2 * b2);
-  const double wt = w0 + w1 + w2;
-  const double o0 = w0 / wt, o1 = w1 / wt, o2 = w2 / wt;
-  const double r0 = c00 * a0 + c01 * a1 + c02 * a2;
-  const double r1 = c10 * a1 + c11 * a2 + c12 * a3;
-  const double r2 = c20 * a2 + c21 * a3 + c22 * a4;
+  const R c1312 = R(13.0f / 12.0f), c14 = R(0.25), epsi = R(1.0e-6);
+  const R cd0 = R(1.0) / R(10.0), cd1 = R(6.0) / R(10.0), cd2 = R(3.0) / R(10.0);
+  const R c00 = R(2.0) / R(6.0), c01 = -R(7.0) / R(6.0), c02 = R(11.0) / R(6.0), c10 = -R(1.0) / R(6.0), c11 = R(5.0) / R(6.0),
+               c12 = R(2.0) / R(6.0), c20 = R(2.0) / R(6.0), c21 = R(5.0) / R(6.0), c22 = -R(1.0) / R(6.0);
+  R a, b;
+  a = a0 - R(2.0) * a1 + a2; b = a0 - R(4.0) * a1 + R(3.0) * a2;
+  const R b0 = c1312 * (a * a) + c14 * (b * b) + epsi;
+  a = a1 - R(2.0) * a2 + a3; b = a1 - a3;
+  const R b1 = c1312 * (a * a) + c14 * (b * b) + epsi;
+  a = a2 - R(2.0) * a3 + a4; b = R(3.0) * a2 - R(4.0) * a3 + a4;
+  const R b2 = c1312 * (a * a) + c14 * (b * b) + epsi;
+  const R w0 = cd0 / (b0 * b0), w1 = cd1 / (b1 * b1), w2 = cd2 / (b2 * b2);
+  const R wt = w0 + w1 + w2;
+  const R o0 = w0 / wt, o1 = w1 / wt, o2 = w2 / wt;
+  const R r0 = c00 * a0 + c01 * a1 + c02 * a2;
+  const R r1 = c10 * a1 + c11 * a2 + c12 * a3;
+  const R r2 = c20 * a2 + c21 * a3 + c22 * a4;
   return o0 * r0 + o1 * r1 + o2 * r2;
 }
 

@@ -103,112 +103,122 @@ __global__ void __launch_bounds__(1024) k_sum_counters(unsigned long long* __res
 // p is the 1-based index of the cell along D; the loads are along stride s.
 // dxs[d] holds the dummy dx*b(d) of rec2 (= actual dx*b(d-1), rec.f90:58).
 // ------------------------------------------------------------------------
-template <int D, class Store>
-RH_DEV void ppm_cell(const double* __restrict__ W, long long N, long long c, long long s, int p,
-                     const double* __restrict__ dxs, const double* __restrict__ xs, const double* __restrict__ pc,
-                     long long npc, double gam, Store store) {
-  const double dm2 = dxs[p - 2], dm1 = dxs[p - 1], d0 = dxs[p], dp1 = dxs[p + 1], dp2 = dxs[p + 2];
-  // interp's width-only coefficients come from the table pc (ppm_coefficients
-  // on the host, same expressions and order): planes 0..2 of dma at q
-  // (rec.f90:1353-1355), planes 3..6 of equation (51) at q+1/2 (rec.f90:1427-1436)
-  auto dma_of = [&](double um, double u0, double up, int q) -> double {
-    const double t1a = pc[q], t1b = pc[npc + q], t1c = pc[2 * npc + q];
-    const double sp1 = (up - u0);
-    const double sm1 = (u0 - um);
-    const double spm1 = sp1 * sm1;
-    const double da = t1a * (t1b * sp1 + t1c * sm1);
-    if (spm1 <= 0.0) return 0.0;
-    const double t1 = (da < 0.0) ? -1.0 : 1.0;
-    const double t1d = fminf_(fabs(sm1), fabs(sp1));
-    if (D == 3) return t1 * fminf_(fabs(da), t1d);   // rec.f90:1414
-    return t1 * fminf_(fabs(da), 2.0 * t1d);
-  };
-  auto eq51 = [&](double uq, double uq1, double dmq, double dmq1, int q) -> double {
-    const double t2 = pc[3 * npc + q] * (uq1 - uq);
-    const double t3 = pc[4 * npc + q] * (uq1 - uq);
-    const double t4 = pc[5 * npc + q];
-    const double t5 = pc[6 * npc + q];
-    return uq + t2 + t3 - t4 * dmq1 + t5 * dmq;
-  };
-
-  // --- flattening coefficient f2 of this cell (rec.f90:1736-1858) ---
+// Shape coefficients of the cell: flattening f2 (rec.f90:1736-1858) and the
+// contact-steepening eta of the density (rec.f90:1537-1629).  R is the
+// arithmetic type (double: the reference's; float: the mixed-precision form).
+template <int D, class R>
+RH_DEV void ppm_shape(const double* __restrict__ W, long long N, long long c, long long s, int p,
+                      const double* __restrict__ dxs, const double* __restrict__ xs, R gam, R& f2_out, R& eta_out) {
+  const R dm2 = R(dxs[p - 2]), dm1 = R(dxs[p - 1]), d0 = R(dxs[p]), dp1 = R(dxs[p + 1]), dp2 = R(dxs[p + 2]);
+  // --- flattening coefficient f2 of this cell ---
   const int VEL = D;  // velocity component along D is variable D+1 (0-based D)
   const double* P5 = W + 4 * N + c;
   const double* PV = W + (long long)VEL * N + c;
-  double f1[3];
+  R f1[3];
 #pragma unroll
   for (int q = -1; q <= 1; q++) {
-    const double pp1 = P5[(q + 1) * s], pm1 = P5[(q - 1) * s], pp2 = P5[(q + 2) * s], pm2 = P5[(q - 2) * s];
-    const double t3a = fabs(pp1 - pm1);
-    const double t1 = 1.0 * fminf_(pp1, pm1) - t3a;
-    const double t2 = PV[(q + 1) * s] - PV[(q - 1) * s];
-    const double omg1 = (t1 < 0.0 && t2 < 0.0) ? 1.0 : 0.0;
-    const double t3b = (pp2 - pm2);
-    double t4;
-    if (t3b == 0.0) t4 = (t3a == 0.0) ? -0.52 : 1.0 - 0.52;
-    else t4 = (t3a / t3b) - 0.52;
-    f1[q + 1] = fminf_(1.0, omg1 * fmaxf_(0.0, t4 * 10.0));
+    const R pp1 = R(P5[(q + 1) * s]), pm1 = R(P5[(q - 1) * s]), pp2 = R(P5[(q + 2) * s]), pm2 = R(P5[(q - 2) * s]);
+    const R t3a = absr(pp1 - pm1);
+    const R t1 = R(1) * fminf_(pp1, pm1) - t3a;
+    const R t2 = R(PV[(q + 1) * s]) - R(PV[(q - 1) * s]);
+    const R omg1 = (t1 < R(0) && t2 < R(0)) ? R(1) : R(0);
+    const R t3b = (pp2 - pm2);
+    R t4;
+    if (t3b == R(0)) t4 = (t3a == R(0)) ? R(-0.52) : R(1.0) - R(0.52);
+    else t4 = (t3a / t3b) - R(0.52);
+    f1[q + 1] = fminf_(R(1), omg1 * fmaxf_(R(0), t4 * R(10.0)));
   }
-  const double f1b = (P5[s] - P5[-s] < 0.0) ? f1[2] : f1[0];
-  const double f2 = fmaxf_(f1[1], f1b);
+  const R p5p = R(P5[s]), p5m = R(P5[-s]);
+  const R f1b = (p5p - p5m < R(0)) ? f1[2] : f1[0];
+  f2_out = fmaxf_(f1[1], f1b);
 
-  // --- contact steepening parameter eta (density only; rec.f90:1537-1629) ---
-  double eta;
+  // --- contact steepening parameter eta (density only) ---
+  R eta;
   {
     const double* P1 = W + c;
-    const double um2 = P1[-2 * s], um1 = P1[-s], u0 = P1[0], up1 = P1[s], up2 = P1[2 * s];
-    const double ddm = (1.0 / (dm2 + dm1 + d0)) * ((u0 - um1) / (d0 + dm1) - (um1 - um2) / (dm1 + dm2));
-    const double ddp = (1.0 / (d0 + dp1 + dp2)) * ((up2 - up1) / (dp2 + dp1) - (up1 - u0) / (dp1 + d0));
-    const double t1 = ddp * ddm;
-    const double t2 = fabs(up1 - um1) - 0.1 * fminf_(fabs(up1), fabs(um1));
-    double etatil;
-    if (t1 >= 0.0 && t2 <= 0.0) etatil = 0.0;
+    const R um2 = R(P1[-2 * s]), um1 = R(P1[-s]), u0 = R(P1[0]), up1 = R(P1[s]), up2 = R(P1[2 * s]);
+    const R ddm = (R(1) / (dm2 + dm1 + d0)) * ((u0 - um1) / (d0 + dm1) - (um1 - um2) / (dm1 + dm2));
+    const R ddp = (R(1) / (d0 + dp1 + dp2)) * ((up2 - up1) / (dp2 + dp1) - (up1 - u0) / (dp1 + d0));
+    const R t1 = ddp * ddm;
+    const R t2 = absr(up1 - um1) - R(0.1) * fminf_(absr(up1), absr(um1));
+    R etatil;
+    if (t1 >= R(0) && t2 <= R(0)) etatil = R(0);
     else {
-      const double xm = xs[p - 2], x0 = xs[p - 1], xp = xs[p];
-      const double t2a = -(ddp - ddm) / (xp - xm);
-      const double dl = x0 - xm, dr = xp - x0;
-      const double t2b = ((dl * dl * dl) + (dr * dr * dr)) / (up1 - um1);
+      const R xm = R(xs[p - 2]), x0 = R(xs[p - 1]), xp = R(xs[p]);
+      const R t2a = -(ddp - ddm) / (xp - xm);
+      const R dl = x0 - xm, dr = xp - x0;
+      const R t2b = ((dl * dl * dl) + (dr * dr * dr)) / (up1 - um1);
       etatil = t2a * t2b;
     }
-    eta = fmaxf_(0.0, fminf_(5.0 * (etatil - 0.05), 1.0));
-    const double t3 = fabs(up1 - um1) / fminf_(up1, um1);
-    const double t4 = fabs(P5[s] - P5[-s]) / fminf_(P5[s], P5[-s]);
-    if (gam * 1.0 * t3 < t4) eta = 0.0;
+    eta = fmaxf_(R(0), fminf_(R(5.0) * (etatil - R(0.05)), R(1)));
+    const R t3 = absr(up1 - um1) / fminf_(up1, um1);
+    const R t4 = absr(p5p - p5m) / fminf_(p5p, p5m);
+    if (gam * R(1) * t3 < t4) eta = R(0);
   }
+  eta_out = eta;
+}
 
-#pragma unroll 1
-  for (int n = 0; n < 9; n++) {
-    const double* Pn = W + (long long)n * N + c;
-    const double um2 = Pn[-2 * s], um1 = Pn[-s], u0 = Pn[0], up1 = Pn[s], up2 = Pn[2 * s];
-    const double dmm1 = dma_of(um2, um1, u0, p - 1);
-    const double dm0 = dma_of(um1, u0, up1, p);
-    const double dmp1 = dma_of(u0, up1, up2, p + 1);
-    double r = eq51(u0, up1, dm0, dmp1, p);          // value at p+1/2
-    double l = eq51(um1, u0, dmm1, dm0, p - 1);      // value at p-1/2
-    if (n == 0) {
-      r = r * (1.0 - eta) + (up1 - 0.5 * dmp1) * eta;
-      l = l * (1.0 - eta) + (um1 + 0.5 * dmm1) * eta;
-    }
-    r = u0 * f2 + r * (1.0 - f2);
-    l = u0 * f2 + l * (1.0 - f2);
-    if ((r - u0) * (l - u0) >= 0.0) { r = u0; l = u0; }            // equation (73)
-    const double t3 = 6.0 * (r - l) * (u0 - 0.5 * (r + l));         // equations (74), (75)
-    const double t4 = (r - l) * (r - l);
-    if (t3 > t4) l = 3.0 * u0 - 2.0 * r;
-    if (-t3 > t4) r = 3.0 * u0 - 2.0 * l;
-    store(n, l, r);
+// Interface values of variable n of the cell: interp (rec.f90:1309-1476),
+// steepening of the density (rec.f90:1661-1677), flattening and monoto
+// (rec.f90:1887-2056).  l = uri*r(c-1), r = uri*l(c).
+template <int D, class R>
+RH_DEV void ppm_var(const double* __restrict__ W, long long N, long long c, long long s, int p,
+                    const double* __restrict__ pc, long long npc, int n, R f2, R eta, R& l_out, R& r_out) {
+  // interp's width-only coefficients come from the table pc (ppm_coefficients
+  // on the host, same expressions and order): planes 0..2 of dma at q
+  // (rec.f90:1353-1355), planes 3..6 of equation (51) at q+1/2 (rec.f90:1427-1436)
+  auto dma_of = [&](R um, R u0, R up, int q) -> R {
+    const R t1a = R(pc[q]), t1b = R(pc[npc + q]), t1c = R(pc[2 * npc + q]);
+    const R sp1 = (up - u0);
+    const R sm1 = (u0 - um);
+    const R spm1 = sp1 * sm1;
+    const R da = t1a * (t1b * sp1 + t1c * sm1);
+    if (spm1 <= R(0)) return R(0);
+    const R t1 = (da < R(0)) ? R(-1) : R(1);
+    const R t1d = fminf_(absr(sm1), absr(sp1));
+    if (D == 3) return t1 * fminf_(absr(da), t1d);   // rec.f90:1414
+    return t1 * fminf_(absr(da), R(2.0) * t1d);
+  };
+  auto eq51 = [&](R uq, R uq1, R dmq, R dmq1, int q) -> R {
+    const R t2 = R(pc[3 * npc + q]) * (uq1 - uq);
+    const R t3 = R(pc[4 * npc + q]) * (uq1 - uq);
+    const R t4 = R(pc[5 * npc + q]);
+    const R t5 = R(pc[6 * npc + q]);
+    return uq + t2 + t3 - t4 * dmq1 + t5 * dmq;
+  };
+  const double* Pn = W + (long long)n * N + c;
+  const R um2 = R(Pn[-2 * s]), um1 = R(Pn[-s]), u0 = R(Pn[0]), up1 = R(Pn[s]), up2 = R(Pn[2 * s]);
+  const R dmm1 = dma_of(um2, um1, u0, p - 1);
+  const R dm0 = dma_of(um1, u0, up1, p);
+  const R dmp1 = dma_of(u0, up1, up2, p + 1);
+  R r = eq51(u0, up1, dm0, dmp1, p);          // value at p+1/2
+  R l = eq51(um1, u0, dmm1, dm0, p - 1);      // value at p-1/2
+  if (n == 0) {
+    r = r * (R(1) - eta) + (up1 - R(0.5) * dmp1) * eta;
+    l = l * (R(1) - eta) + (um1 + R(0.5) * dmm1) * eta;
   }
+  r = u0 * f2 + r * (R(1) - f2);
+  l = u0 * f2 + l * (R(1) - f2);
+  if ((r - u0) * (l - u0) >= R(0)) { r = u0; l = u0; }            // equation (73)
+  const R t3 = R(6.0) * (r - l) * (u0 - R(0.5) * (r + l));         // equations (74), (75)
+  const R t4 = (r - l) * (r - l);
+  if (t3 > t4) l = R(3.0) * u0 - R(2.0) * r;
+  if (-t3 > t4) r = R(3.0) * u0 - R(2.0) * l;
+  l_out = l;
+  r_out = r;
 }
 
 // ------------------------------------------------------------------------
-// PPM face values of every cell once (rec.f90:1261-2056 along D): planes 0..8
-// the left edge (uri*r(c-1)), 9..17 the right edge (uri*l(c)).  Cells
-// nm0..n-nm0+1 along D, the sweep's box across.
+// PPM face values of every cell once, along D: one thread per cell computes
+// f2 and eta, then each variable's edges in turn, writing E indexed by
+// interface: planes 0..8 at c-s the left edge of cell c (uri*r(c-1)), planes
+// 9..17 at c its right edge (uri*l(c)).  Cells nm0..n-nm0+1 along D, the
+// sweep's box across.  R: arithmetic and storage type of the edges.
 // ------------------------------------------------------------------------
-template <int D>
+template <int D, class R>
 __global__ void __launch_bounds__(256) k_ppm_edges(Geo G, const double* __restrict__ W, const double* __restrict__ dxs,
                                                    const double* __restrict__ xs, const double* __restrict__ pc,
-                                                   double gam, double* __restrict__ E) {
+                                                   double gam, R* __restrict__ E) {
   int i, j, k;
   const long long t = xcd_thread();
   const int hi = G.ni - G.nm0 + (D == 1), hj = G.nj - G.nm0 + (D == 2), hk = G.nk - G.nm0 + (D == 3);
@@ -220,10 +230,15 @@ __global__ void __launch_bounds__(256) k_ppm_edges(Geo G, const double* __restri
   const long long c = cidx(i, j, k, G);
   const int p = (D == 1) ? i : (D == 2) ? j : k;
   const long long npc = ((D == 1) ? G.ni : (D == 2) ? G.nj : G.nk) + 2;
-  ppm_cell<D>(W, N, c, s, p, dxs, xs, pc, npc, gam, [&](int n, double l, double r) {
-    E[(long long)n * N + c] = l;
+  R f2, eta;
+  ppm_shape<D, R>(W, N, c, s, p, dxs, xs, R(gam), f2, eta);
+#pragma unroll 1
+  for (int n = 0; n < 9; n++) {
+    R l, r;
+    ppm_var<D, R>(W, N, c, s, p, pc, npc, n, f2, eta, l, r);
+    E[(long long)n * N + c - s] = l;   // stored at the interface it belongs to (c-1/2)
     E[(long long)(9 + n) * N + c] = r;
-  });
+  }
 }
 
 // ------------------------------------------------------------------------
@@ -237,10 +252,11 @@ __global__ void __launch_bounds__(256) k_ppm_edges(Geo G, const double* __restri
 // averaging); inside the CT box fluxct then writes F itself.
 //
 // PPM's face values come from k_ppm_edges (E); MP5 and WENO5 reconstruct here.
-template <int D, int IREC>
+template <int D, int IREC, class R>
 __global__ void __launch_bounds__(256) k_sweep(Geo G, const double* __restrict__ W, const double* __restrict__ metF,
-                                               const double* __restrict__ E, double gam, double* __restrict__ F,
+                                               const R* __restrict__ E, double gam_d, double* __restrict__ F,
                                                double* __restrict__ CTsrc) {
+  const R gam = R(gam_d);
   int i, j, k;
   const long long t = xcd_thread();
   const bool in = (D == 3) ? box_index_ikj(t, G.nm0, G.ni - G.nm0, G.nm0, G.nj - G.nm0, G.nm0, G.nk - G.nm0, i, j, k)
@@ -249,50 +265,51 @@ __global__ void __launch_bounds__(256) k_sweep(Geo G, const double* __restrict__
   const long long N = G.N;
   const long long s = (D == 1) ? 1LL : (D == 2) ? (long long)G.ni : (long long)G.ni * G.nj;
   const long long c = cidx(i, j, k, G);
-  double wl[9], wr[9];
+  R wl[9], wr[9];
   if (IREC == 5) {
-    // right edge of cell c (uri*l(c)) and left edge of cell c+1 (uri*r(c))
+    // right edge of cell c (uri*l(c)) and left edge of cell c+1 (uri*r(c)),
+    // both stored at interface c+1/2's index
 #pragma unroll
-    for (int n = 0; n < 9; n++) { wl[n] = E[(long long)(9 + n) * N + c]; wr[n] = E[(long long)n * N + c + s]; }
+    for (int n = 0; n < 9; n++) { wl[n] = E[(long long)(9 + n) * N + c]; wr[n] = E[(long long)n * N + c]; }
   } else {
 #pragma unroll
     for (int n = 0; n < 9; n++) {
       const double* Pn = W + (long long)n * N + c;
-      const double a = Pn[-2 * s], b = Pn[-s], cc = Pn[0], d = Pn[s], e = Pn[2 * s], f = Pn[3 * s];
+      const R a = R(Pn[-2 * s]), b = R(Pn[-s]), cc = R(Pn[0]), d = R(Pn[s]), e = R(Pn[2 * s]), f = R(Pn[3 * s]);
       if (IREC == 6) {
-        const double sl[5] = {a, b, cc, d, e};
-        const double sr[5] = {b, cc, d, e, f};
-        wl[n] = mp5_right(sl);
-        wr[n] = mp5_left(sr);
+        const R sl[5] = {a, b, cc, d, e};
+        const R sr[5] = {b, cc, d, e, f};
+        wl[n] = mp5_right<R>(sl);
+        wr[n] = mp5_left<R>(sr);
       } else {
-        wl[n] = weno_edge(a, b, cc, d, e);
-        wr[n] = weno_edge(f, e, d, cc, b);
+        wl[n] = weno_edge<R>(a, b, cc, d, e);
+        wr[n] = weno_edge<R>(f, e, d, cc, b);
       }
     }
   }
   const Met M{metF, G.P2, q2(i, k, G)};
-  const double detg = M.detg();
-  double Ul[9], Fl[9], vpl, vml;
+  const R detg = R(M.detg());
+  R Ul[9], Fl[9], vpl, vml;
   {
-    Prel sl;
-    prelude(wl, gam, M, sl);
-    const double pw = pow(sl.de, gam);
-    tmunu_row(sl, 0, detg, pw, Ul);
-    tmunu_row(sl, D, detg, pw, Fl);
-    face_speed(sl, D, gam, M, vpl, vml);
+    PrelT<R> sl;
+    prelude<R>(wl, gam, M, sl);
+    const R pw = powr(sl.de, gam);
+    tmunu_row<R>(sl, 0, detg, pw, Ul);
+    tmunu_row<R>(sl, D, detg, pw, Fl);
+    face_speed<R>(sl, D, gam, M, vpl, vml);
   }
-  double Ur[9], Fr[9], vpr, vmr;
+  R Ur[9], Fr[9], vpr, vmr;
   {
-    Prel sr;
-    prelude(wr, gam, M, sr);
-    const double pw = pow(sr.de, gam);
-    tmunu_row(sr, 0, detg, pw, Ur);
-    tmunu_row(sr, D, detg, pw, Fr);
-    face_speed(sr, D, gam, M, vpr, vmr);
+    PrelT<R> sr;
+    prelude<R>(wr, gam, M, sr);
+    const R pw = powr(sr.de, gam);
+    tmunu_row<R>(sr, 0, detg, pw, Ur);
+    tmunu_row<R>(sr, D, detg, pw, Fr);
+    face_speed<R>(sr, D, gam, M, vpr, vmr);
   }
-  const double cmax = fmaxf_(fmaxf_(0.0, vpr), vpl);
-  const double cmin = (D == 3) ? fminf_(fminf_(0.0, vpr), vml)   // calcha.f90:107
-                               : fminf_(fminf_(0.0, vmr), vml);
+  const R cmax = fmaxf_(fmaxf_(R(0), vpr), vpl);
+  const R cmin = (D == 3) ? fminf_(fminf_(R(0), vpr), vml)   // calcha.f90:107
+                          : fminf_(fminf_(R(0), vmr), vml);
   double* Fo = F + c;
   // transverse B fluxes of this direction and their CTsrc planes
   // (0 ww(2,7) 1 ww(3,7) 2 ww(1,8) 3 ww(3,8) 4 ww(1,9) 5 ww(2,9))
@@ -303,13 +320,13 @@ __global__ void __launch_bounds__(256) k_sweep(Geo G, const double* __restrict__
 #pragma unroll
   for (int n = 0; n < 9; n++) {
     double o;
-    if (cmin > 0.0) o = Fl[n];
-    else if (cmin <= 0.0 && cmax > 0.0) {
-      const double c1 = fmaxf_(fabs(cmax), fabs(cmin));
-      o = 0.5 * (Fl[n] + Fr[n] - c1 * (Ur[n] - Ul[n]));
-    } else if (cmax <= 0.0) o = Fr[n];
+    if (cmin > R(0)) o = double(Fl[n]);
+    else if (cmin <= R(0) && cmax > R(0)) {
+      const R c1 = fmaxf_(absr(cmax), absr(cmin));
+      o = double(R(0.5) * (Fl[n] + Fr[n] - c1 * (Ur[n] - Ul[n])));
+    } else if (cmax <= R(0)) o = double(Fr[n]);
     else o = Fo[(long long)n * N];  // unordered (NaN) speeds: hlle keeps ww, then flushes it
-    if (fabs(o) < 1.0e-40) o = 0.0;
+    if (absr(o) < 1.0e-40) o = 0.0;
     if (n == nA || n == nB) {
       CTsrc[(long long)(n == nA ? pA : pB) * N + c] = o;
       if (!in_ct) Fo[(long long)n * N] = o;  // inside the CT box fluxct writes this slot
@@ -1185,27 +1202,38 @@ __global__ void k_planes_to_ww(const double* __restrict__ F, double* __restrict_
 static inline unsigned nblk(long long n, int b) { return (unsigned)((n + b - 1) / b); }
 static inline long long box(int lo, int hi) { return hi >= lo ? (long long)(hi - lo + 1) : 0; }
 
+template <class R>
+static void ppm_edges_t(const Geo& G, int dir, const double* W, const double* dxs, const double* xs, const double* pc,
+                        double gam, R* E, long long n, hipStream_t st) {
+  const unsigned nb = nblk(n, 256);
+  if (dir == 1) hipLaunchKernelGGL((k_ppm_edges<1, R>), dim3(nb), dim3(256), 0, st, G, W, dxs, xs, pc, gam, E);
+  else if (dir == 2) hipLaunchKernelGGL((k_ppm_edges<2, R>), dim3(nb), dim3(256), 0, st, G, W, dxs, xs, pc, gam, E);
+  else hipLaunchKernelGGL((k_ppm_edges<3, R>), dim3(nb), dim3(256), 0, st, G, W, dxs, xs, pc, gam, E);
+}
+
 hipError_t launch_ppm_edges(const Geo& G, int dir, const double* W, const double* dxs, const double* xs,
-                            const double* pc, double gam, double* E, hipStream_t st) {
+                            const double* pc, double gam, void* E, bool mixed, hipStream_t st) {
   const long long n = box(G.nm0, G.ni - G.nm0 + (dir == 1)) * box(G.nm0, G.nj - G.nm0 + (dir == 2)) *
                       box(G.nm0, G.nk - G.nm0 + (dir == 3));
   if (n == 0) return hipSuccess;
-  const unsigned nb = nblk(n, 256);
-  if (dir == 1) hipLaunchKernelGGL((k_ppm_edges<1>), dim3(nb), dim3(256), 0, st, G, W, dxs, xs, pc, gam, E);
-  else if (dir == 2) hipLaunchKernelGGL((k_ppm_edges<2>), dim3(nb), dim3(256), 0, st, G, W, dxs, xs, pc, gam, E);
-  else hipLaunchKernelGGL((k_ppm_edges<3>), dim3(nb), dim3(256), 0, st, G, W, dxs, xs, pc, gam, E);
+  if (mixed) ppm_edges_t<float>(G, dir, W, dxs, xs, pc, gam, (float*)E, n, st);
+  else ppm_edges_t<double>(G, dir, W, dxs, xs, pc, gam, (double*)E, n, st);
   return hipGetLastError();
 }
 
-hipError_t launch_sweep(const Geo& G, int dir, int irec, const double* W, const double* metF, const double* E,
-                        double gam, double* F, double* CTsrc, hipStream_t st) {
+hipError_t launch_sweep(const Geo& G, int dir, int irec, const double* W, const double* metF, const void* E,
+                        double gam, double* F, double* CTsrc, bool mixed, hipStream_t st) {
   const long long n = box(G.nm0, G.ni - G.nm0) * box(G.nm0, G.nj - G.nm0) * box(G.nm0, G.nk - G.nm0);
   if (n == 0) return hipSuccess;
   const unsigned nb = nblk(n, 256);
-#define L(Dd, R) hipLaunchKernelGGL((k_sweep<Dd, R>), dim3(nb), dim3(256), 0, st, G, W, metF, E, gam, F, CTsrc)
-  if (irec == 5) { if (dir == 1) L(1, 5); else if (dir == 2) L(2, 5); else L(3, 5); }
-  else if (irec == 6) { if (dir == 1) L(1, 6); else if (dir == 2) L(2, 6); else L(3, 6); }
-  else { if (dir == 1) L(1, 7); else if (dir == 2) L(2, 7); else L(3, 7); }
+#define L(Dd, Rc, T) \
+  hipLaunchKernelGGL((k_sweep<Dd, Rc, T>), dim3(nb), dim3(256), 0, st, G, W, metF, (const T*)E, gam, F, CTsrc)
+#define LT(Dd, Rc) \
+  do { if (mixed) L(Dd, Rc, float); else L(Dd, Rc, double); } while (0)
+  if (irec == 5) { if (dir == 1) LT(1, 5); else if (dir == 2) LT(2, 5); else LT(3, 5); }
+  else if (irec == 6) { if (dir == 1) LT(1, 6); else if (dir == 2) LT(2, 6); else LT(3, 6); }
+  else { if (dir == 1) LT(1, 7); else if (dir == 2) LT(2, 7); else LT(3, 7); }
+#undef LT
 #undef L
   return hipGetLastError();
 }

@@ -57,11 +57,13 @@ struct FixArgs {
 
 // PPM face values of every cell along dir into E (18 planes: left edges, right edges).
 // pc: the width-only interp coefficients of the direction (7 planes of n+2, ppm_coefficients).
+// mixed: the reconstruction and the face arithmetic in float (E holds
+// floats), else double.
 hipError_t launch_ppm_edges(const Geo& G, int dir, const double* W, const double* dxs, const double* xs,
-                            const double* pc, double gam, double* E, hipStream_t st);
+                            const double* pc, double gam, void* E, bool mixed, hipStream_t st);
 // Fluxes at the interfaces along dir; E = the PPM edges (irec 5), unused otherwise.
-hipError_t launch_sweep(const Geo& G, int dir, int irec, const double* W, const double* metF, const double* E,
-                        double gam, double* F, double* CTsrc, hipStream_t st);
+hipError_t launch_sweep(const Geo& G, int dir, int irec, const double* W, const double* metF, const void* E,
+                        double gam, double* F, double* CTsrc, bool mixed, hipStream_t st);
 hipError_t launch_fluxct(const Geo& G, const double* CTsrc, double* F1, double* F2, double* F3, hipStream_t st);
 // RK update + sources into A.Ukeep (every interior cell), then recov1a into A.Wpre.
 hipError_t launch_update(const Geo& G, int stage, const StageArgs& A, hipStream_t st);

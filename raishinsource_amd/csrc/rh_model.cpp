@@ -309,10 +309,10 @@ bool stage_sweeps(raishin_model* m, const double* W, double* F) {
   for (int d = 1; d <= 3; d++) {
     if (irec == 5) {
       TimedLaunch t(m, K_EDGES);
-      HIPCHK(launch_ppm_edges(G, d, W, m->dxs[d - 1], m->xs[d - 1], m->ppmc[d - 1], m->p.gam, m->E, m->st));
+      HIPCHK(launch_ppm_edges(G, d, W, m->dxs[d - 1], m->xs[d - 1], m->ppmc[d - 1], m->p.gam, m->E, m->mixed, m->st));
     }
     TimedLaunch t(m, K_SWEEP1 + d - 1);
-    HIPCHK(launch_sweep(G, d, irec, W, m->met[d], m->E, m->p.gam, F + (d - 1) * 9 * N, m->CT, m->st));
+    HIPCHK(launch_sweep(G, d, irec, W, m->met[d], m->E, m->p.gam, F + (d - 1) * 9 * N, m->CT, m->mixed, m->st));
   }
   return true;
 }

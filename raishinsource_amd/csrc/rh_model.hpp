@@ -32,6 +32,7 @@ struct raishin_model {
   rh::Geo G;
   hipStream_t st = nullptr;
   bool need_wo = false;
+  bool mixed = false;  // reconstruction + face fluxes in float (raishin_model_set_precision)
   // state (9 planes each)
   double *W = nullptr, *Wb = nullptr, *Wpre = nullptr, *Wo = nullptr;
   double *Ucur = nullptr, *Uh = nullptr, *Us = nullptr, *Unext = nullptr;

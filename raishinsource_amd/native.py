@@ -49,7 +49,7 @@ EXPORTS = (
     "raishin_model_counters", "raishin_model_reset_counters", "raishin_model_sync", "raishin_model_stream",
     "raishin_cdtcfl4", "raishin_init", "raishin_grid", "raishin_coord1", "raishin_calchrist",
     "raishin_mdtorus1", "raishin_model_set_timing", "raishin_model_get_timing",
-    "raishin_model_debug_stage",
+    "raishin_model_debug_stage", "raishin_model_set_precision",
 )
 
 # reference-named entry points (include/raishin_ref.h)
@@ -102,6 +102,7 @@ def lib():
         L.raishin_model_get_timing.argtypes = [vp, C.POINTER(C.c_double), C.POINTER(C.c_longlong), C.c_int]
         L.raishin_model_get_timing.restype = C.c_int
         L.raishin_model_debug_stage.argtypes = [vp, C.c_int, vp, vp]
+        L.raishin_model_set_precision.argtypes = [vp, C.c_int]
         _lib = L
     return _lib
 
@@ -248,6 +249,12 @@ class Model:
         lib().raishin_model_debug_stage(self.h, st, _p(ww), _p(wp))
         check()
         return ww, wp
+
+    def set_precision(self, mixed):
+        """False: FP64 throughout (the reference's arithmetic); True:
+        reconstruction and face fluxes in FP32, everything else FP64."""
+        lib().raishin_model_set_precision(self.h, 1 if mixed else 0)
+        check()
 
     def set_timing(self, on=True):
         lib().raishin_model_set_timing(self.h, 1 if on else 0)
