@@ -569,7 +569,16 @@ RH_DEV void recov_cell(const double u[9], const double wold[9], const Met& M, do
 // (rkt.f90:2-180) plus its source add (rkt.f90:183-320), then recov1a.
 // STAGE: 1 rk2fst, 2 rk2snd, 3 rk3snd, 4 rk3trd.
 // ------------------------------------------------------------------------
-template <int STAGE>
+// Plane of Gamma^l_{ab} in the Christoffel tables (christ(l,a,b) at l+4a+16b).
+// SYM: the tables are bit-for-bit symmetric in (a,b) (checked on the host),
+// so (a,b) and (b,a) read one plane and the compiler loads it once.
+template <bool SYM>
+RH_DEV long long chr_plane(int l, int a, int b) {
+  if (SYM && a > b) { const int t = a; a = b; b = t; }
+  return (long long)(l + 4 * a + 16 * b);
+}
+
+template <int STAGE, bool SYM>
 __global__ void __launch_bounds__(256) k_rk(Geo G, StageArgs A) {
   int i, j, k;
   const long long t = xcd_thread();
@@ -606,10 +615,10 @@ __global__ void __launch_bounds__(256) k_rk(Geo G, StageArgs A) {
       for (int m = 0; m < 4; m++) {
         const double dl = (m == n) ? 1.0 : 0.0;
         const double tv = (s.roh + s.bbsq) * s.ucon[m] * s.ucov[n] + (s.pr + 0.5 * s.bbsq) * dl - s.bb[m] * s.bbcov[n];
-        ta = ta + tv * A.chr[(long long)(n + 4 * 1 + 16 * m) * G.P2 + q];
-        tb = tb + tv * A.chr[(long long)(n + 4 * 2 + 16 * m) * G.P2 + q];
-        tc = tc + tv * A.chr[(long long)(n + 4 * 3 + 16 * m) * G.P2 + q];
-        td = td + tv * A.chr[(long long)(n + 4 * 0 + 16 * m) * G.P2 + q];
+        ta = ta + tv * A.chr[chr_plane<SYM>(n, 1, m) * G.P2 + q];
+        tb = tb + tv * A.chr[chr_plane<SYM>(n, 2, m) * G.P2 + q];
+        tc = tc + tv * A.chr[chr_plane<SYM>(n, 3, m) * G.P2 + q];
+        td = td + tv * A.chr[chr_plane<SYM>(n, 0, m) * G.P2 + q];
       }
     const double dg = M.detg();
     sf[0] = dg * ta; sf[1] = dg * tb; sf[2] = dg * tc; sf[3] = dg * td;
@@ -806,7 +815,7 @@ __global__ void __launch_bounds__(256) k_bnd(Geo G, PostArgs A) {
   for (int n = 0; n < 9; n++) A.Wghost[(long long)n * N + c] = bnd_value(B, n, i, j, k);
 }
 
-// infchk + atmosphere1a + pminmax2b on every cell, after k_bnd; caluu1 follows (k_caluu1).
+// infchk + atmosphere1a + pminmax2b + caluu1 on every cell, after k_bnd.
 __global__ void __launch_bounds__(256) k_post(Geo G, PostArgs A) {
   const long long t = xcd_thread();
   int i, j, k;
@@ -913,6 +922,15 @@ __global__ void __launch_bounds__(256) k_post(Geo G, PostArgs A) {
   count_wave(A.cpart, C_PMMFLOOR, hit);
 #pragma unroll
   for (int n = 0; n < 9; n++) A.Wout[(long long)n * N + c] = w[n];
+  // caluu1 (caluu.f90:2-126): conserved variables of the final primitives
+  {
+    Prel s2;
+    prelude(w, gam, M, s2);
+    double u[9];
+    tmunu_row(s2, 0, M.detg(), pow(s2.de, gam), u);
+#pragma unroll
+    for (int n = 0; n < 9; n++) A.Uout[(long long)n * N + c] = u[n];
+  }
 }
 
 // ------------------------------------------------------------------------
@@ -1249,10 +1267,14 @@ hipError_t launch_update(const Geo& G, int stage, const StageArgs& A, hipStream_
   const long long n = box(G.nm1 + 1, G.ni - G.nm1) * box(G.nm1 + 1, G.nj - G.nm1) * box(G.nm1 + 1, G.nk - G.nm1);
   if (n == 0) return hipSuccess;
   const unsigned nb = nblk(n, 256);
-  if (stage == 1) hipLaunchKernelGGL(k_rk<1>, dim3(nb), dim3(256), 0, st, G, A);
-  else if (stage == 2) hipLaunchKernelGGL(k_rk<2>, dim3(nb), dim3(256), 0, st, G, A);
-  else if (stage == 3) hipLaunchKernelGGL(k_rk<3>, dim3(nb), dim3(256), 0, st, G, A);
-  else hipLaunchKernelGGL(k_rk<4>, dim3(nb), dim3(256), 0, st, G, A);
+#define L(S) \
+  do { if (A.chr_sym) hipLaunchKernelGGL((k_rk<S, true>), dim3(nb), dim3(256), 0, st, G, A); \
+       else hipLaunchKernelGGL((k_rk<S, false>), dim3(nb), dim3(256), 0, st, G, A); } while (0)
+  if (stage == 1) L(1);
+  else if (stage == 2) L(2);
+  else if (stage == 3) L(3);
+  else L(4);
+#undef L
   return hipGetLastError();
 }
 
@@ -1273,7 +1295,6 @@ hipError_t launch_post(const Geo& G, const PostArgs& A, hipStream_t st) {
   const long long ng = G.N - (long long)(G.ni - 2 * g) * (G.nj - 2 * g) * (G.nk - 2 * g);
   if (ng > 0) hipLaunchKernelGGL(k_bnd, dim3(nblk(ng, 256)), dim3(256), 0, st, G, A);
   hipLaunchKernelGGL(k_post, dim3(nblk(G.N, 256)), dim3(256), 0, st, G, A);
-  hipLaunchKernelGGL(k_caluu1, dim3(nblk(G.N, 256)), dim3(256), 0, st, G, (const double*)A.Wout, A.metC, A.gam, A.Uout);
   return hipGetLastError();
 }
 

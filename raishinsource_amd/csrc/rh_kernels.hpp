@@ -24,6 +24,7 @@ struct StageArgs {
   long long carry_cap;
   double gam, cfl, dmin, pmin;
   int iter;
+  bool chr_sym;                    // Christoffel tables symmetric in the lower indices
 };
 
 struct PostArgs {

@@ -252,6 +252,13 @@ bool build_model(raishin_model* m, const raishin_pram* p, const double* x1, cons
           }
         }
     HIPCHK(hipMemcpy(m->chr, ct.data(), ct.size() * 8, hipMemcpyHostToDevice));
+    // Gamma^l_{ab} = Gamma^l_{ba} bit for bit lets the source kernel read each pair once
+    m->chr_sym = true;
+    for (int l = 0; l < 4 && m->chr_sym; l++)
+      for (int a = 0; a < 4 && m->chr_sym; a++)
+        for (int b = a + 1; b < 4 && m->chr_sym; b++)
+          m->chr_sym = std::memcmp(&ct[(size_t)(l + 4 * a + 16 * b) * G.P2], &ct[(size_t)(l + 4 * b + 16 * a) * G.P2],
+                                   (size_t)G.P2 * 8) == 0;
   }
   // 1D grids; dxs[d] = rec2's dummy dx*b(d) = actual dx*b(d-1) (rec.f90:58), clamped at the ends
   const double* xsrc[3] = {x1, x2, x3};
@@ -335,6 +342,7 @@ bool stage_update(raishin_model* m, int rkstage, const double* W, const double* 
   A.r15 = m->r15; A.r25 = m->r25; A.dtbits = m->dtbits; A.Wpre = m->Wpre; A.wgout = m->wgout; A.Ukeep = Uout;
   A.cnt = m->cnt; A.cpart = m->cpart; A.carry = m->carry; A.cflag = m->cflag; A.carry_cap = m->carry_cap;
   A.gam = m->p.gam; A.cfl = m->p.cfl; A.dmin = m->p.dmin; A.pmin = m->p.pmin; A.iter = m->p.iter;
+  A.chr_sym = m->chr_sym;
   if (hd) { A.ak1 = hd->ak1; A.ak2 = hd->ak2; A.ak3 = hd->ak3; A.dt_host = hd->dt; }
   { TimedLaunch t(m, K_UPDATE); HIPCHK(launch_update(G, rkstage, A, m->st)); }
   { TimedLaunch t(m, K_RECOV); HIPCHK(launch_recov(G, A, m->st)); }

@@ -32,7 +32,8 @@ struct raishin_model {
   rh::Geo G;
   hipStream_t st = nullptr;
   bool need_wo = false;
-  bool mixed = false;  // reconstruction + face fluxes in float (raishin_model_set_precision)
+  bool mixed = false;    // reconstruction + face fluxes in float (raishin_model_set_precision)
+  bool chr_sym = false;  // Christoffel tables symmetric in the lower indices, bit for bit
   // state (9 planes each)
   double *W = nullptr, *Wb = nullptr, *Wpre = nullptr, *Wo = nullptr;
   double *Ucur = nullptr, *Uh = nullptr, *Us = nullptr, *Unext = nullptr;
