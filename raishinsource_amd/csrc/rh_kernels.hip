@@ -158,17 +158,39 @@ RH_DEV void ppm_shape(const double* __restrict__ W, long long N, long long c, lo
   eta_out = eta;
 }
 
-// Interface values of variable n of the cell: interp (rec.f90:1309-1476),
-// steepening of the density (rec.f90:1661-1677), flattening and monoto
-// (rec.f90:1887-2056).  l = uri*r(c-1), r = uri*l(c).
+// interp's width-only coefficients of a cell at index p (ppm_coefficients on
+// the host, same expressions and order): t1a,t1b,t1c of dma at q = p-1, p,
+// p+1 (rec.f90:1353-1355) and the four of equation (51) at q+1/2 for q = p
+// and p-1 (rec.f90:1427-1436).  The same for every variable: loaded once.
+template <class R>
+struct PpmCoef {
+  R ta[3], tb[3], tc[3];
+  R e2[2], e3[2], e4[2], e5[2];
+};
+
+template <class R>
+RH_DEV void ppm_coef(const double* __restrict__ pc, long long npc, int p, PpmCoef<R>& k) {
+#pragma unroll
+  for (int o = 0; o < 3; o++) {
+    const int q = p - 1 + o;
+    k.ta[o] = R(pc[q]); k.tb[o] = R(pc[npc + q]); k.tc[o] = R(pc[2 * npc + q]);
+  }
+#pragma unroll
+  for (int o = 0; o < 2; o++) {
+    const int q = p - o;
+    k.e2[o] = R(pc[3 * npc + q]); k.e3[o] = R(pc[4 * npc + q]);
+    k.e4[o] = R(pc[5 * npc + q]); k.e5[o] = R(pc[6 * npc + q]);
+  }
+}
+
+// Interface values of variable n of the cell from its 5-point stencil
+// u = (u(p-2) .. u(p+2)): interp (rec.f90:1309-1476), steepening of the
+// density (rec.f90:1661-1677), flattening and monoto (rec.f90:1887-2056).
+// l = uri*r(c-1), r = uri*l(c).
 template <int D, class R>
-RH_DEV void ppm_var(const double* __restrict__ W, long long N, long long c, long long s, int p,
-                    const double* __restrict__ pc, long long npc, int n, R f2, R eta, R& l_out, R& r_out) {
-  // interp's width-only coefficients come from the table pc (ppm_coefficients
-  // on the host, same expressions and order): planes 0..2 of dma at q
-  // (rec.f90:1353-1355), planes 3..6 of equation (51) at q+1/2 (rec.f90:1427-1436)
-  auto dma_of = [&](R um, R u0, R up, int q) -> R {
-    const R t1a = R(pc[q]), t1b = R(pc[npc + q]), t1c = R(pc[2 * npc + q]);
+RH_DEV void ppm_var(const R u[5], const PpmCoef<R>& k, int n, R f2, R eta, R& l_out, R& r_out) {
+  auto dma_of = [&](R um, R u0, R up, int o) -> R {
+    const R t1a = k.ta[o], t1b = k.tb[o], t1c = k.tc[o];
     const R sp1 = (up - u0);
     const R sm1 = (u0 - um);
     const R spm1 = sp1 * sm1;
@@ -179,20 +201,17 @@ RH_DEV void ppm_var(const double* __restrict__ W, long long N, long long c, long
     if (D == 3) return t1 * fminf_(absr(da), t1d);   // rec.f90:1414
     return t1 * fminf_(absr(da), R(2.0) * t1d);
   };
-  auto eq51 = [&](R uq, R uq1, R dmq, R dmq1, int q) -> R {
-    const R t2 = R(pc[3 * npc + q]) * (uq1 - uq);
-    const R t3 = R(pc[4 * npc + q]) * (uq1 - uq);
-    const R t4 = R(pc[5 * npc + q]);
-    const R t5 = R(pc[6 * npc + q]);
-    return uq + t2 + t3 - t4 * dmq1 + t5 * dmq;
+  auto eq51 = [&](R uq, R uq1, R dmq, R dmq1, int o) -> R {
+    const R t2 = k.e2[o] * (uq1 - uq);
+    const R t3 = k.e3[o] * (uq1 - uq);
+    return uq + t2 + t3 - k.e4[o] * dmq1 + k.e5[o] * dmq;
   };
-  const double* Pn = W + (long long)n * N + c;
-  const R um2 = R(Pn[-2 * s]), um1 = R(Pn[-s]), u0 = R(Pn[0]), up1 = R(Pn[s]), up2 = R(Pn[2 * s]);
-  const R dmm1 = dma_of(um2, um1, u0, p - 1);
-  const R dm0 = dma_of(um1, u0, up1, p);
-  const R dmp1 = dma_of(u0, up1, up2, p + 1);
-  R r = eq51(u0, up1, dm0, dmp1, p);          // value at p+1/2
-  R l = eq51(um1, u0, dmm1, dm0, p - 1);      // value at p-1/2
+  const R um2 = u[0], um1 = u[1], u0 = u[2], up1 = u[3], up2 = u[4];
+  const R dmm1 = dma_of(um2, um1, u0, 0);
+  const R dm0 = dma_of(um1, u0, up1, 1);
+  const R dmp1 = dma_of(u0, up1, up2, 2);
+  R r = eq51(u0, up1, dm0, dmp1, 0);          // value at p+1/2
+  R l = eq51(um1, u0, dmm1, dm0, 1);          // value at p-1/2
   if (n == 0) {
     r = r * (R(1) - eta) + (up1 - R(0.5) * dmp1) * eta;
     l = l * (R(1) - eta) + (um1 + R(0.5) * dmm1) * eta;
@@ -230,14 +249,28 @@ __global__ void __launch_bounds__(256) k_ppm_edges(Geo G, const double* __restri
   const long long c = cidx(i, j, k, G);
   const int p = (D == 1) ? i : (D == 2) ? j : k;
   const long long npc = ((D == 1) ? G.ni : (D == 2) ? G.nj : G.nk) + 2;
+  PpmCoef<R> kc;
+  ppm_coef<R>(pc, npc, p, kc);
   R f2, eta;
   ppm_shape<D, R>(W, N, c, s, p, dxs, xs, R(gam), f2, eta);
+  // one variable at a time, the next variable's stencil loads in flight
+  // while the current one is computed
+  R cur[5], nxt[5];
+#pragma unroll
+  for (int o = 0; o < 5; o++) cur[o] = R(W[c + (o - 2) * s]);
 #pragma unroll 1
   for (int n = 0; n < 9; n++) {
+    if (n < 8) {
+      const double* Pn = W + (long long)(n + 1) * N + c;
+#pragma unroll
+      for (int o = 0; o < 5; o++) nxt[o] = R(Pn[(o - 2) * s]);
+    }
     R l, r;
-    ppm_var<D, R>(W, N, c, s, p, pc, npc, n, f2, eta, l, r);
+    ppm_var<D, R>(cur, kc, n, f2, eta, l, r);
     E[(long long)n * N + c - s] = l;   // stored at the interface it belongs to (c-1/2)
     E[(long long)(9 + n) * N + c] = r;
+#pragma unroll
+    for (int o = 0; o < 5; o++) cur[o] = nxt[o];
   }
 }
 
