@@ -108,6 +108,9 @@ def main():
                     help="interior phi cells of the CPU-baseline sample (0: skip the CPU baseline)")
     ap.add_argument("--mixed-steps", type=int, default=100,
                     help="steps of the FP64-vs-mixed error comparison (0: skip the mixed-precision run)")
+    ap.add_argument("--ensemble", type=int, default=64,
+                    help="models of the 2D-torus ensemble run (0: skip)")
+    ap.add_argument("--ensemble-steps", type=int, default=500, help="steps per ensemble model (even)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {args.gpus}): every box here has one MI355X and "
@@ -212,8 +215,95 @@ def main():
     }
     if args.mixed_steps > 0:
         line["mixed"] = mixed_precision(p, g, m, uri, args, el / args.steps)
+    if args.ensemble > 0:
+        line["ensemble"] = ensemble(args)
     print(json.dumps(line))
     return 0
+
+
+def ensemble(args):
+    """Independent 2D tori of uneven size (r x theta drawn from 64x64,
+    128x128, 256x128, 512x256; spin U[0,0.99]; beta_min from 10, 100, 1000;
+    r_in U[6,10], r_max = r_in + 6; PPM, RK2, CFL 0.4; seeded).  The same
+    steps are run one model at a time (each model's steps launched eagerly,
+    then the next model) and all models together (two-step HIP graphs queued
+    round robin, one stream per model); results must be bit-identical."""
+    from dataclasses import replace
+
+    import numpy as np
+
+    from raishinsource_amd import configs, native
+
+    rng = np.random.default_rng(2024)
+    sizes = [(64, 64), (128, 128), (256, 128), (512, 256)]
+    steps = max(2, args.ensemble_steps - args.ensemble_steps % 2)
+    t_set = time.perf_counter()
+    models, states, zones = [], [], 0
+    for q in range(args.ensemble):
+        ni, nk = sizes[rng.integers(len(sizes))]
+        a = float(rng.uniform(0.0, 0.99))
+        beta = float(rng.choice([10.0, 100.0, 1000.0]))
+        rin = float(rng.uniform(6.0, 10.0))
+        p = replace(configs.C1, name=f"ens{q}", imax=ni + 2 * configs.C1.nm1, kmax=nk + 2 * configs.C1.nm1,
+                    akm=a, xmin=None).resolved()
+        g = native.setup_grid(p)
+        m = native.setup_metric(p, g)
+        uri = native.setup_torus(p, g, m, rin=rin, rmax=rin + 6.0, beta=beta, noise=0.02, seed=q)
+        mod = native.Model(p, g, m)
+        models.append(mod)
+        states.append(uri)
+        zones += ni * nk
+    t_set = time.perf_counter() - t_set
+    print(f"[bench] ensemble: {len(models)} models set up in {t_set:.1f}s", file=sys.stderr, flush=True)
+    for mod, uri in zip(models, states):
+        mod.set_state(uri)
+        mod.sync()
+    t0 = time.perf_counter()
+    for mod in models:
+        mod.step(steps, sync=True)
+    t_seq = time.perf_counter() - t0
+    ref = [mod.get_state()[0] for mod in models]
+    for mod, uri in zip(models, states):
+        mod.set_state(uri)
+        mod.step_graph(1)  # capture (and run) two steps, then start again
+        mod.set_state(uri)
+        mod.sync()
+    t0 = time.perf_counter()
+    native.step_models_graph(models, steps // 2)
+    t_conc = time.perf_counter() - t0
+    same = all(np.array_equal(np.asarray(r).view(np.uint64), np.asarray(mod.get_state()[0]).view(np.uint64))
+               for r, mod in zip(ref, models))
+    # the mixed FP32/FP64 form of the same ensemble, together
+    for mod, uri in zip(models, states):
+        mod.set_precision(True)
+        mod.set_state(uri)
+        mod.step_graph(1)
+        mod.set_state(uri)
+        mod.sync()
+    t0 = time.perf_counter()
+    native.step_models_graph(models, steps // 2)
+    t_mix = time.perf_counter() - t0
+    l1 = []
+    for r, mod in zip(ref, models):
+        d = np.asarray(mod.get_state()[0])[0]
+        r0 = np.asarray(r)[0]
+        l1.append(float(np.abs(d - r0).sum() / np.abs(r0).sum()))
+    for mod in models:
+        mod.close()
+    print(f"[bench] ensemble: {steps} steps, one at a time {t_seq:.2f}s, together {t_conc:.2f}s, "
+          f"mixed together {t_mix:.2f}s", file=sys.stderr, flush=True)
+    return {
+        "models": len(models),
+        "steps_per_model": steps,
+        "interior_zones": zones,
+        "one_at_a_time": {"value": zones * steps / t_seq, "unit": "zone-updates/s", "seconds": t_seq},
+        "together_graphs": {"value": zones * steps / t_conc, "unit": "zone-updates/s", "seconds": t_conc},
+        "speedup": t_seq / t_conc,
+        "bitwise_same": bool(same),
+        "mixed_together_graphs": {"value": zones * steps / t_mix, "unit": "zone-updates/s", "seconds": t_mix,
+                                  "max_rel_l1_density": max(l1)},
+        "setup_seconds": t_set,
+    }
 
 
 def mixed_precision(p, g, m, uri, args, s_per_step_fp64):
@@ -229,11 +319,13 @@ def mixed_precision(p, g, m, uri, args, s_per_step_fp64):
     model.set_precision(True)
     for _ in range(args.warmup):
         model.step(1, sync=False)
+    model.set_timing(True)
     model.sync()
     t0 = time.perf_counter()
     model.step(args.steps, sync=False)
     model.sync()
     el = time.perf_counter() - t0
+    kt = model.get_timing()
     model.close()
     nint = (p.imax - 2 * p.nm1) * (p.jmax - 2 * p.nm1) * (p.kmax - 2 * p.nm1)
     print(f"[bench] mixed: timed {args.steps} steps: {el:.3f}s", file=sys.stderr, flush=True)
@@ -261,6 +353,7 @@ def mixed_precision(p, g, m, uri, args, s_per_step_fp64):
         "ms_per_step": el / args.steps * 1e3,
         "speedup_vs_fp64": (s_per_step_fp64 * args.steps) / el,
         "dtype": "f32 reconstruction + face fluxes, f64 elsewhere",
+        "kernel_avg_ms": {k: round(v[0] / max(v[1], 1), 4) for k, v in kt.items()},
         "error_steps": args.mixed_steps,
         "rel_l1_vs_fp64": l1,
         "rel_linf_vs_fp64": linf,

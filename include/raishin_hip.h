@@ -77,6 +77,16 @@ void raishin_model_get_state(raishin_model *m, double *uri, double *uu, double *
  * *time and *dt_last are written when non-NULL (that synchronises). */
 void raishin_model_step(raishin_model *m, int nsteps, double *time, double *dt_last);
 
+/* 2*npairs steps through a HIP graph of two steps, captured on first use (and
+ * again after raishin_model_step ran an odd number of steps or the precision
+ * changed): one graph launch per two steps instead of ~40 kernel launches per
+ * step.  Same results as raishin_model_step. */
+void raishin_model_step_graph(raishin_model *m, int npairs, double *time, double *dt_last);
+/* Many models, each on its own stream: 2*npairs steps of every model, graph
+ * launches queued round robin so the models run concurrently.  Asynchronous:
+ * raishin_model_sync or raishin_model_get_state waits for a model. */
+void raishin_models_step_graph(raishin_model *const *models, int nmodels, int npairs);
+
 /* Only the CFL reduction (cdtcfl4 + MIN): dtcfl and dt1..dt3 of the current state. */
 void raishin_model_cdtcfl4(raishin_model *m, double *dtcfl, double *dt1, double *dt2, double *dt3);
 

@@ -19,7 +19,7 @@ ARCH = os.environ.get("RAISHIN_OFFLOAD_ARCH", "gfx950")
 HIP_FLAGS = ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}", "-ffp-contract=off",
              "-fno-fast-math", "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
              "-Wno-unused-command-line-argument"]
-HOST_FLAGS = ["-O2", "-std=c++17", "-fPIC", "-ffp-contract=off", "-fno-fast-math"]
+HOST_FLAGS = ["-O2", "-std=c++17", "-fPIC", "-ffp-contract=off", "-fno-fast-math", "-fopenmp"]
 
 FC = os.environ.get("FC_FLANG", "/opt/rocm/lib/llvm/bin/flang")
 FORTRAN = ["raishin_c.f90", "raishin_wrappers.f90", "raishin_main.f90"]
@@ -51,7 +51,7 @@ def build(verbose=False):
         if verbose and out:
             print(out)
         objs.append(o)
-    _run([HIPCC, "-shared", "-fPIC", f"--offload-arch={ARCH}", "-o", LIB] + objs)
+    _run([HIPCC, "-shared", "-fPIC", f"--offload-arch={ARCH}", "-o", LIB] + objs + ["-fopenmp"])
     build_fortran(verbose)
     return LIB
 

@@ -56,6 +56,8 @@ raishin_model* raishin_model_create(const raishin_pram* p, const double* x1, con
 void raishin_model_destroy(raishin_model* m) {
   if (!m) return;
   (void)hipStreamSynchronize(m->st);
+  if (m->gexec) (void)hipGraphExecDestroy(m->gexec);
+  if (m->graph) (void)hipGraphDestroy(m->graph);
   for (int k = 0; k < K_NKINDS; k++)
     for (auto& pr : m->ev[k]) { (void)hipEventDestroy(pr.first); (void)hipEventDestroy(pr.second); }
   dfree_all(m);
@@ -124,6 +126,86 @@ void raishin_model_step(raishin_model* m, int nsteps, double* time, double* dt_l
     if (time) *time = tdt[0];
     if (dt_last) *dt_last = tdt[1];
   }
+}
+
+}  // extern "C"
+
+namespace {
+
+// Capture two steps into a graph.  Each stage swaps the primitive buffers and
+// each step swaps the conserved buffers once on the host; after two steps
+// every pointer is back where it started, so the graph replays.  An eager
+// step in between leaves the conserved buffers (and, for RK3, the primitive
+// buffers) swapped, and the graph is captured again.
+bool capture_pair(raishin_model* m) {
+  if (m->gexec) { (void)hipGraphExecDestroy(m->gexec); m->gexec = nullptr; }
+  if (m->graph) { (void)hipGraphDestroy(m->graph); m->graph = nullptr; }
+  const bool timing = m->timing;
+  m->timing = false;
+  HIPCHK(hipStreamBeginCapture(m->st, hipStreamCaptureModeThreadLocal));
+  const double* W0 = m->W;
+  const double* U0 = m->Ucur;
+  const bool ok = run_step(m) && run_step(m);
+  hipGraph_t g = nullptr;
+  const hipError_t e = hipStreamEndCapture(m->st, &g);
+  m->timing = timing;
+  if (!ok || e != hipSuccess) {
+    if (g) (void)hipGraphDestroy(g);
+    if (e != hipSuccess) set_error("raishin_model_step_graph: capture failed: %s", hipGetErrorString(e));
+    return false;
+  }
+  if (m->W != W0 || m->Ucur != U0) {
+    (void)hipGraphDestroy(g);
+    set_error("raishin_model_step_graph: buffers did not return");
+    return false;
+  }
+  HIPCHK(hipGraphInstantiate(&m->gexec, g, nullptr, nullptr, 0));
+  m->graph = g;
+  m->graph_W = m->W;
+  m->graph_U = m->Ucur;
+  m->graph_mixed = m->mixed;
+  return true;
+}
+
+bool read_time(raishin_model* m, double* time, double* dt_last, const char* who) {
+  if (!time && !dt_last) return true;
+  double tdt[2] = {0.0, 0.0};
+  if (hipMemcpyAsync(tdt, m->tdt, 16, hipMemcpyDeviceToHost, m->st) != hipSuccess ||
+      hipStreamSynchronize(m->st) != hipSuccess) {
+    set_error("%s: synchronisation failed", who);
+    return false;
+  }
+  if (time) *time = tdt[0];
+  if (dt_last) *dt_last = tdt[1];
+  return true;
+}
+
+bool graph_ready(raishin_model* m) {
+  if (m->gexec && m->graph_W == m->W && m->graph_U == m->Ucur && m->graph_mixed == m->mixed) return true;
+  return capture_pair(m);
+}
+
+}  // namespace
+
+extern "C" {
+
+void raishin_model_step_graph(raishin_model* m, int npairs, double* time, double* dt_last) {
+  if (!graph_ready(m)) return;
+  for (int q = 0; q < npairs; q++)
+    if (hipGraphLaunch(m->gexec, m->st) != hipSuccess) { set_error("raishin_model_step_graph: launch failed"); return; }
+  (void)read_time(m, time, dt_last, "raishin_model_step_graph");
+}
+
+void raishin_models_step_graph(raishin_model* const* ms, int nmodels, int npairs) {
+  for (int a = 0; a < nmodels; a++)
+    if (!graph_ready(ms[a])) return;
+  // round robin: every model's next graph is queued before any model's second
+  for (int q = 0; q < npairs; q++)
+    for (int a = 0; a < nmodels; a++)
+      if (hipGraphLaunch(ms[a]->gexec, ms[a]->st) != hipSuccess) {
+        set_error("raishin_models_step_graph: launch failed");
+        return;
+      }
 }
 
 void raishin_model_cdtcfl4(raishin_model* m, double* dtcfl, double* dt1, double* dt2, double* dt3) {

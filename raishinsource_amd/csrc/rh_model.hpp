@@ -55,6 +55,13 @@ struct raishin_model {
   long long* cfl_last = nullptr;
   double* cfl_min = nullptr;
   double* tdt = nullptr;  // [0] time [1] last dt
+  // two steps captured as a graph (raishin_model_step_graph); valid while the
+  // buffer pointers and the precision mode are those it was captured with
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t gexec = nullptr;
+  const double* graph_W = nullptr;
+  const double* graph_U = nullptr;
+  bool graph_mixed = false;
 };
 
 namespace rhm {

@@ -227,29 +227,35 @@ void raishin_coord1(const raishin_pram* pp, double* gcov, double* gcovi, double*
                     double* detgk, const double* x1, const double* x2, const double* x3, const double* x1a,
                     const double* x2a, const double* x3a) {
   // coord.f90:5-146 (metric 1 and 303) and caldetg (coord.f90:1016-1086)
+  // The metrics built here (303, 1) have no phi dependence: each (i,k) is
+  // computed once and copied to every j.  Columns k are independent.
   const raishin_pram& p = *pp;
   (void)x2; (void)x2a;
+#pragma omp parallel for schedule(dynamic)
   for (int k = 1; k <= p.kmax; k++)
-    for (int j = 1; j <= p.jmax; j++)
-      for (int i = 1; i <= p.imax; i++) {
+    for (int i = 1; i <= p.imax; i++) {
+      double g[16], gi[16], gk[16], h[16], hi[16], hk[16];
+      metric_cov(p, g, x1[i - 1], x3[k - 1]);
+      metric_cov(p, gi, x1a[i - 1], x3[k - 1]);
+      metric_cov(p, gk, x1[i - 1], x3a[k - 1]);
+      metric_con(p, h, x1[i - 1], x3[k - 1]);
+      metric_con(p, hi, x1a[i - 1], x3[k - 1]);
+      metric_con(p, hk, x1[i - 1], x3a[k - 1]);
+      for (int j = 1; j <= p.jmax; j++) {
         const size_t c = cell(p, i, j, k);
-        double g[16], gi[16], gk[16], h[16], hi[16], hk[16];
-        metric_cov(p, g, x1[i - 1], x3[k - 1]);
-        metric_cov(p, gi, x1a[i - 1], x3[k - 1]);
-        metric_cov(p, gk, x1[i - 1], x3a[k - 1]);
-        metric_con(p, h, x1[i - 1], x3[k - 1]);
-        metric_con(p, hi, x1a[i - 1], x3[k - 1]);
-        metric_con(p, hk, x1[i - 1], x3a[k - 1]);
         std::memcpy(&gcov[16 * c], g, 128); std::memcpy(&gcovi[16 * c], gi, 128);
         std::memcpy(&gcovj[16 * c], g, 128); std::memcpy(&gcovk[16 * c], gk, 128);
         std::memcpy(&gcon[16 * c], h, 128); std::memcpy(&gconi[16 * c], hi, 128);
         std::memcpy(&gconj[16 * c], h, 128); std::memcpy(&gconk[16 * c], hk, 128);
       }
+    }
   double* D[4] = {detg, detgi, detgj, detgk};
   const double* Gs[4] = {gcov, gcovi, gcovj, gcovk};
   const size_t N = (size_t)p.imax * p.jmax * p.kmax;
   for (int s = 0; s < 4; s++)
-    for (size_t c = 0; c < N; c++) {
+#pragma omp parallel for schedule(static)
+    for (long long cc = 0; cc < (long long)N; cc++) {
+      const size_t c = (size_t)cc;
       const double* g = &Gs[s][16 * c];
       double aa, bb, cc, pp_, qq, rr, xx, yy, zz;
       aa = GM(g, 1, 1); bb = GM(g, 1, 2); cc = GM(g, 1, 3); pp_ = GM(g, 2, 1); qq = GM(g, 2, 2); rr = GM(g, 2, 3);
@@ -267,10 +273,14 @@ void raishin_coord1(const raishin_pram* pp, double* gcov, double* gcovi, double*
 
 void raishin_calchrist(const raishin_pram* pp, double* christ, const double* x1, const double* x2, const double* x3) {
   // calchrist: coord.f90:1089-1321 (finite-difference metric derivatives, delta=1e-5)
+  // Computed for j = 1 and copied to every j: the metrics built here have no
+  // phi dependence, and the phi derivative's two metric evaluations are equal
+  // (its difference is zero for every phi).
   const raishin_pram& p = *pp;
   const double delta = 1.0e-5, small = 1.0e-12;
+#pragma omp parallel for schedule(dynamic)
   for (int k = 1; k <= p.kmax; k++)
-    for (int j = 1; j <= p.jmax; j++)
+    for (int j = 1; j <= 1; j++)
       for (int i = 1; i <= p.imax; i++) {
         const double r = x1[i - 1], ph = x2[j - 1], th = x3[k - 1];
         const double ria = r + delta, rib = r - delta, pja = ph + delta, pjb = ph - delta, tka = th + delta,
@@ -298,6 +308,7 @@ void raishin_calchrist(const raishin_pram* pp, double* christ, const double* x1,
             for (int l = 0; l < 4; l++)
               ch[l + 4 * m + 16 * n] = GM(gi, l, 0) * t2[0][m][n] + GM(gi, l, 1) * t2[1][m][n] +
                                        GM(gi, l, 2) * t2[2][m][n] + GM(gi, l, 3) * t2[3][m][n];
+        for (int jj = 2; jj <= p.jmax; jj++) std::memcpy(&christ[64 * cell(p, i, jj, k)], ch, 64 * sizeof(double));
       }
 }
 

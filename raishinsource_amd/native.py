@@ -49,7 +49,8 @@ EXPORTS = (
     "raishin_model_counters", "raishin_model_reset_counters", "raishin_model_sync", "raishin_model_stream",
     "raishin_cdtcfl4", "raishin_init", "raishin_grid", "raishin_coord1", "raishin_calchrist",
     "raishin_mdtorus1", "raishin_model_set_timing", "raishin_model_get_timing",
-    "raishin_model_debug_stage", "raishin_model_set_precision",
+    "raishin_model_debug_stage", "raishin_model_set_precision", "raishin_model_step_graph",
+    "raishin_models_step_graph",
 )
 
 # reference-named entry points (include/raishin_ref.h)
@@ -103,6 +104,8 @@ def lib():
         L.raishin_model_get_timing.restype = C.c_int
         L.raishin_model_debug_stage.argtypes = [vp, C.c_int, vp, vp]
         L.raishin_model_set_precision.argtypes = [vp, C.c_int]
+        L.raishin_model_step_graph.argtypes = [vp, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_double)]
+        L.raishin_models_step_graph.argtypes = [C.POINTER(vp), C.c_int, C.c_int]
         _lib = L
     return _lib
 
@@ -183,6 +186,16 @@ def setup_torus(p: Pram, g, m, rin=6.0, rmax=12.0, beta=100.0, acut=0.2, noise=0
     return uri
 
 
+def step_models_graph(models, npairs):
+    """2*npairs steps of every model, concurrently (one stream per model,
+    graph launches queued round robin); returns after all have finished."""
+    arr = (C.c_void_p * len(models))(*[mm.h for mm in models])
+    lib().raishin_models_step_graph(arr, len(models), npairs)
+    check()
+    for mm in models:
+        mm.sync()
+
+
 class Model:
     """One resident model on the GPU (raishin_model_* of include/raishin_hip.h)."""
 
@@ -219,6 +232,17 @@ class Model:
             check()
             return t.value, dt.value
         lib().raishin_model_step(self.h, n, None, None)
+        check()
+        return None
+
+    def step_graph(self, npairs=1, sync=True):
+        """2*npairs steps through a captured two-step HIP graph."""
+        if sync:
+            t, dt = C.c_double(0.0), C.c_double(0.0)
+            lib().raishin_model_step_graph(self.h, npairs, C.byref(t), C.byref(dt))
+            check()
+            return t.value, dt.value
+        lib().raishin_model_step_graph(self.h, npairs, None, None)
         check()
         return None
 
