@@ -226,8 +226,10 @@ def ensemble(args):
     128x128, 256x128, 512x256; spin U[0,0.99]; beta_min from 10, 100, 1000;
     r_in U[6,10], r_max = r_in + 6; PPM, RK2, CFL 0.4; seeded).  The same
     steps are run one model at a time (each model's steps launched eagerly,
-    then the next model) and all models together (two-step HIP graphs queued
-    round robin, one stream per model); results must be bit-identical."""
+    then the next model), with one stream and one two-step HIP graph per
+    model queued round robin, and as one batch (one launch per kernel for all
+    models, two steps per graph launch), in FP64 and mixed precision.  The
+    FP64 results of the three must be bit-identical."""
     from dataclasses import replace
 
     import numpy as np
@@ -271,18 +273,32 @@ def ensemble(args):
     t0 = time.perf_counter()
     native.step_models_graph(models, steps // 2)
     t_conc = time.perf_counter() - t0
-    same = all(np.array_equal(np.asarray(r).view(np.uint64), np.asarray(mod.get_state()[0]).view(np.uint64))
-               for r, mod in zip(ref, models))
-    # the mixed FP32/FP64 form of the same ensemble, together
-    for mod, uri in zip(models, states):
-        mod.set_precision(True)
-        mod.set_state(uri)
-        mod.step_graph(1)
-        mod.set_state(uri)
-        mod.sync()
-    t0 = time.perf_counter()
-    native.step_models_graph(models, steps // 2)
-    t_mix = time.perf_counter() - t0
+    same_streams = all(np.array_equal(np.asarray(r).view(np.uint64), np.asarray(mod.get_state()[0]).view(np.uint64))
+                       for r, mod in zip(ref, models))
+
+    def batched(mixed):
+        for mod, uri in zip(models, states):
+            mod.set_precision(mixed)
+            mod.set_state(uri)
+        b = native.Batch(models)
+        b.step_graph(1)  # plan and capture (two steps run), then start again
+        for mod, uri in zip(models, states):
+            mod.set_state(uri)
+        b.sync()
+        for mod in models:
+            mod.sync()
+        t0 = time.perf_counter()
+        b.step_graph(steps // 2)
+        b.sync()
+        el = time.perf_counter() - t0
+        b.set([])
+        b.close()
+        return el
+
+    t_bat = batched(False)
+    same_batch = all(np.array_equal(np.asarray(r).view(np.uint64), np.asarray(mod.get_state()[0]).view(np.uint64))
+                     for r, mod in zip(ref, models))
+    t_mix = batched(True)
     l1 = []
     for r, mod in zip(ref, models):
         d = np.asarray(mod.get_state()[0])[0]
@@ -290,18 +306,18 @@ def ensemble(args):
         l1.append(float(np.abs(d - r0).sum() / np.abs(r0).sum()))
     for mod in models:
         mod.close()
-    print(f"[bench] ensemble: {steps} steps, one at a time {t_seq:.2f}s, together {t_conc:.2f}s, "
-          f"mixed together {t_mix:.2f}s", file=sys.stderr, flush=True)
+    print(f"[bench] ensemble: {steps} steps, one at a time {t_seq:.2f}s, streams {t_conc:.2f}s, "
+          f"batched {t_bat:.2f}s, batched mixed {t_mix:.2f}s", file=sys.stderr, flush=True)
+    rate = lambda t: {"value": zones * steps / t, "unit": "zone-updates/s", "seconds": t}
     return {
         "models": len(models),
         "steps_per_model": steps,
         "interior_zones": zones,
-        "one_at_a_time": {"value": zones * steps / t_seq, "unit": "zone-updates/s", "seconds": t_seq},
-        "together_graphs": {"value": zones * steps / t_conc, "unit": "zone-updates/s", "seconds": t_conc},
-        "speedup": t_seq / t_conc,
-        "bitwise_same": bool(same),
-        "mixed_together_graphs": {"value": zones * steps / t_mix, "unit": "zone-updates/s", "seconds": t_mix,
-                                  "max_rel_l1_density": max(l1)},
+        "one_at_a_time": rate(t_seq),
+        "streams_graphs": dict(rate(t_conc), bitwise_same=bool(same_streams)),
+        "batched_graph": dict(rate(t_bat), bitwise_same=bool(same_batch), speedup_vs_one_at_a_time=t_seq / t_bat),
+        "batched_graph_mixed": dict(rate(t_mix), max_rel_l1_density_vs_fp64=max(l1),
+                                    speedup_vs_one_at_a_time=t_seq / t_mix),
         "setup_seconds": t_set,
     }
 

@@ -120,6 +120,36 @@ void raishin_model_set_timing(raishin_model *m, int on);
 void raishin_model_debug_stage(raishin_model *m, int stage, double *ww, double *wpre);
 int raishin_model_get_timing(raishin_model *m, double *ms, long long *launches, int nkinds);
 
+/* Device memory the model holds, in bytes (state, fluxes, metric tables and
+ * work arrays): what admission of a new model has to find free. */
+long long raishin_model_device_bytes(raishin_model *m);
+
+/* ------------------------------------------------------------------------
+ * Batches: many resident models advanced together (no counterpart in the
+ * reference, whose main loop runs one model per MPI job).  Every kernel of a
+ * step is one launch for all members; each member owns a contiguous range
+ * of the launch's workgroups.  Members may differ in size, metric, initial
+ * data and precision, and must share irec, irkt and the boundary types that
+ * decide which kernels a step makes.  Each member's results are those of
+ * raishin_model_step on that member alone, bit for bit.  The calls are
+ * stream-ordered with the members' own streams (the batch waits for work
+ * queued on them, and later work on them waits for the batch).  Returns 0,
+ * or -1 with raishin_last_error set.
+ * ------------------------------------------------------------------------ */
+typedef struct raishin_batch raishin_batch;
+raishin_batch *raishin_batch_create(void);
+void raishin_batch_destroy(raishin_batch *b);
+/* Replace the members (a model appears at most once; n = 0 empties it). */
+int raishin_batch_set(raishin_batch *b, raishin_model *const *models, int n);
+/* nsteps steps of every member. */
+int raishin_batch_step(raishin_batch *b, int nsteps);
+/* 2*npairs steps of every member through a captured HIP graph of two steps
+ * (captured again after the members or their buffers change). */
+int raishin_batch_step_graph(raishin_batch *b, int npairs);
+/* Simulation time of each member, in member order (waits for the batch). */
+int raishin_batch_times(raishin_batch *b, double *times);
+int raishin_batch_sync(raishin_batch *b);
+
 #ifdef __cplusplus
 }
 #endif

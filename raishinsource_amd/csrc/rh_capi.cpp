@@ -236,6 +236,11 @@ void raishin_model_sync(raishin_model* m) {
 
 void* raishin_model_stream(raishin_model* m) { return (void*)m->st; }
 
+long long raishin_model_device_bytes(raishin_model* m) {
+  // build_model's allocations plus the staging buffer set_state/get_state add
+  return m ? (long long)m->dev_bytes + (m->stage_aos ? 0LL : 9LL * m->G.N * 8) : 0;
+}
+
 void raishin_model_debug_stage(raishin_model* m, int stage, double* ww, double* wpre) {
   // stage 1 also runs the CFL reduction and time advance of a step
   StageTap tap;
@@ -244,8 +249,8 @@ void raishin_model_debug_stage(raishin_model* m, int stage, double* ww, double* 
   auto go = [&]() -> bool {
     if (stage == 1) {
       if (!run_cfl(m, m->W)) return false;
-      HIPCHK(launch_advance_time(m->dtbits, m->p.cfl, m->tdt, m->st));
-      if (m->need_wo) HIPCHK(hipMemcpyAsync(m->Wo, m->W, 9 * m->G.N * 8, hipMemcpyDeviceToDevice, m->st));
+      HIPCHK(launch_advance_time(m->G, m->dtbits, m->p.cfl, m->tdt, m->st));
+      if (m->need_wo) HIPCHK(launch_copy(m->G, m->Wo, m->W, 9 * m->G.N, m->st));
       return run_stage(m, 1, m->Ucur, nullptr, m->Uh, &tap);
     }
     if (stage == 2) {

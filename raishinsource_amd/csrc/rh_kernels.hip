@@ -54,15 +54,16 @@ enum { C_FALLBACK = 0, C_FAIL, C_NEGRHO, C_NEGP, C_VSQGT1, C_DET0, C_WGCARRY, C_
        C_GUARD, C_NCOUNT };
 
 // Event counts without atomics: each wavefront of a launch owns the row
-// cpart[w*C_NCOUNT ..] (w = its index in the launch); the lowest flagged lane
+// cpart[w*C_NCOUNT ..] (w = its index in the model's part of the launch, from
+// the workgroup's index blk there); the lowest flagged lane
 // adds the wave's popcount.  Launches are stream-ordered, so a row has one
 // writer at a time; k_sum_counters folds the rows when counters are read.
 // (Device-scope atomics on a few addresses from every wave serialise across
 // the XCDs: they cost more than the kernels that raise the events.)
-RH_DEV void count_wave(unsigned long long* cpart, int slot, bool flag) {
+RH_DEV void count_wave(unsigned long long* cpart, long long blk, int slot, bool flag) {
   const unsigned long long b = __ballot(flag);
   if (b && (threadIdx.x % warpSize) == (unsigned)(__ffsll((long long)b) - 1)) {
-    const long long w = ((long long)blockIdx.x * blockDim.x + threadIdx.x) / warpSize;
+    const long long w = (blk * blockDim.x + threadIdx.x) / warpSize;
     cpart[w * C_NCOUNT + slot] += (unsigned long long)__popcll(b);
   }
 }
@@ -235,11 +236,15 @@ RH_DEV void ppm_var(const R u[5], const PpmCoef<R>& k, int n, R f2, R eta, R& l_
 // sweep's box across.  R: arithmetic and storage type of the edges.
 // ------------------------------------------------------------------------
 template <int D, class R>
-__global__ void __launch_bounds__(256) k_ppm_edges(Geo G, const double* __restrict__ W, const double* __restrict__ dxs,
-                                                   const double* __restrict__ xs, const double* __restrict__ pc,
-                                                   double gam, R* __restrict__ E) {
+RH_DEV void b_ppm_edges(const Geo& G, const EdgeArgs& A, long long blk) {
+  const double* __restrict__ W = A.W;
+  const double* __restrict__ dxs = A.dxs;
+  const double* __restrict__ xs = A.xs;
+  const double* __restrict__ pc = A.pc;
+  const double gam = A.gam;
+  R* __restrict__ E = (R*)A.E;
   int i, j, k;
-  const long long t = xcd_thread();
+  const long long t = blk * blockDim.x + threadIdx.x;
   const int hi = G.ni - G.nm0 + (D == 1), hj = G.nj - G.nm0 + (D == 2), hk = G.nk - G.nm0 + (D == 3);
   const bool in = (D == 3) ? box_index_ikj(t, G.nm0, hi, G.nm0, hj, G.nm0, hk, i, j, k)
                            : box_index(t, G.nm0, hi, G.nm0, hj, G.nm0, hk, i, j, k);
@@ -286,12 +291,15 @@ __global__ void __launch_bounds__(256) k_ppm_edges(Geo G, const double* __restri
 //
 // PPM's face values come from k_ppm_edges (E); MP5 and WENO5 reconstruct here.
 template <int D, int IREC, class R>
-__global__ void __launch_bounds__(256) k_sweep(Geo G, const double* __restrict__ W, const double* __restrict__ metF,
-                                               const R* __restrict__ E, double gam_d, double* __restrict__ F,
-                                               double* __restrict__ CTsrc) {
-  const R gam = R(gam_d);
+RH_DEV void b_sweep(const Geo& G, const SweepArgs& A, long long blk) {
+  const double* __restrict__ W = A.W;
+  const double* __restrict__ metF = A.metF;
+  const R* __restrict__ E = (const R*)A.E;
+  double* __restrict__ F = A.F;
+  double* __restrict__ CTsrc = A.CTsrc;
+  const R gam = R(A.gam);
   int i, j, k;
-  const long long t = xcd_thread();
+  const long long t = blk * blockDim.x + threadIdx.x;
   const bool in = (D == 3) ? box_index_ikj(t, G.nm0, G.ni - G.nm0, G.nm0, G.nj - G.nm0, G.nm0, G.nk - G.nm0, i, j, k)
                            : box_index(t, G.nm0, G.ni - G.nm0, G.nm0, G.nj - G.nm0, G.nm0, G.nk - G.nm0, i, j, k);
   if (!in) return;
@@ -375,10 +383,13 @@ __global__ void __launch_bounds__(256) k_sweep(Geo G, const double* __restrict__
 // 5 ww(2,9), as the sweeps left them) and writes the averaged values into
 // the flux array in place, like ct.f90:98-109.
 // ------------------------------------------------------------------------
-__global__ void __launch_bounds__(256) k_fluxct(Geo G, const double* __restrict__ CTsrc, double* __restrict__ F1,
-                                                double* __restrict__ F2, double* __restrict__ F3) {
+RH_DEV void b_fluxct(const Geo& G, const CtArgs& A, long long blk) {
+  const double* __restrict__ CTsrc = A.CTsrc;
+  double* __restrict__ F1 = A.F1;
+  double* __restrict__ F2 = A.F2;
+  double* __restrict__ F3 = A.F3;
   int i, j, k;
-  const long long t = xcd_thread();
+  const long long t = blk * blockDim.x + threadIdx.x;
   if (!box_index(t, G.nm1, G.ni - G.nm1, G.nm1, G.nj - G.nm1, G.nm1, G.nk - G.nm1, i, j, k)) return;
   const long long N = G.N, c = cidx(i, j, k, G);
   const long long si = 1, sj = G.ni, sk = (long long)G.ni * G.nj;
@@ -612,9 +623,9 @@ RH_DEV long long chr_plane(int l, int a, int b) {
 }
 
 template <int STAGE, bool SYM>
-__global__ void __launch_bounds__(256) k_rk(Geo G, StageArgs A) {
+RH_DEV void b_rk(const Geo& G, const StageArgs& A, long long blk) {
   int i, j, k;
-  const long long t = xcd_thread();
+  const long long t = blk * blockDim.x + threadIdx.x;
   if (!box_index(t, G.nm1 + 1, G.ni - G.nm1, G.nm1 + 1, G.nj - G.nm1, G.nm1 + 1, G.nk - G.nm1, i, j, k)) return;
   const long long N = G.N, c = cidx(i, j, k, G);
   const long long sj = G.ni, sk = (long long)G.ni * G.nj;
@@ -684,9 +695,9 @@ __global__ void __launch_bounds__(256) k_rk(Geo G, StageArgs A) {
 
 // recov1a (recov.f90:578-1228) on every interior cell from the stage's RK
 // result; cells whose initial W is inherited are flagged for k_recov_carry.
-__global__ void __launch_bounds__(256) k_recov(Geo G, StageArgs A) {
+RH_DEV void b_recov(const Geo& G, const StageArgs& A, long long blk) {
   int i, j, k;
-  const long long t = xcd_thread();
+  const long long t = blk * blockDim.x + threadIdx.x;
   if (!box_index(t, G.nm1 + 1, G.ni - G.nm1, G.nm1 + 1, G.nj - G.nm1, G.nm1 + 1, G.nk - G.nm1, i, j, k)) return;
   const long long N = G.N, c = cidx(i, j, k, G);
   const Met M{A.metC, G.P2, q2(i, k, G)};
@@ -705,14 +716,14 @@ __global__ void __launch_bounds__(256) k_recov(Geo G, StageArgs A) {
   for (int n = 0; n < 9; n++) A.Wpre[(long long)n * N + c] = ro.w[n];
   A.wgout[c] = wgout;
   // a flagged cell's outcome is counted when the fix-up computes it
-  count_wave(A.cpart, C_FALLBACK, ro.fallback && !dep);
-  count_wave(A.cpart, C_FAIL, ro.fail && !dep);
-  count_wave(A.cpart, C_NEGRHO, ro.negrho && !dep);
-  count_wave(A.cpart, C_NEGP, ro.negp && !dep);
-  count_wave(A.cpart, C_VSQGT1, ro.vsqgt1 && !dep);
-  count_wave(A.cpart, C_DET0, ro.det0 && !dep);
-  count_wave(A.cpart, C_WGCARRY, carry);
-  count_wave(A.cpart, C_GUARD, ro.guard && !dep);
+  count_wave(A.cpart, blk, C_FALLBACK, ro.fallback && !dep);
+  count_wave(A.cpart, blk, C_FAIL, ro.fail && !dep);
+  count_wave(A.cpart, blk, C_NEGRHO, ro.negrho && !dep);
+  count_wave(A.cpart, blk, C_NEGP, ro.negp && !dep);
+  count_wave(A.cpart, blk, C_VSQGT1, ro.vsqgt1 && !dep);
+  count_wave(A.cpart, blk, C_DET0, ro.det0 && !dep);
+  count_wave(A.cpart, blk, C_WGCARRY, carry);
+  count_wave(A.cpart, blk, C_GUARD, ro.guard && !dep);
   if (dep) {
     // remember the cell for the in-order resolution
     A.cflag[c] = 1;
@@ -839,9 +850,9 @@ RH_DEV bool ghost_index(long long t, const Geo& G, int& i, int& j, int& k) {
 // ghost slots of the recovered primitives.  bnd_value reads only interior
 // cells of that array (and the stage-start values of other ghosts), so the
 // writes cannot race with the reads.
-__global__ void __launch_bounds__(256) k_bnd(Geo G, PostArgs A) {
+RH_DEV void b_bnd(const Geo& G, const PostArgs& A, long long blk) {
   int i, j, k;
-  if (!ghost_index((long long)blockIdx.x * blockDim.x + threadIdx.x, G, i, j, k)) return;
+  if (!ghost_index(blk * blockDim.x + threadIdx.x, G, i, j, k)) return;
   const long long N = G.N, c = cidx(i, j, k, G);
   const BndCtx B{&G, A.Wpre, A.Wold, A.Wo, A.bc};
 #pragma unroll 1
@@ -849,8 +860,8 @@ __global__ void __launch_bounds__(256) k_bnd(Geo G, PostArgs A) {
 }
 
 // infchk + atmosphere1a + pminmax2b + caluu1 on every cell, after k_bnd.
-__global__ void __launch_bounds__(256) k_post(Geo G, PostArgs A) {
-  const long long t = xcd_thread();
+RH_DEV void b_post(const Geo& G, const PostArgs& A, long long blk) {
+  const long long t = blk * blockDim.x + threadIdx.x;
   int i, j, k;
   if (!box_index(t, 1, G.ni, 1, G.nj, 1, G.nk, i, j, k)) return;
   const long long N = G.N, c = cidx(i, j, k, G);
@@ -903,7 +914,7 @@ __global__ void __launch_bounds__(256) k_post(Geo G, PostArgs A) {
     }
     w[5] = 0.0;
   }
-  count_wave(A.cpart, C_ATM, atm);
+  count_wave(A.cpart, blk, C_ATM, atm);
   // pminmax2b: correction.f90:808-961
   bool scaled = false, hit = false;
   {
@@ -951,8 +962,8 @@ __global__ void __launch_bounds__(256) k_post(Geo G, PostArgs A) {
     if (bbro > 50.0) { w[0] = s.bbsq / 50.0; hit = true; }
     if (bbpr > 2.0e3) { w[4] = s.bbsq / 2.0e3; hit = true; }
   }
-  count_wave(A.cpart, C_PMMSCALE, scaled);
-  count_wave(A.cpart, C_PMMFLOOR, hit);
+  count_wave(A.cpart, blk, C_PMMSCALE, scaled);
+  count_wave(A.cpart, blk, C_PMMFLOOR, hit);
 #pragma unroll
   for (int n = 0; n < 9; n++) A.Wout[(long long)n * N + c] = w[n];
   // caluu1 (caluu.f90:2-126): conserved variables of the final primitives
@@ -995,10 +1006,10 @@ RH_DEV long long wave_max_ll(long long v) {
   return v;
 }
 
-__global__ void __launch_bounds__(256) k_cfl(Geo G, CflArgs A) {
+RH_DEV void b_cfl(const Geo& G, const CflArgs& A, long long blk) {
   __shared__ long long s_last[4][4];
   __shared__ double s_min[4][4];
-  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long long t = blk * blockDim.x + threadIdx.x;
   int i, j, k;
   double e[4] = {__builtin_inf(), __builtin_inf(), __builtin_inf(), 0.0};  // dtx1a, dtx2a, dtx3a
   bool valid = box_index(t, 2, G.ni - 1, 2, G.nj - 1, 2, G.nk - 1, i, j, k);
@@ -1081,15 +1092,16 @@ __global__ void __launch_bounds__(256) k_cfl(Geo G, CflArgs A) {
     const int a = threadIdx.x;
     double v = s_min[0][a];
     for (int w = 1; w < (int)(blockDim.x / 64); w++) v = (s_min[w][a] < v) ? s_min[w][a] : v;
-    A.blast[4 * blockIdx.x + a] = L[a];
-    A.bmin[4 * blockIdx.x + a] = v;
+    A.blast[4 * blk + a] = L[a];
+    A.bmin[4 * blk + a] = v;
   }
 }
 
 // Combine the per-block results in block order (one workgroup): the last NaN
 // lies in the last block that has one; that block's minimum after it and the
 // full minima of every later block make the result.
-__global__ void __launch_bounds__(1024) k_cfl_finish(CflArgs A, int nblocks) {
+RH_DEV void b_cfl_finish(const CflArgs& A) {
+  const int nblocks = A.nblocks;
   __shared__ long long s_l[16];
   __shared__ double s_m[16];
   const int lane = threadIdx.x % 64, wid = threadIdx.x / 64, nw = blockDim.x / 64;
@@ -1121,11 +1133,11 @@ __global__ void __launch_bounds__(1024) k_cfl_finish(CflArgs A, int nblocks) {
 }
 
 // time <- time + cfl*dtcfl ; last dt kept (main.f90:760-781)
-__global__ void k_advance_time(const unsigned long long* dtbits, double cfl, double* tdt) {
+RH_DEV void b_advance_time(const AdvArgs& A) {
   if (threadIdx.x == 0) {
-    const double dt = cfl * __longlong_as_double((long long)dtbits[0]);
-    tdt[0] = tdt[0] + dt;
-    tdt[1] = dt;
+    const double dt = A.cfl * __longlong_as_double((long long)A.dtbits[0]);
+    A.tdt[0] = A.tdt[0] + dt;
+    A.tdt[1] = dt;
   }
 }
 
@@ -1135,7 +1147,7 @@ __global__ void k_advance_time(const unsigned long long* dtbits, double cfl, dou
 // predecessor's final W is known: passes over the flagged list, each taking
 // the cells whose predecessor is resolved, until none is left.  Chains of
 // consecutive flagged cells take one pass per link.  One workgroup.
-__global__ void __launch_bounds__(1024) k_recov_carry(Geo G, FixArgs A) {
+RH_DEV void b_recov_carry(const Geo& G, const FixArgs& A) {
   __shared__ int s_progress, s_left;
   unsigned long long n = A.carry[0];
   if (n == 0) return;
@@ -1248,37 +1260,167 @@ __global__ void k_planes_to_ww(const double* __restrict__ F, double* __restrict_
 }
 
 // ------------------------------------------------------------------------
-// Launch helpers (host side)
+// Kernels: one model per launch (the workgroups renumbered per XCD where the
+// stencil benefits), and the batched twins, where one launch covers the same
+// step of many models.  A batched launch reads a table of BatchItem (the
+// model's Geo, its arguments, and the first workgroup of its part of the
+// grid); each workgroup finds its model by binary search and runs the same
+// body with its index inside that model's part.
+// ------------------------------------------------------------------------
+RH_DEV int batch_find(const BatchItem* __restrict__ it, int n, long long b) {
+  int lo = 0, hi = n - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (it[mid].off <= b) lo = mid;
+    else hi = mid - 1;
+  }
+  return lo;
+}
+
+RH_DEV void b_copy(const CopyArgs& A, long long blk) {
+  const long long t = blk * blockDim.x + threadIdx.x;
+  if (t < A.n) A.dst[t] = A.src[t];
+}
+
+template <int D, class R>
+__global__ void __launch_bounds__(256) k_ppm_edges(Geo G, EdgeArgs A) { b_ppm_edges<D, R>(G, A, xcd_block()); }
+template <int D, int IREC, class R>
+__global__ void __launch_bounds__(256) k_sweep(Geo G, SweepArgs A) { b_sweep<D, IREC, R>(G, A, xcd_block()); }
+__global__ void __launch_bounds__(256) k_fluxct(Geo G, CtArgs A) { b_fluxct(G, A, xcd_block()); }
+template <int STAGE, bool SYM>
+__global__ void __launch_bounds__(256) k_rk(Geo G, StageArgs A) { b_rk<STAGE, SYM>(G, A, xcd_block()); }
+__global__ void __launch_bounds__(256) k_recov(Geo G, StageArgs A) { b_recov(G, A, xcd_block()); }
+__global__ void __launch_bounds__(1024) k_recov_carry(Geo G, FixArgs A) { b_recov_carry(G, A); }
+__global__ void __launch_bounds__(256) k_bnd(Geo G, PostArgs A) { b_bnd(G, A, blockIdx.x); }
+__global__ void __launch_bounds__(256) k_post(Geo G, PostArgs A) { b_post(G, A, xcd_block()); }
+__global__ void __launch_bounds__(256) k_cfl(Geo G, CflArgs A) { b_cfl(G, A, blockIdx.x); }
+__global__ void __launch_bounds__(1024) k_cfl_finish(CflArgs A) { b_cfl_finish(A); }
+__global__ void __launch_bounds__(64) k_advance_time(AdvArgs A) { b_advance_time(A); }
+
+// batched twins: BODY(G, A, blk) for the model that owns workgroup blockIdx.x
+#define RH_BATCH_PROLOGUE                                  \
+  const int q = batch_find(it, n, (long long)blockIdx.x);  \
+  const Geo G = it[q].G;                                   \
+  const long long blk = (long long)blockIdx.x - it[q].off;
+template <int D, class R>
+__global__ void __launch_bounds__(256) kb_ppm_edges(const BatchItem* __restrict__ it, int n) {
+  RH_BATCH_PROLOGUE
+  const EdgeArgs A = it[q].A.e;
+  b_ppm_edges<D, R>(G, A, blk);
+}
+template <int D, int IREC, class R>
+__global__ void __launch_bounds__(256) kb_sweep(const BatchItem* __restrict__ it, int n) {
+  RH_BATCH_PROLOGUE
+  const SweepArgs A = it[q].A.s;
+  b_sweep<D, IREC, R>(G, A, blk);
+}
+__global__ void __launch_bounds__(256) kb_fluxct(const BatchItem* __restrict__ it, int n) {
+  RH_BATCH_PROLOGUE
+  const CtArgs A = it[q].A.c;
+  b_fluxct(G, A, blk);
+}
+template <int STAGE, bool SYM>
+__global__ void __launch_bounds__(256) kb_rk(const BatchItem* __restrict__ it, int n) {
+  RH_BATCH_PROLOGUE
+  const StageArgs A = it[q].A.st;
+  b_rk<STAGE, SYM>(G, A, blk);
+}
+__global__ void __launch_bounds__(256) kb_recov(const BatchItem* __restrict__ it, int n) {
+  RH_BATCH_PROLOGUE
+  const StageArgs A = it[q].A.st;
+  b_recov(G, A, blk);
+}
+__global__ void __launch_bounds__(256) kb_bnd(const BatchItem* __restrict__ it, int n) {
+  RH_BATCH_PROLOGUE
+  const PostArgs A = it[q].A.p;
+  b_bnd(G, A, blk);
+}
+__global__ void __launch_bounds__(256) kb_post(const BatchItem* __restrict__ it, int n) {
+  RH_BATCH_PROLOGUE
+  const PostArgs A = it[q].A.p;
+  b_post(G, A, blk);
+}
+__global__ void __launch_bounds__(256) kb_cfl(const BatchItem* __restrict__ it, int n) {
+  RH_BATCH_PROLOGUE
+  const CflArgs A = it[q].A.cf;
+  b_cfl(G, A, blk);
+}
+__global__ void __launch_bounds__(256) kb_copy(const BatchItem* __restrict__ it, int n) {
+  RH_BATCH_PROLOGUE
+  (void)G;
+  const CopyArgs A = it[q].A.cp;
+  b_copy(A, blk);
+}
+#undef RH_BATCH_PROLOGUE
+// one workgroup per model
+__global__ void __launch_bounds__(1024) kb_recov_carry(const BatchItem* __restrict__ it, int n) {
+  if ((int)blockIdx.x >= n) return;
+  const Geo G = it[blockIdx.x].G;
+  const FixArgs A = it[blockIdx.x].A.fx;
+  b_recov_carry(G, A);
+}
+__global__ void __launch_bounds__(1024) kb_cfl_finish(const BatchItem* __restrict__ it, int n) {
+  if ((int)blockIdx.x >= n) return;
+  const CflArgs A = it[blockIdx.x].A.cf;
+  b_cfl_finish(A);
+}
+__global__ void __launch_bounds__(64) kb_advance_time(const BatchItem* __restrict__ it, int n) {
+  if ((int)blockIdx.x >= n) return;
+  const AdvArgs A = it[blockIdx.x].A.a;
+  b_advance_time(A);
+}
+
+// ------------------------------------------------------------------------
+// Launch helpers (host side).  While a recorder is set, each helper appends
+// the launch it would make (kind, template variant, workgroups, arguments)
+// instead of launching; the batch executor merges the records of many models.
 // ------------------------------------------------------------------------
 static inline unsigned nblk(long long n, int b) { return (unsigned)((n + b - 1) / b); }
 static inline long long box(int lo, int hi) { return hi >= lo ? (long long)(hi - lo + 1) : 0; }
 
-template <class R>
-static void ppm_edges_t(const Geo& G, int dir, const double* W, const double* dxs, const double* xs, const double* pc,
-                        double gam, R* E, long long n, hipStream_t st) {
-  const unsigned nb = nblk(n, 256);
-  if (dir == 1) hipLaunchKernelGGL((k_ppm_edges<1, R>), dim3(nb), dim3(256), 0, st, G, W, dxs, xs, pc, gam, E);
-  else if (dir == 2) hipLaunchKernelGGL((k_ppm_edges<2, R>), dim3(nb), dim3(256), 0, st, G, W, dxs, xs, pc, gam, E);
-  else hipLaunchKernelGGL((k_ppm_edges<3, R>), dim3(nb), dim3(256), 0, st, G, W, dxs, xs, pc, gam, E);
+static thread_local std::vector<LaunchRec>* g_rec = nullptr;
+
+void set_recorder(std::vector<LaunchRec>* r) { g_rec = r; }
+
+static void record(int kind, int variant, long long nb, const Geo& G, const AnyArgs& A) {
+  LaunchRec r;
+  r.kind = kind; r.variant = variant; r.nb = nb; r.G = G; r.A = A;
+  g_rec->push_back(r);
 }
+
+int block_threads(int kind) {
+  switch (kind) {
+    case LK_CARRY: case LK_CFL_FINISH: return 1024;
+    case LK_ADVANCE: return 64;
+    default: return 256;
+  }
+}
+
+bool one_group_per_model(int kind) { return kind == LK_CARRY || kind == LK_CFL_FINISH || kind == LK_ADVANCE; }
 
 hipError_t launch_ppm_edges(const Geo& G, int dir, const double* W, const double* dxs, const double* xs,
                             const double* pc, double gam, void* E, bool mixed, hipStream_t st) {
   const long long n = box(G.nm0, G.ni - G.nm0 + (dir == 1)) * box(G.nm0, G.nj - G.nm0 + (dir == 2)) *
                       box(G.nm0, G.nk - G.nm0 + (dir == 3));
+  EdgeArgs A{W, dxs, xs, pc, gam, E};
+  const unsigned nb = nblk(n, 256);
+  if (g_rec) { AnyArgs a{}; a.e = A; record(LK_EDGES, dir + 4 * (int)mixed, nb, G, a); return hipSuccess; }
   if (n == 0) return hipSuccess;
-  if (mixed) ppm_edges_t<float>(G, dir, W, dxs, xs, pc, gam, (float*)E, n, st);
-  else ppm_edges_t<double>(G, dir, W, dxs, xs, pc, gam, (double*)E, n, st);
+#define L(Dd, T) hipLaunchKernelGGL((k_ppm_edges<Dd, T>), dim3(nb), dim3(256), 0, st, G, A)
+  if (mixed) { if (dir == 1) L(1, float); else if (dir == 2) L(2, float); else L(3, float); }
+  else { if (dir == 1) L(1, double); else if (dir == 2) L(2, double); else L(3, double); }
+#undef L
   return hipGetLastError();
 }
 
 hipError_t launch_sweep(const Geo& G, int dir, int irec, const double* W, const double* metF, const void* E,
                         double gam, double* F, double* CTsrc, bool mixed, hipStream_t st) {
   const long long n = box(G.nm0, G.ni - G.nm0) * box(G.nm0, G.nj - G.nm0) * box(G.nm0, G.nk - G.nm0);
-  if (n == 0) return hipSuccess;
+  SweepArgs A{W, metF, E, gam, F, CTsrc};
   const unsigned nb = nblk(n, 256);
-#define L(Dd, Rc, T) \
-  hipLaunchKernelGGL((k_sweep<Dd, Rc, T>), dim3(nb), dim3(256), 0, st, G, W, metF, (const T*)E, gam, F, CTsrc)
+  if (g_rec) { AnyArgs a{}; a.s = A; record(LK_SWEEP, dir + 4 * (int)mixed + 8 * irec, nb, G, a); return hipSuccess; }
+  if (n == 0) return hipSuccess;
+#define L(Dd, Rc, T) hipLaunchKernelGGL((k_sweep<Dd, Rc, T>), dim3(nb), dim3(256), 0, st, G, A)
 #define LT(Dd, Rc) \
   do { if (mixed) L(Dd, Rc, float); else L(Dd, Rc, double); } while (0)
   if (irec == 5) { if (dir == 1) LT(1, 5); else if (dir == 2) LT(2, 5); else LT(3, 5); }
@@ -1291,15 +1433,18 @@ hipError_t launch_sweep(const Geo& G, int dir, int irec, const double* W, const 
 
 hipError_t launch_fluxct(const Geo& G, const double* CTsrc, double* F1, double* F2, double* F3, hipStream_t st) {
   const long long n = box(G.nm1, G.ni - G.nm1) * box(G.nm1, G.nj - G.nm1) * box(G.nm1, G.nk - G.nm1);
+  CtArgs A{CTsrc, F1, F2, F3};
+  if (g_rec) { AnyArgs a{}; a.c = A; record(LK_FLUXCT, 0, nblk(n, 256), G, a); return hipSuccess; }
   if (n == 0) return hipSuccess;
-  hipLaunchKernelGGL(k_fluxct, dim3(nblk(n, 256)), dim3(256), 0, st, G, CTsrc, F1, F2, F3);
+  hipLaunchKernelGGL(k_fluxct, dim3(nblk(n, 256)), dim3(256), 0, st, G, A);
   return hipGetLastError();
 }
 
 hipError_t launch_update(const Geo& G, int stage, const StageArgs& A, hipStream_t st) {
   const long long n = box(G.nm1 + 1, G.ni - G.nm1) * box(G.nm1 + 1, G.nj - G.nm1) * box(G.nm1 + 1, G.nk - G.nm1);
-  if (n == 0) return hipSuccess;
   const unsigned nb = nblk(n, 256);
+  if (g_rec) { AnyArgs a{}; a.st = A; record(LK_RK, stage + 8 * (int)A.chr_sym, nb, G, a); return hipSuccess; }
+  if (n == 0) return hipSuccess;
 #define L(S) \
   do { if (A.chr_sym) hipLaunchKernelGGL((k_rk<S, true>), dim3(nb), dim3(256), 0, st, G, A); \
        else hipLaunchKernelGGL((k_rk<S, false>), dim3(nb), dim3(256), 0, st, G, A); } while (0)
@@ -1313,12 +1458,14 @@ hipError_t launch_update(const Geo& G, int stage, const StageArgs& A, hipStream_
 
 hipError_t launch_recov(const Geo& G, const StageArgs& A, hipStream_t st) {
   const long long n = box(G.nm1 + 1, G.ni - G.nm1) * box(G.nm1 + 1, G.nj - G.nm1) * box(G.nm1 + 1, G.nk - G.nm1);
+  if (g_rec) { AnyArgs a{}; a.st = A; record(LK_RECOV, 0, nblk(n, 256), G, a); return hipSuccess; }
   if (n == 0) return hipSuccess;
   hipLaunchKernelGGL(k_recov, dim3(nblk(n, 256)), dim3(256), 0, st, G, A);
   return hipGetLastError();
 }
 
 hipError_t launch_recov_carry(const Geo& G, const FixArgs& A, hipStream_t st) {
+  if (g_rec) { AnyArgs a{}; a.fx = A; record(LK_CARRY, 0, 1, G, a); return hipSuccess; }
   hipLaunchKernelGGL(k_recov_carry, dim3(1), dim3(1024), 0, st, G, A);
   return hipGetLastError();
 }
@@ -1326,6 +1473,13 @@ hipError_t launch_recov_carry(const Geo& G, const FixArgs& A, hipStream_t st) {
 hipError_t launch_post(const Geo& G, const PostArgs& A, hipStream_t st) {
   const long long g = G.nm1;
   const long long ng = G.N - (long long)(G.ni - 2 * g) * (G.nj - 2 * g) * (G.nk - 2 * g);
+  if (g_rec) {
+    AnyArgs a{};
+    a.p = A;
+    record(LK_BND, 0, ng > 0 ? nblk(ng, 256) : 0, G, a);
+    record(LK_POST, 0, nblk(G.N, 256), G, a);
+    return hipSuccess;
+  }
   if (ng > 0) hipLaunchKernelGGL(k_bnd, dim3(nblk(ng, 256)), dim3(256), 0, st, G, A);
   hipLaunchKernelGGL(k_post, dim3(nblk(G.N, 256)), dim3(256), 0, st, G, A);
   return hipGetLastError();
@@ -1341,15 +1495,103 @@ long long cfl_blocks(const Geo& G) {
   return n > 0 ? (long long)nblk(n, 256) : 1;
 }
 
-hipError_t launch_cfl(const Geo& G, const CflArgs& A, hipStream_t st) {
+hipError_t launch_cfl(const Geo& G, const CflArgs& A0, hipStream_t st) {
+  CflArgs A = A0;
   const unsigned nb = (unsigned)cfl_blocks(G);
+  A.nblocks = (int)nb;
+  if (g_rec) {
+    AnyArgs a{};
+    a.cf = A;
+    record(LK_CFL, 0, nb, G, a);
+    record(LK_CFL_FINISH, 0, 1, G, a);
+    return hipSuccess;
+  }
   hipLaunchKernelGGL(k_cfl, dim3(nb), dim3(256), 0, st, G, A);
-  hipLaunchKernelGGL(k_cfl_finish, dim3(1), dim3(1024), 0, st, A, (int)nb);
+  hipLaunchKernelGGL(k_cfl_finish, dim3(1), dim3(1024), 0, st, A);
   return hipGetLastError();
 }
 
-hipError_t launch_advance_time(const unsigned long long* dtbits, double cfl, double* tdt, hipStream_t st) {
-  hipLaunchKernelGGL(k_advance_time, dim3(1), dim3(64), 0, st, dtbits, cfl, tdt);
+hipError_t launch_advance_time(const Geo& G, const unsigned long long* dtbits, double cfl, double* tdt,
+                               hipStream_t st) {
+  AdvArgs A{dtbits, cfl, tdt};
+  if (g_rec) { AnyArgs a{}; a.a = A; record(LK_ADVANCE, 0, 1, G, a); return hipSuccess; }
+  hipLaunchKernelGGL(k_advance_time, dim3(1), dim3(64), 0, st, A);
+  return hipGetLastError();
+}
+
+hipError_t launch_copy(const Geo& G, double* dst, const double* src, long long n, hipStream_t st) {
+  if (g_rec) {
+    AnyArgs a{};
+    a.cp = CopyArgs{src, dst, n};
+    record(LK_COPY, 0, nblk(n, 256), G, a);
+    return hipSuccess;
+  }
+  return hipMemcpyAsync(dst, src, (size_t)n * sizeof(double), hipMemcpyDeviceToDevice, st);
+}
+
+hipError_t launch_batched(int kind, int variant, const BatchItem* it, int n, long long total_blocks,
+                          hipStream_t st) {
+  if (n <= 0 || total_blocks <= 0) return hipSuccess;
+  if (total_blocks > 0x7fffffffLL) return hipErrorInvalidValue;
+  const dim3 g((unsigned)total_blocks), b(256);
+  switch (kind) {
+    case LK_EDGES: {
+      const int d = variant & 3;
+      const bool mx = (variant & 4) != 0;
+#define L(Dd, T) hipLaunchKernelGGL((kb_ppm_edges<Dd, T>), g, b, 0, st, it, n)
+      if (mx) { if (d == 1) L(1, float); else if (d == 2) L(2, float); else L(3, float); }
+      else { if (d == 1) L(1, double); else if (d == 2) L(2, double); else L(3, double); }
+#undef L
+      break;
+    }
+    case LK_SWEEP: {
+      const int d = variant & 3, irec = variant >> 3;
+      const bool mx = (variant & 4) != 0;
+#define L(Dd, Rc, T) hipLaunchKernelGGL((kb_sweep<Dd, Rc, T>), g, b, 0, st, it, n)
+#define LT(Dd, Rc) \
+  do { if (mx) L(Dd, Rc, float); else L(Dd, Rc, double); } while (0)
+      if (irec == 5) { if (d == 1) LT(1, 5); else if (d == 2) LT(2, 5); else LT(3, 5); }
+      else if (irec == 6) { if (d == 1) LT(1, 6); else if (d == 2) LT(2, 6); else LT(3, 6); }
+      else { if (d == 1) LT(1, 7); else if (d == 2) LT(2, 7); else LT(3, 7); }
+#undef LT
+#undef L
+      break;
+    }
+    case LK_FLUXCT: hipLaunchKernelGGL(kb_fluxct, g, b, 0, st, it, n); break;
+    case LK_RK: {
+      const int stg = variant & 7;
+      const bool sym = (variant & 8) != 0;
+#define L(S) \
+  do { if (sym) hipLaunchKernelGGL((kb_rk<S, true>), g, b, 0, st, it, n); \
+       else hipLaunchKernelGGL((kb_rk<S, false>), g, b, 0, st, it, n); } while (0)
+      if (stg == 1) L(1);
+      else if (stg == 2) L(2);
+      else if (stg == 3) L(3);
+      else L(4);
+#undef L
+      break;
+    }
+    case LK_RECOV: hipLaunchKernelGGL(kb_recov, g, b, 0, st, it, n); break;
+    case LK_BND: hipLaunchKernelGGL(kb_bnd, g, b, 0, st, it, n); break;
+    case LK_POST: hipLaunchKernelGGL(kb_post, g, b, 0, st, it, n); break;
+    case LK_CFL: hipLaunchKernelGGL(kb_cfl, g, b, 0, st, it, n); break;
+    case LK_COPY: hipLaunchKernelGGL(kb_copy, g, b, 0, st, it, n); break;
+    case LK_CARRY: hipLaunchKernelGGL(kb_recov_carry, dim3(n), dim3(1024), 0, st, it, n); break;
+    case LK_CFL_FINISH: hipLaunchKernelGGL(kb_cfl_finish, dim3(n), dim3(1024), 0, st, it, n); break;
+    case LK_ADVANCE: hipLaunchKernelGGL(kb_advance_time, dim3(n), dim3(64), 0, st, it, n); break;
+    default: return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
+__global__ void k_gather_first(const double* const* __restrict__ src, double* __restrict__ dst, int n) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < n) dst[t] = src[t][0];
+}
+
+hipError_t launch_gather_first(const double* const* src, double* dst, int n, hipStream_t st) {
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_gather_first, dim3(nblk(n, 256)), dim3(256), 0, st, src, dst, n);
   return hipGetLastError();
 }
 

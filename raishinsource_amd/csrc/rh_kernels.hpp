@@ -1,6 +1,7 @@
 // Kernel argument blocks and launchers of the GRMHD step (see rh_kernels.hip).
 #pragma once
 #include <hip/hip_runtime.h>
+#include <vector>
 #include "rh_device.hpp"
 
 namespace rh {
@@ -44,6 +45,7 @@ struct CflArgs {
   long long *blast;                // per block and accumulator: position of the last NaN
   double *bmin;                    // per block and accumulator: minimum after it
   double gam;
+  int nblocks;                     // workgroups of the k_cfl launch (set by launch_cfl)
 };
 
 struct FixArgs {
@@ -55,6 +57,83 @@ struct FixArgs {
   double gam, dmin, pmin;
   int iter;
 };
+
+struct EdgeArgs {
+  const double *W, *dxs, *xs, *pc;
+  double gam;
+  void *E;                         // float or double planes, by the launch's variant
+};
+
+struct SweepArgs {
+  const double *W, *metF;
+  const void *E;
+  double gam;
+  double *F, *CTsrc;
+};
+
+struct CtArgs {
+  const double *CTsrc;
+  double *F1, *F2, *F3;
+};
+
+struct AdvArgs {
+  const unsigned long long *dtbits;
+  double cfl;
+  double *tdt;
+};
+
+struct CopyArgs {
+  const double *src;
+  double *dst;
+  long long n;
+};
+
+// Launch kinds of one step, in the order run_step makes them.
+enum LaunchKind {
+  LK_EDGES, LK_SWEEP, LK_FLUXCT, LK_RK, LK_RECOV, LK_CARRY, LK_BND, LK_POST, LK_CFL, LK_CFL_FINISH,
+  LK_ADVANCE, LK_COPY, LK_N
+};
+
+union AnyArgs {
+  EdgeArgs e;
+  SweepArgs s;
+  CtArgs c;
+  StageArgs st;
+  FixArgs fx;
+  PostArgs p;
+  CflArgs cf;
+  AdvArgs a;
+  CopyArgs cp;
+};
+
+// A launch as recorded for one model: kind, template variant (edges: dir +
+// 4*mixed; sweep: dir + 4*mixed + 8*irec; rk: stage + 8*sym), workgroups.
+struct LaunchRec {
+  int kind, variant;
+  long long nb;
+  Geo G;
+  AnyArgs A;
+};
+
+// One model's entry in the device table of a batched launch; off = its first
+// workgroup (kinds with one workgroup per model: its index).
+struct BatchItem {
+  Geo G;
+  AnyArgs A;
+  long long off;
+};
+
+// While r is set, the launch_* helpers below append a record to *r instead
+// of launching (thread-local; nullptr ends recording).
+void set_recorder(std::vector<LaunchRec>* r);
+// One launch of a kind and variant over a table of n models (total_blocks
+// workgroups; for one-workgroup-per-model kinds, n).
+hipError_t launch_batched(int kind, int variant, const BatchItem* items, int n, long long total_blocks,
+                          hipStream_t st);
+int block_threads(int kind);
+// dst[q] = src[q][0] for q < n (the members' times of a batch)
+hipError_t launch_gather_first(const double* const* src, double* dst, int n, hipStream_t st);
+bool one_group_per_model(int kind);
 
 // PPM face values of every cell along dir into E (18 planes: left edges, right edges).
 // pc: the width-only interp coefficients of the direction (7 planes of n+2, ppm_coefficients).
@@ -75,7 +154,10 @@ hipError_t launch_cfl(const Geo& G, const CflArgs& A, hipStream_t st);
 long long cfl_blocks(const Geo& G);
 // Fold the per-wavefront event rows into cnt (and clear them).
 hipError_t launch_sum_counters(unsigned long long* cpart, long long rows, unsigned long long* cnt, hipStream_t st);
-hipError_t launch_advance_time(const unsigned long long* dtbits, double cfl, double* tdt, hipStream_t st);
+hipError_t launch_advance_time(const Geo& G, const unsigned long long* dtbits, double cfl, double* tdt,
+                               hipStream_t st);
+// dst <- src (n doubles), stream-ordered
+hipError_t launch_copy(const Geo& G, double* dst, const double* src, long long n, hipStream_t st);
 hipError_t launch_caluu1(const Geo& G, const double* W, const double* metC, double gam, double* U, hipStream_t st);
 hipError_t launch_aos_to_soa(const double* aos, double* soa, long long N, hipStream_t st);
 hipError_t launch_soa_to_aos(const double* soa, double* aos, long long N, hipStream_t st);

@@ -14,6 +14,7 @@ using namespace rh;
 namespace rhm {
 
 std::mutex g_err_mu;
+thread_local size_t g_dalloc_bytes = 0;
 std::string g_err;
 
 void set_error(const char* fmt, ...) {
@@ -189,9 +190,24 @@ std::vector<double> ppm_coefficients(const std::vector<double>& ds) {
   return pc;
 }
 
+static bool build_model_(raishin_model* m, const raishin_pram* p, const double* x1, const double* x2,
+                         const double* x3, const double* dx1b, const double* dx2b, const double* dx3b,
+                         const double* const gcv[4], const double* const gcn[4], const double* const dtg[4],
+                         const double* christ);
+
 bool build_model(raishin_model* m, const raishin_pram* p, const double* x1, const double* x2, const double* x3,
                  const double* dx1b, const double* dx2b, const double* dx3b, const double* const gcv[4],
                  const double* const gcn[4], const double* const dtg[4], const double* christ) {
+  const size_t bytes0 = g_dalloc_bytes;
+  const bool ok = build_model_(m, p, x1, x2, x3, dx1b, dx2b, dx3b, gcv, gcn, dtg, christ);
+  m->dev_bytes = g_dalloc_bytes - bytes0;
+  return ok;
+}
+
+static bool build_model_(raishin_model* m, const raishin_pram* p, const double* x1, const double* x2,
+                         const double* x3, const double* dx1b, const double* dx2b, const double* dx3b,
+                         const double* const gcv[4], const double* const gcn[4], const double* const dtg[4],
+                         const double* christ) {
   m->p = *p;
   if (!check_pram(*p)) return false;
   Geo& G = m->G;
@@ -407,8 +423,8 @@ bool cdtcfl4_on(raishin_model* m, const double* W, double* dtcfl, double* dt1, d
 bool run_step(raishin_model* m) {
   const long long N = m->G.N;
   if (!run_cfl(m, m->W)) return false;
-  HIPCHK(launch_advance_time(m->dtbits, m->p.cfl, m->tdt, m->st));
-  if (m->need_wo) HIPCHK(hipMemcpyAsync(m->Wo, m->W, 9 * N * 8, hipMemcpyDeviceToDevice, m->st));
+  HIPCHK(launch_advance_time(m->G, m->dtbits, m->p.cfl, m->tdt, m->st));
+  if (m->need_wo) HIPCHK(launch_copy(m->G, m->Wo, m->W, 9 * N, m->st));
   if (!run_stage(m, 1, m->Ucur, nullptr, m->Uh)) return false;
   if (m->p.irkt == 2) {
     if (!run_stage(m, 2, m->Ucur, m->Uh, m->Unext)) return false;

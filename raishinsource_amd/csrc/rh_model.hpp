@@ -62,6 +62,7 @@ struct raishin_model {
   const double* graph_W = nullptr;
   const double* graph_U = nullptr;
   bool graph_mixed = false;
+  size_t dev_bytes = 0;  // device memory allocated by build_model
 };
 
 namespace rhm {
@@ -72,9 +73,12 @@ void set_error(const char* fmt, ...);
 int ghost_nm0(int irec);
 bool check_pram(const raishin_pram& p);
 
+extern thread_local size_t g_dalloc_bytes;  // running total of dalloc sizes on this thread
+
 template <typename T>
 bool dalloc(T** p, size_t n) {
   HIPCHK(hipMalloc((void**)p, sizeof(T) * (n ? n : 1)));
+  g_dalloc_bytes += sizeof(T) * (n ? n : 1);
   return true;
 }
 void dfree_all(raishin_model* m);

@@ -50,7 +50,9 @@ EXPORTS = (
     "raishin_cdtcfl4", "raishin_init", "raishin_grid", "raishin_coord1", "raishin_calchrist",
     "raishin_mdtorus1", "raishin_model_set_timing", "raishin_model_get_timing",
     "raishin_model_debug_stage", "raishin_model_set_precision", "raishin_model_step_graph",
-    "raishin_models_step_graph",
+    "raishin_models_step_graph", "raishin_model_device_bytes", "raishin_batch_create", "raishin_batch_destroy",
+    "raishin_batch_set", "raishin_batch_step", "raishin_batch_step_graph", "raishin_batch_times",
+    "raishin_batch_sync",
 )
 
 # reference-named entry points (include/raishin_ref.h)
@@ -106,6 +108,18 @@ def lib():
         L.raishin_model_set_precision.argtypes = [vp, C.c_int]
         L.raishin_model_step_graph.argtypes = [vp, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_double)]
         L.raishin_models_step_graph.argtypes = [C.POINTER(vp), C.c_int, C.c_int]
+        L.raishin_model_device_bytes.argtypes = [vp]
+        L.raishin_model_device_bytes.restype = C.c_longlong
+        L.raishin_batch_create.restype = vp
+        L.raishin_batch_destroy.argtypes = [vp]
+        L.raishin_batch_set.argtypes = [vp, C.POINTER(vp), C.c_int]
+        L.raishin_batch_step.argtypes = [vp, C.c_int]
+        L.raishin_batch_step_graph.argtypes = [vp, C.c_int]
+        L.raishin_batch_times.argtypes = [vp, C.POINTER(C.c_double)]
+        L.raishin_batch_sync.argtypes = [vp]
+        for f in ("raishin_batch_set", "raishin_batch_step", "raishin_batch_step_graph", "raishin_batch_times",
+                  "raishin_batch_sync"):
+            getattr(L, f).restype = C.c_int
         _lib = L
     return _lib
 
@@ -194,6 +208,56 @@ def step_models_graph(models, npairs):
     check()
     for mm in models:
         mm.sync()
+
+
+class Batch:
+    """Resident models advanced together, one launch per kernel of the step
+    for all members (raishin_batch_* of include/raishin_hip.h)."""
+
+    def __init__(self, models=()):
+        self.h = lib().raishin_batch_create()
+        if not self.h:
+            check()
+            raise RaishinError("raishin_batch_create failed")
+        self.models = []
+        if models:
+            self.set(models)
+
+    def _rc(self, rc):
+        if rc != 0:
+            check()
+            raise RaishinError("raishin_batch call failed")
+
+    def set(self, models):
+        self.models = list(models)
+        arr = (C.c_void_p * max(len(self.models), 1))(*[mm.h for mm in self.models])
+        self._rc(lib().raishin_batch_set(self.h, arr, len(self.models)))
+
+    def step(self, nsteps=1):
+        self._rc(lib().raishin_batch_step(self.h, nsteps))
+
+    def step_graph(self, npairs=1):
+        self._rc(lib().raishin_batch_step_graph(self.h, npairs))
+
+    def times(self):
+        n = len(self.models)
+        buf = (C.c_double * max(n, 1))()
+        self._rc(lib().raishin_batch_times(self.h, buf))
+        return [buf[q] for q in range(n)]
+
+    def sync(self):
+        self._rc(lib().raishin_batch_sync(self.h))
+
+    def close(self):
+        if self.h:
+            lib().raishin_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 class Model:
@@ -292,6 +356,9 @@ class Model:
         lib().raishin_model_get_timing(self.h, ms, cnt, n)
         check()
         return {k: (ms[q], int(cnt[q])) for q, k in enumerate(KERNEL_KINDS)}
+
+    def device_bytes(self):
+        return int(lib().raishin_model_device_bytes(self.h))
 
     @property
     def stream(self):
