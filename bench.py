@@ -111,6 +111,10 @@ def main():
     ap.add_argument("--ensemble", type=int, default=64,
                     help="models of the 2D-torus ensemble run (0: skip)")
     ap.add_argument("--ensemble-steps", type=int, default=500, help="steps per ensemble model (even)")
+    ap.add_argument("--serve-rates", default="10,100,500",
+                    help="Poisson arrival rates (requests/s) of the serving runs, comma-separated ('' skips)")
+    ap.add_argument("--serve-requests", type=int, default=40, help="requests per serving run")
+    ap.add_argument("--serve-budget-gib", type=float, default=64.0, help="HBM budget of the serving loop")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {args.gpus}): every box here has one MI355X and "
@@ -217,6 +221,8 @@ def main():
         line["mixed"] = mixed_precision(p, g, m, uri, args, el / args.steps)
     if args.ensemble > 0:
         line["ensemble"] = ensemble(args)
+    if args.serve_rates:
+        line["serving"] = serve(args)
     print(json.dumps(line))
     return 0
 
@@ -320,6 +326,67 @@ def ensemble(args):
                                     speedup_vs_one_at_a_time=t_seq / t_mix),
         "setup_seconds": t_set,
     }
+
+
+def serve(args):
+    """Continuous batching of Poisson arrivals (60% 1D SRMHD tubes of 1024
+    zones to t=0.4, 30% 2D tori as in the ensemble for 500 steps, 10% 3D 64^3
+    tori for 100 steps; FP64 or mixed per request), admitted by device
+    footprint, against the same requests served one at a time.  One request
+    mix (seed 1) is used at every rate, its arrival times scaled by the rate.
+    One at a time: each request built and run alone, in arrival order,
+    starting when it has arrived and the previous one has finished (service
+    times measured once, queueing computed per rate)."""
+    from dataclasses import replace as dc_replace
+
+    import numpy as np
+
+    from raishinsource_amd import serving
+
+    rates = [float(x) for x in args.serve_rates.split(",") if x]
+    base = serving.make_requests(args.serve_requests, 1.0, seed=1)
+    budget = int(args.serve_budget_gib * (1 << 30))
+    # one at a time: service times (build + steps) of each request, alone
+    service = {}
+    zu_alone = 0
+    for req in base:
+        r, tb, ts = serving.run_alone(req, keep_state=False, strict=False)
+        service[req.rid] = (tb, ts, r.steps)
+        zu_alone += req.zones * r.steps
+        print(f"[bench] serving: request {req.rid} ({req.kind}) alone: build {tb:.2f}s, {r.steps} steps {ts:.2f}s",
+              file=sys.stderr, flush=True)
+    print(f"[bench] serving: {len(base)} requests run one at a time: "
+          f"{sum(a + b for a, b, _ in service.values()):.1f}s of service", file=sys.stderr, flush=True)
+    out = {"requests": len(base), "hbm_budget_gib": args.serve_budget_gib,
+           "mix": {k: sum(1 for r in base if r.kind == k) for k in ("tube", "torus2d", "torus3d")},
+           "mixed_precision_requests": sum(1 for r in base if r.mixed), "rates": {}}
+    for lam in rates:
+        reqs = [dc_replace(r, arrival=r.arrival / lam) for r in base]
+        srv = serving.Server(hbm_budget=budget)
+        t0 = time.perf_counter()
+        res = srv.run(reqs)
+        wall = time.perf_counter() - t0
+        srv.close()
+        summ = serving.summarize(res, wall)
+        summ["ticks"] = srv.ticks
+        summ["deferred_by_budget"] = len(srv.deferred)
+        same_steps = all(r.steps == service[r.rid][2] for r in res)
+        # the single worker: FIFO queue over the measured service times
+        free_at, lat = 0.0, []
+        for r in sorted(reqs, key=lambda x: x.arrival):
+            tb, ts, _ = service[r.rid]
+            start = max(r.arrival, free_at)
+            free_at = start + tb + ts
+            lat.append(free_at - r.arrival)
+        wall1 = free_at - min(r.arrival for r in reqs)
+        one = {"p50_s": float(np.percentile(lat, 50)), "p99_s": float(np.percentile(lat, 99)),
+               "zone_updates_per_s": zu_alone / wall1, "wall_s": wall1}
+        out["rates"][str(int(lam) if lam == int(lam) else lam)] = {
+            "batched": summ, "one_at_a_time": one, "same_step_counts": bool(same_steps)}
+        print(f"[bench] serving at {lam:g}/s: batched p50 {summ['p50_s']:.2f}s p99 {summ['p99_s']:.2f}s "
+              f"{summ['zone_updates_per_s']:.3g} zu/s; one at a time p50 {one['p50_s']:.2f}s "
+              f"p99 {one['p99_s']:.2f}s {one['zone_updates_per_s']:.3g} zu/s", file=sys.stderr, flush=True)
+    return out
 
 
 def mixed_precision(p, g, m, uri, args, s_per_step_fp64):

@@ -46,6 +46,19 @@ int raishin_mdtorus1(const raishin_pram *p, const raishin_torus *t, double *uri,
                      const double *gcon, const double *detg, const double *x1, const double *x2,
                      const double *x3);
 
+/* Riemann-tube states (left [0], right [1]): density, pressure, 3-velocity,
+ * magnetic field. */
+typedef struct {
+  double de[2], pr[2];
+  double v[2][3];
+  double b[2][3];
+} raishin_tube;
+
+/* Primitive variables uri(9,imax,jmax,kmax) of a Riemann tube on a flat
+ * metric, jump at i = imax/2 (mdgrmhd.f90:55 mdshoctub with the states as
+ * parameters); returns 0, or 1 for a non-physical state. */
+int raishin_mdshoctub(const raishin_pram *p, const raishin_tube *t, double *uri);
+
 #ifdef __cplusplus
 }
 #endif

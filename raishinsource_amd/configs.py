@@ -103,4 +103,25 @@ G_MP5 = Pram("gmp5", 12, 9, 10, irec=6, irkt=3, cfl=0.5)
 G_WENO = Pram("gweno", 12, 9, 10, irec=7, irkt=3, cfl=0.5)
 T3D = Pram("t3d", 32, 16, 32, irec=6, irkt=3, cfl=0.5)
 
-ALL = {p.name: p for p in (C1, C2, C2_PPM, C2_WENO, U_PPM, U_MP5, U_WENO, T2D, T3D, G_PPM, G_MP5, G_WENO)}
+
+
+def tube(name, nzones, nm1=5):
+    """1D special-relativistic MHD Riemann tube: Minkowski (metric 1),
+    Gamma 5/3, model 2 (mdshoctub, mdgrmhd.f90:55), x1 in [1, 2] uniform
+    over nzones interior cells, one interior cell in x2 and x3 (cube cells),
+    PPM, RK2, CFL 0.4; outflow (copy) in x1, periodic in x2 and x3."""
+    imax = nzones + 2 * nm1
+    d = 1.0 / (imax - 1)
+    w = d * (2 * nm1)
+    return Pram(name, imax, 2 * nm1 + 1, 2 * nm1 + 1, irec=5, irkt=2, cfl=0.4, gam=5.0 / 3.0, model=2, metric=1,
+                ix1=1, xmin=1.0, xmax=2.0, ymin=0.0, ymax=w, zmin=0.0, zmax=w,
+                iboux1in=(5,) * 9, iboux1ot=(5,) * 9, iboux2in=(1,) * 9, iboux2ot=(1,) * 9,
+                iboux3in=(1,) * 9, iboux3ot=(1,) * 9)
+
+
+# the serving mix's tube (1024 zones) and a small one for the oracle
+TUBE = tube("tube", 1024)
+U_TUBE = tube("utube", 64)
+
+ALL = {p.name: p for p in (C1, C2, C2_PPM, C2_WENO, U_PPM, U_MP5, U_WENO, T2D, T3D, G_PPM, G_MP5, G_WENO, TUBE,
+                           U_TUBE)}

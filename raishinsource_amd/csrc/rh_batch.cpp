@@ -160,7 +160,8 @@ bool make_plan(raishin_batch* b) {
   } else {
     HIPCHK(hipStreamSynchronize(b->st));
   }
-  HIPCHK(hipMemcpy(b->items, b->host_items.data(), need * sizeof(BatchItem), hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpyAsync(b->items, b->host_items.data(), need * sizeof(BatchItem), hipMemcpyHostToDevice, b->st));
+  HIPCHK(hipStreamSynchronize(b->st));
   // time gather table
   if ((int)n > b->tcap) {
     if (b->tptr) HIPCHK(hipFree((void*)b->tptr));
@@ -174,7 +175,10 @@ bool make_plan(raishin_batch* b) {
   }
   std::vector<const double*> tp(n);
   for (size_t q = 0; q < n; q++) tp[q] = b->ms[q]->tdt;
-  if (n) HIPCHK(hipMemcpy((void*)b->tptr, tp.data(), n * sizeof(double*), hipMemcpyHostToDevice));
+  if (n) {
+    HIPCHK(hipMemcpyAsync((void*)b->tptr, tp.data(), n * sizeof(double*), hipMemcpyHostToDevice, b->st));
+    HIPCHK(hipStreamSynchronize(b->st));
+  }
   b->parity = 0;
   b->planned = true;
   return true;

@@ -236,10 +236,7 @@ void raishin_model_sync(raishin_model* m) {
 
 void* raishin_model_stream(raishin_model* m) { return (void*)m->st; }
 
-long long raishin_model_device_bytes(raishin_model* m) {
-  // build_model's allocations plus the staging buffer set_state/get_state add
-  return m ? (long long)m->dev_bytes + (m->stage_aos ? 0LL : 9LL * m->G.N * 8) : 0;
-}
+long long raishin_model_device_bytes(raishin_model* m) { return m ? (long long)m->dev_bytes : 0; }
 
 void raishin_model_debug_stage(raishin_model* m, int stage, double* ww, double* wpre) {
   // stage 1 also runs the CFL reduction and time advance of a step

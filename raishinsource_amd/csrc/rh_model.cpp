@@ -25,6 +25,7 @@ void set_error(const char* fmt, ...) {
   va_end(ap);
   std::lock_guard<std::mutex> lk(g_err_mu);
   if (g_err.empty()) g_err = buf;  // sticky: keep the first error
+  (void)hipGetLastError();         // recorded: later launches on this thread report their own errors
 }
 
 
@@ -163,6 +164,15 @@ bool upload(double* dst, const double* src, size_t n, hipStream_t st) {
   return true;
 }
 
+// Host-to-device copy ordered on the model's stream, complete on return.  No
+// copy here goes through the legacy default stream: models are built on
+// worker threads while other streams may be capturing a graph.
+bool h2d_sync(void* dst, const void* src, size_t bytes, hipStream_t st) {
+  HIPCHK(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, st));
+  HIPCHK(hipStreamSynchronize(st));
+  return true;
+}
+
 // interp's coefficients that depend only on the cell widths (rec.f90:1353-1355,
 // 1427-1436), per index q of the dummy width array ds (n+2 entries), in the
 // reference's expressions and evaluation order: planes 0..2 t1a,t1b,t1c of
@@ -234,6 +244,7 @@ static bool build_model_(raishin_model* m, const raishin_pram* p, const double* 
     return false;
   if (m->need_wo && !dalloc(&m->Wo, 9 * N)) return false;
   if (p->irec == 5 && !dalloc(&m->E, 18 * N)) return false;  // PPM face values
+  if (!dalloc(&m->stage_aos, 9 * N)) return false;            // set_state / get_state staging
   m->crows = ((N + 255) / 256) * 4;  // wavefronts of the largest launch (every cell, 256 per workgroup)
   if (!dalloc(&m->cpart, (size_t)m->crows * NCOUNT)) return false;
   HIPCHK(hipMemsetAsync(m->cpart, 0, (size_t)m->crows * NCOUNT * 8, m->st));
@@ -253,7 +264,7 @@ static bool build_model_(raishin_model* m, const raishin_pram* p, const double* 
   for (int a = 0; a < 4; a++) {
     if (!metric_tables(*p, gcv[a], gcn[a], dtg[a], tab, names[a])) return false;
     if (!dalloc(&m->met[a], tab.size())) return false;
-    HIPCHK(hipMemcpy(m->met[a], tab.data(), tab.size() * 8, hipMemcpyHostToDevice));
+    if (!h2d_sync(m->met[a], tab.data(), tab.size() * 8, m->st)) return false;
   }
   {
     std::vector<double> ct(64 * G.P2);
@@ -267,7 +278,7 @@ static bool build_model_(raishin_model* m, const raishin_pram* p, const double* 
             else if (std::memcmp(&v, &ct[e * G.P2 + q], 8)) { set_error("Christoffel symbols depend on j"); return false; }
           }
         }
-    HIPCHK(hipMemcpy(m->chr, ct.data(), ct.size() * 8, hipMemcpyHostToDevice));
+    if (!h2d_sync(m->chr, ct.data(), ct.size() * 8, m->st)) return false;
     // Gamma^l_{ab} = Gamma^l_{ba} bit for bit lets the source kernel read each pair once
     m->chr_sym = true;
     for (int l = 0; l < 4 && m->chr_sym; l++)
@@ -291,24 +302,24 @@ static bool build_model_(raishin_model* m, const raishin_pram* p, const double* 
       if (q > n - 1) q = n - 1;
       ds[d] = (n > 1) ? dsrc[a][q - 1] : 0.0;
     }
-    HIPCHK(hipMemcpy(m->xs[a], xsrc[a], n * 8, hipMemcpyHostToDevice));
-    HIPCHK(hipMemcpy(m->dxb[a], db.data(), n * 8, hipMemcpyHostToDevice));
-    HIPCHK(hipMemcpy(m->dxs[a], ds.data(), (n + 2) * 8, hipMemcpyHostToDevice));
+    if (!h2d_sync(m->xs[a], xsrc[a], n * 8, m->st)) return false;
+    if (!h2d_sync(m->dxb[a], db.data(), n * 8, m->st)) return false;
+    if (!h2d_sync(m->dxs[a], ds.data(), (n + 2) * 8, m->st)) return false;
     if (p->irec == 5) {
       const std::vector<double> pc = ppm_coefficients(ds);
       if (!dalloc(&m->ppmc[a], pc.size())) return false;
-      HIPCHK(hipMemcpy(m->ppmc[a], pc.data(), pc.size() * 8, hipMemcpyHostToDevice));
+      if (!h2d_sync(m->ppmc[a], pc.data(), pc.size() * 8, m->st)) return false;
     }
   }
   // r-power tables from the host libm, as the reference computes them (correction.f90:318-319)
   std::vector<double> r15(G.ni), r25(G.ni);
   for (int i = 0; i < G.ni; i++) { r15[i] = std::pow(x1[i], -1.5); r25[i] = std::pow(x1[i], -2.5); }
-  HIPCHK(hipMemcpy(m->r15, r15.data(), G.ni * 8, hipMemcpyHostToDevice));
-  HIPCHK(hipMemcpy(m->r25, r25.data(), G.ni * 8, hipMemcpyHostToDevice));
+  if (!h2d_sync(m->r15, r15.data(), G.ni * 8, m->st)) return false;
+  if (!h2d_sync(m->r25, r25.data(), G.ni * 8, m->st)) return false;
   int bch[54];
   const int* bsrc[6] = {p->iboux1in, p->iboux1ot, p->iboux2in, p->iboux2ot, p->iboux3in, p->iboux3ot};
   for (int a = 0; a < 6; a++) for (int q = 0; q < 9; q++) bch[a * 9 + q] = bsrc[a][q];
-  HIPCHK(hipMemcpy(m->bc, bch, sizeof bch, hipMemcpyHostToDevice));
+  if (!h2d_sync(m->bc, bch, sizeof bch, m->st)) return false;
   HIPCHK(hipStreamSynchronize(m->st));
   return true;
 }

@@ -463,3 +463,30 @@ int raishin_mdtorus1(const raishin_pram* pp, const raishin_torus* tp, double* ur
 }
 
 }  // extern "C"
+
+int raishin_mdshoctub(const raishin_pram* pp, const raishin_tube* t, double* uri) {
+  // mdshoctub (mdgrmhd.f90:55-117) with the two states as parameters: cells
+  // i <= imax/2 take the left state, the rest the right; flat metric, so
+  // u~^i = gamma v^i.  uri(6) = p / rho**(gam-1) (mdgrmhd.f90:103).
+  const raishin_pram& p = *pp;
+  double w[2][9];
+  for (int s = 0; s < 2; s++) {
+    const double vsq = t->v[s][0] * t->v[s][0] + t->v[s][1] * t->v[s][1] + t->v[s][2] * t->v[s][2];
+    if (!(t->de[s] > 0.0) || !(t->pr[s] > 0.0) || !(vsq < 1.0)) return 1;
+    const double lor = 1.0 / std::sqrt(1.0 - vsq);
+    w[s][0] = t->de[s];
+    for (int a = 0; a < 3; a++) w[s][1 + a] = lor * t->v[s][a];
+    w[s][4] = t->pr[s];
+    w[s][5] = t->pr[s] * (1.0 / std::pow(t->de[s], p.gam - 1.0));
+    for (int a = 0; a < 3; a++) w[s][6 + a] = t->b[s][a];
+  }
+  for (int k = 1; k <= p.kmax; k++)
+    for (int j = 1; j <= p.jmax; j++)
+      for (int i = 1; i <= p.imax; i++) {
+        const double* ws = w[i <= p.imax / 2 ? 0 : 1];
+        const size_t c = cell(p, i, j, k);
+        for (int n = 0; n < 9; n++) uri[9 * c + n] = ws[n];
+      }
+  return 0;
+}
+

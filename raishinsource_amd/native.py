@@ -39,6 +39,11 @@ class GridExtS(C.Structure):
     _fields_ = [("ix1", C.c_int)] + [(n, C.c_double) for n in ("xmin", "xmax", "ymin", "ymax", "zmin", "zmax")]
 
 
+class TubeS(C.Structure):
+    _fields_ = [("de", C.c_double * 2), ("pr", C.c_double * 2), ("v", (C.c_double * 3) * 2),
+                ("b", (C.c_double * 3) * 2)]
+
+
 class TorusS(C.Structure):
     _fields_ = [(n, C.c_double) for n in ("rin", "rmax", "beta", "acut", "noise")] + [("seed", C.c_ulonglong)]
 
@@ -52,7 +57,7 @@ EXPORTS = (
     "raishin_model_debug_stage", "raishin_model_set_precision", "raishin_model_step_graph",
     "raishin_models_step_graph", "raishin_model_device_bytes", "raishin_batch_create", "raishin_batch_destroy",
     "raishin_batch_set", "raishin_batch_step", "raishin_batch_step_graph", "raishin_batch_times",
-    "raishin_batch_sync",
+    "raishin_batch_sync", "raishin_mdshoctub",
 )
 
 # reference-named entry points (include/raishin_ref.h)
@@ -101,6 +106,8 @@ def lib():
         L.raishin_calchrist.argtypes = [C.POINTER(PramS)] + [vp] * 4
         L.raishin_mdtorus1.argtypes = [C.POINTER(PramS), C.POINTER(TorusS)] + [vp] * 7
         L.raishin_mdtorus1.restype = C.c_int
+        L.raishin_mdshoctub.argtypes = [C.POINTER(PramS), C.POINTER(TubeS), vp]
+        L.raishin_mdshoctub.restype = C.c_int
         L.raishin_model_set_timing.argtypes = [vp, C.c_int]
         L.raishin_model_get_timing.argtypes = [vp, C.POINTER(C.c_double), C.POINTER(C.c_longlong), C.c_int]
         L.raishin_model_get_timing.restype = C.c_int
@@ -198,6 +205,37 @@ def setup_torus(p: Pram, g, m, rin=6.0, rmax=12.0, beta=100.0, acut=0.2, noise=0
     if rc != 0:
         raise RaishinError("torus set-up found no torus cells")
     return uri
+
+
+def setup_tube(p: Pram, left, right):
+    """Riemann tube primitives (after mdgrmhd.f90:55 mdshoctub); left and
+    right are dicts with de, pr, v (3-velocity) and b (3 components)."""
+    p = p.resolved()
+    uri = fzeros(9, p.imax, p.jmax, p.kmax)
+    t = TubeS()
+    for s, st in enumerate((left, right)):
+        t.de[s] = st["de"]
+        t.pr[s] = st["pr"]
+        for a in range(3):
+            t.v[s][a] = st["v"][a]
+            t.b[s][a] = st["b"][a]
+    if lib().raishin_mdshoctub(C.byref(pram_struct(p)), C.byref(t), _p(uri)) != 0:
+        raise RaishinError("non-physical tube state")
+    return uri
+
+
+def random_tube_states(rng):
+    """Left/right states: rho and p log-uniform in [0.01, 1], B components
+    U[-1, 1], velocity components U[-0.5, 0.5] (redrawn until |v| < 1)."""
+    def one():
+        de, pr = np.exp(rng.uniform(np.log(0.01), np.log(1.0), 2))
+        while True:
+            v = rng.uniform(-0.5, 0.5, 3)
+            if float(v @ v) < 1.0:
+                break
+        return {"de": float(de), "pr": float(pr), "v": [float(x) for x in v],
+                "b": [float(x) for x in rng.uniform(-1.0, 1.0, 3)]}
+    return one(), one()
 
 
 def step_models_graph(models, npairs):

@@ -233,3 +233,34 @@ def test_rk3_single_precision_constants(oracle_lib):
     assert third32 != 1.0 / 3.0
     expect = (uo + 2.0 * us) * third32 + 0.0 * float(np.float32(2.0) / np.float32(3.0))
     assert np.array_equal(un[s], expect[s])
+
+
+def test_tube_uniform_state_stays_uniform(oracle_lib):
+    """A Riemann tube with equal left and right states on the flat metric:
+    every interface sees the same face states, so the flux divergence is
+    exactly zero and every cell along the tube keeps one value (bit for bit),
+    within the recovery's tolerance of the initial state.  (The x2-x3 corner
+    ghosts keep their initial values: the periodic pass copies only over the
+    interior ranges of the other two axes, bnd.f90:50-70.)"""
+    p = configs.U_TUBE.resolved()
+    g = native.setup_grid(p)
+    m = native.setup_metric(p, g)
+    st = {"de": 0.3, "pr": 0.2, "v": [0.1, -0.2, 0.05], "b": [0.3, 0.2, -0.1]}
+    uri = native.setup_tube(p, st, st)
+    o = Oracle(p)
+    uu = fzeros(9, p.imax, p.jmax, p.kmax)
+    o.call("caluu1", uri, uu, m["gcov"], m["gcon"], m["detg"])
+    w = uri.copy(order="F")
+    t = 0.0
+    for _ in range(5):
+        dt, t = o.step(uu, w, g, m, t)
+    o.lib.orc_free_work()
+    j = p.nm1
+    line = w[:, :, j, j]
+    for v in range(9):
+        assert np.unique(line[v]).size == 1, v
+    assert np.allclose(line[[0, 1, 2, 3, 4, 6, 7, 8], 0], uri[[0, 1, 2, 3, 4, 6, 7, 8], 0, 0, 0], rtol=1e-7)
+    # recov1a rewrites the entropy tracer as rho * p / rho**(gam-1) (recov.f90:1186)
+    assert np.isclose(line[5, 0], 0.3 * 0.2 / 0.3 ** (p.gam - 1.0), rtol=1e-7)
+    # a tube of cube cells at |v| < 1: dt = cfl * dx / (fastest speed) with speed < 1
+    assert dt > 0.4 * g["dx1b"][1]

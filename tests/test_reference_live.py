@@ -16,7 +16,7 @@ def need(name):
         pytest.skip(f"reference build oracle/_ref/{name} absent (python oracle/build_ref.py {name})")
 
 
-@pytest.mark.parametrize("name", ["uppm", "ump5", "t2d"])
+@pytest.mark.parametrize("name", ["uppm", "ump5", "t2d", "utube"])
 def test_setup_matches_reference(name):
     """grid.f90:2, coord.f90:5 (+caldetg), coord.f90:1089 against raishin_grid/coord1/calchrist."""
     need(name)
@@ -54,14 +54,18 @@ def test_routines_match_reference(name):
     assert bits_equal(wr, wo) and bits_equal(ur, uo)
 
 
-@pytest.mark.parametrize("name,steps", [("t2d", 3), ("t3d", 2)])
+@pytest.mark.parametrize("name,steps", [("t2d", 3), ("t3d", 2), ("utube", 6)])
 def test_step_matches_reference(name, steps):
     """Composed step: reference routines in main.f90's order (bnd4/infchk from the
-    oracle, bnd.f90 is not buildable here) against orc_step; NaN-aware."""
+    oracle, bnd.f90 is not buildable here) against orc_step; NaN-aware.  The
+    tube runs the flat metric (metric 1) and model 2 with random states."""
     need(name)
     r, g, m = grid_and_metric(name)
     p = r.p
-    uri = native.setup_torus(p, g, m)
+    if p.metric == 1:
+        uri = native.setup_tube(p, *native.random_tube_states(np.random.default_rng(5)))
+    else:
+        uri = native.setup_torus(p, g, m)
     o = Oracle(p)
     uu = fzeros(9, p.imax, p.jmax, p.kmax)
     o.call("caluu1", uri, uu, m["gcov"], m["gcon"], m["detg"])
