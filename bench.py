@@ -111,9 +111,10 @@ def main():
     ap.add_argument("--ensemble", type=int, default=64,
                     help="models of the 2D-torus ensemble run (0: skip)")
     ap.add_argument("--ensemble-steps", type=int, default=500, help="steps per ensemble model (even)")
-    ap.add_argument("--serve-rates", default="10,100,500",
+    ap.add_argument("--no-autotune", action="store_true", help="keep 256-thread workgroups everywhere")
+    ap.add_argument("--serve-rates", default="10,500",
                     help="Poisson arrival rates (requests/s) of the serving runs, comma-separated ('' skips)")
-    ap.add_argument("--serve-requests", type=int, default=40, help="requests per serving run")
+    ap.add_argument("--serve-requests", type=int, default=12, help="requests per serving run")
     ap.add_argument("--serve-budget-gib", type=float, default=64.0, help="HBM budget of the serving loop")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -132,6 +133,8 @@ def main():
     model.set_state(uri)
     t_setup = time.perf_counter() - t_setup
     print(f"[bench] {args.config} {p.imax}x{p.jmax}x{p.kmax} set up in {t_setup:.1f}s", file=sys.stderr, flush=True)
+    block_sizes = model.block_sizes() if args.no_autotune else model.autotune(uri, steps=2)
+    print(f"[bench] workgroup sizes {block_sizes}", file=sys.stderr, flush=True)
 
     for _ in range(args.warmup):
         model.step(1, sync=False)
@@ -214,6 +217,7 @@ def main():
         },
         "kernel_avg_ms": kernel_ms,
         "cpu_baseline": cpu,
+        "workgroup_sizes": block_sizes,
         "sim_time": sim_time,
         "counters": counters,
     }
@@ -261,8 +265,17 @@ def ensemble(args):
         models.append(mod)
         states.append(uri)
         zones += ni * nk
+    # the autotuner once per grid shape; the other models of that shape take its choice
+    tuned = {}
+    if not args.no_autotune:
+        for mod, uri in zip(models, states):
+            key = (mod.p.imax, mod.p.kmax)
+            if key in tuned:
+                mod.set_block_sizes(tuned[key])
+            else:
+                tuned[key] = mod.autotune(uri, steps=2)
     t_set = time.perf_counter() - t_set
-    print(f"[bench] ensemble: {len(models)} models set up in {t_set:.1f}s", file=sys.stderr, flush=True)
+    print(f"[bench] ensemble: {len(models)} models set up (and tuned) in {t_set:.1f}s", file=sys.stderr, flush=True)
     for mod, uri in zip(models, states):
         mod.set_state(uri)
         mod.sync()
@@ -325,6 +338,7 @@ def ensemble(args):
         "batched_graph_mixed": dict(rate(t_mix), max_rel_l1_density_vs_fp64=max(l1),
                                     speedup_vs_one_at_a_time=t_seq / t_mix),
         "setup_seconds": t_set,
+        "workgroup_sizes_by_shape": {f"{k[0] - 10}x{k[1] - 10}": v for k, v in tuned.items()},
     }
 
 
@@ -389,6 +403,37 @@ def serve(args):
     return out
 
 
+def divb_max(uu, g, p):
+    """Largest |div B| over the corners of the updated cells: the divergence
+    of the densitised field (uu(7..9) = sqrt(-g) B^i) at each corner from
+    the four cell-centred values on each side, divided by the distance
+    between the cell centres.  The reference does not keep it at round-off
+    (its HLLE flux of the normal field component is nonzero), so it is
+    reported for FP64 and mixed side by side."""
+    import numpy as np
+
+    n = p.nm1
+    core = (slice(n, p.imax - n), slice(n, p.jmax - n), slice(n, p.kmax - n))
+    Bs = [np.asarray(uu[6 + a])[core] for a in range(3)]
+    xs = [np.asarray(g["x1"])[n:p.imax - n], np.asarray(g["x2"])[n:p.jmax - n], np.asarray(g["x3"])[n:p.kmax - n]]
+    if min(b.shape[a] for a, b in enumerate(Bs)) < 2:
+        return None
+    div = 0.0
+    for a in range(3):
+        B = Bs[a]
+        d = np.diff(B, axis=a) / np.diff(xs[a]).reshape([-1 if q == a else 1 for q in range(3)])
+        # average over the 2x2 cells around the corner in the other two axes
+        for q in range(3):
+            if q != a:
+                sl0 = [slice(None)] * 3
+                sl1 = [slice(None)] * 3
+                sl0[q] = slice(0, -1)
+                sl1[q] = slice(1, None)
+                d = 0.5 * (d[tuple(sl0)] + d[tuple(sl1)])
+        div = div + d
+    return float(np.abs(div).max())
+
+
 def mixed_precision(p, g, m, uri, args, s_per_step_fp64):
     """The mixed form (reconstruction and face fluxes in FP32): its rate on
     the same timed steps, and its per-field errors against the FP64 run after
@@ -400,6 +445,7 @@ def mixed_precision(p, g, m, uri, args, s_per_step_fp64):
     model = native.Model(p, g, m)
     model.set_state(uri)
     model.set_precision(True)
+    sizes = model.block_sizes() if args.no_autotune else model.autotune(uri, steps=2)
     for _ in range(args.warmup):
         model.step(1, sync=False)
     model.set_timing(True)
@@ -419,11 +465,16 @@ def mixed_precision(p, g, m, uri, args, s_per_step_fp64):
         mod.set_precision(mixed)
         mod.step(args.mixed_steps, sync=False)
         w, u, t = mod.get_state()
-        runs[mixed] = (w, mod.counters(), t)
+        runs[mixed] = (w, mod.counters(), t, u)
         mod.close()
         print(f"[bench] {'mixed' if mixed else 'fp64'} {args.mixed_steps} steps done", file=sys.stderr, flush=True)
-    w64, c64, t64 = runs[False]
-    w32, c32, t32 = runs[True]
+    w64, c64, t64, u64 = runs[False]
+    w32, c32, t32, u32 = runs[True]
+    uu0 = native.Model(p, g, m)
+    uu0.set_state(uri)
+    u0 = uu0.get_state()[1]
+    uu0.close()
+    divb = {"initial": divb_max(u0, g, p), "fp64": divb_max(u64, g, p), "mixed": divb_max(u32, g, p)}
     n1 = p.nm1
     inner = (slice(None), slice(n1, p.imax - n1), slice(n1, p.jmax - n1), slice(n1, p.kmax - n1))
     a, b = w32[inner], w64[inner]
@@ -437,12 +488,14 @@ def mixed_precision(p, g, m, uri, args, s_per_step_fp64):
         "speedup_vs_fp64": (s_per_step_fp64 * args.steps) / el,
         "dtype": "f32 reconstruction + face fluxes, f64 elsewhere",
         "kernel_avg_ms": {k: round(v[0] / max(v[1], 1), 4) for k, v in kt.items()},
+        "workgroup_sizes": sizes,
         "error_steps": args.mixed_steps,
         "rel_l1_vs_fp64": l1,
         "rel_linf_vs_fp64": linf,
         "sim_time": {"fp64": t64, "mixed": t32},
         "recov_fallback": {"fp64": c64["recov_fallback"], "mixed": c32["recov_fallback"]},
         "recov_fail": {"fp64": c64["recov_fail"], "mixed": c32["recov_fail"]},
+        "max_abs_div_b": divb,
     }
 
 

@@ -120,6 +120,18 @@ void raishin_model_set_timing(raishin_model *m, int on);
 void raishin_model_debug_stage(raishin_model *m, int stage, double *ww, double *wpre);
 int raishin_model_get_timing(raishin_model *m, double *ms, long long *launches, int nkinds);
 
+/* Workgroup sizes of the tuned launches, in the order: PPM face values,
+ * sweep i, sweep j, sweep k, flux-CT, RK update, recov1a, ghosts/floors/
+ * caluu1 (8 entries; 0 = 256, else 64..256 in steps of 64).  Results do not
+ * depend on them.  get returns the sizes in use. */
+int raishin_model_set_block_sizes(raishin_model *m, const int *bs, int n);
+int raishin_model_get_block_sizes(raishin_model *m, int *bs, int n);
+/* Autotuner: for each candidate size (64, 128, 256) runs one warm-up step
+ * and `steps` timed steps from the state (uri, uu or NULL, time), keeps the
+ * fastest size per launch slot, then sets the state again and clears the
+ * counters, so the run starts as if untuned.  Returns 0 or -1. */
+int raishin_model_autotune(raishin_model *m, const double *uri, const double *uu, double time, int steps);
+
 /* Device memory the model holds, in bytes (state, fluxes, metric tables and
  * work arrays): what admission of a new model has to find free. */
 long long raishin_model_device_bytes(raishin_model *m);

@@ -34,7 +34,7 @@ bool same_ptrs(const Ptrs& a, const Ptrs& b) {
 }
 
 struct Op {
-  int kind, variant;
+  int kind, variant, threads;
   long long item0;  // first entry in the device table
   int nitems;
   long long blocks;
@@ -121,20 +121,21 @@ bool make_plan(raishin_batch* b) {
           set_error("raishin_batch: members make different step sequences (irec, irkt and the boundary types must agree)");
           return false;
         }
-      // one launch per variant present at this position, in order of first appearance
-      std::vector<int> variants;
+      // one launch per (variant, workgroup size) present at this position,
+      // in order of first appearance
+      std::vector<std::pair<int, int>> variants;
       for (size_t q = 0; q < n; q++) {
-        const int v = rec[h][q][pos].variant;
+        const std::pair<int, int> v(rec[h][q][pos].variant, rec[h][q][pos].threads);
         bool seen = false;
-        for (int x : variants) seen = seen || x == v;
+        for (const auto& x : variants) seen = seen || x == v;
         if (!seen) variants.push_back(v);
       }
       const bool per_model = one_group_per_model(kind);
-      for (int v : variants) {
-        Op op{kind, v, (long long)b->host_items.size(), 0, 0};
+      for (const auto& v : variants) {
+        Op op{kind, v.first, v.second, (long long)b->host_items.size(), 0, 0};
         for (size_t q = 0; q < n; q++) {
           const LaunchRec& r = rec[h][q][pos];
-          if (r.variant != v || r.nb <= 0) continue;
+          if (r.variant != v.first || r.threads != v.second || r.nb <= 0) continue;
           BatchItem it;
           std::memset(&it, 0, sizeof it);
           it.G = r.G;
@@ -203,7 +204,7 @@ bool release_members(raishin_batch* b) {
 bool run_half(raishin_batch* b) {
   const int h = b->parity;
   for (const Op& op : b->half[h])
-    HIPCHK(launch_batched(op.kind, op.variant, b->items + op.item0, op.nitems, op.blocks, b->st));
+    HIPCHK(launch_batched(op.kind, op.variant, op.threads, b->items + op.item0, op.nitems, op.blocks, b->st));
   b->parity ^= 1;
   for (size_t q = 0; q < b->ms.size(); q++) set_ptrs(b->ms[q], b->at[b->parity][q]);
   return true;
@@ -284,7 +285,7 @@ int raishin_batch_step_graph(raishin_batch* b, int npairs) {
       bool ok = true;
       for (int h = 0; h < 2 && ok; h++)
         for (const Op& op : b->half[h])
-          if (launch_batched(op.kind, op.variant, b->items + op.item0, op.nitems, op.blocks, b->st) != hipSuccess) {
+          if (launch_batched(op.kind, op.variant, op.threads, b->items + op.item0, op.nitems, op.blocks, b->st) != hipSuccess) {
             ok = false;
             break;
           }

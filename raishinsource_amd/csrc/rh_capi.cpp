@@ -280,4 +280,59 @@ int raishin_model_get_timing(raishin_model* m, double* ms, long long* launches, 
   return K_NKINDS;
 }
 
+int raishin_model_set_block_sizes(raishin_model* m, const int* bs, int n) {
+  if (!m || !bs || n != TS_N) { set_error("raishin_model_set_block_sizes: expected %d sizes", (int)TS_N); return -1; }
+  for (int q = 0; q < TS_N; q++)
+    if (bs[q] != 0 && (bs[q] < 64 || bs[q] > 256 || bs[q] % 64)) {
+      set_error("raishin_model_set_block_sizes: size %d of slot %d is not 0 or 64, 128, 192, 256", bs[q], q);
+      return -1;
+    }
+  for (int q = 0; q < TS_N; q++) m->G.bs[q] = (unsigned short)bs[q];
+  // a captured graph holds the old launch shapes
+  if (m->gexec) { (void)hipGraphExecDestroy(m->gexec); m->gexec = nullptr; }
+  if (m->graph) { (void)hipGraphDestroy(m->graph); m->graph = nullptr; }
+  return 0;
+}
+
+int raishin_model_get_block_sizes(raishin_model* m, int* bs, int n) {
+  if (!m || !bs || n != TS_N) { set_error("raishin_model_get_block_sizes: expected %d sizes", (int)TS_N); return -1; }
+  for (int q = 0; q < TS_N; q++) bs[q] = slot_threads(m->G, q);
+  return 0;
+}
+
+int raishin_model_autotune(raishin_model* m, const double* uri, const double* uu, double time, int steps) {
+  if (!m || !uri || steps < 1) { set_error("raishin_model_autotune: bad arguments"); return -1; }
+  // timed kind of each tuned slot
+  const int kind_of[TS_N] = {K_EDGES, K_SWEEP1, K_SWEEP2, K_SWEEP3, K_FLUXCT, K_UPDATE, K_RECOV, K_POST};
+  double best_ms[TS_N];
+  int best[TS_N];
+  for (int q = 0; q < TS_N; q++) { best_ms[q] = 1e300; best[q] = 256; }
+  const bool timing = m->timing;
+  for (int cand : TUNE_CANDIDATES) {
+    int bs[TS_N];
+    for (int q = 0; q < TS_N; q++) bs[q] = cand;
+    if (raishin_model_set_block_sizes(m, bs, TS_N) != 0) return -1;
+    raishin_model_set_state(m, uri, uu, time);
+    if (!run_step(m)) return -1;  // warm-up
+    raishin_model_set_timing(m, 1);
+    for (int s = 0; s < steps; s++)
+      if (!run_step(m)) return -1;
+    double ms[K_NKINDS];
+    long long cnt[K_NKINDS];
+    raishin_model_get_timing(m, ms, cnt, K_NKINDS);
+    raishin_model_set_timing(m, 0);
+    for (int q = 0; q < TS_N; q++) {
+      const int k = kind_of[q];
+      const double avg = cnt[k] ? ms[k] / (double)cnt[k] : 1e300;
+      if (avg < best_ms[q]) { best_ms[q] = avg; best[q] = cand; }
+    }
+  }
+  if (raishin_model_set_block_sizes(m, best, TS_N) != 0) return -1;
+  // start again from the given state, as if no step had been taken
+  raishin_model_set_state(m, uri, uu, time);
+  raishin_model_reset_counters(m);
+  raishin_model_set_timing(m, timing ? 1 : 0);
+  return 0;
+}
+
 }  // extern "C"

@@ -31,6 +31,9 @@ struct Geo {
   int nm0, nm1;          // ghost widths (main.f90:179-211)
   long long N;           // ni*nj*nk
   long long P2;          // ni*nk: plane size of the axisymmetric (i,k) tables
+  // workgroup size per tuned launch slot (TuneSlot in rh_kernels.hpp);
+  // 0 means 256.  Chosen by raishin_model_autotune or set by the caller.
+  unsigned short bs[8];
 };
 
 // One metric location (centre or a face) from the (i,k) tables:

@@ -1382,9 +1382,9 @@ static thread_local std::vector<LaunchRec>* g_rec = nullptr;
 
 void set_recorder(std::vector<LaunchRec>* r) { g_rec = r; }
 
-static void record(int kind, int variant, long long nb, const Geo& G, const AnyArgs& A) {
+static void record(int kind, int variant, int threads, long long nb, const Geo& G, const AnyArgs& A) {
   LaunchRec r;
-  r.kind = kind; r.variant = variant; r.nb = nb; r.G = G; r.A = A;
+  r.kind = kind; r.variant = variant; r.threads = threads; r.nb = nb; r.G = G; r.A = A;
   g_rec->push_back(r);
 }
 
@@ -1403,10 +1403,11 @@ hipError_t launch_ppm_edges(const Geo& G, int dir, const double* W, const double
   const long long n = box(G.nm0, G.ni - G.nm0 + (dir == 1)) * box(G.nm0, G.nj - G.nm0 + (dir == 2)) *
                       box(G.nm0, G.nk - G.nm0 + (dir == 3));
   EdgeArgs A{W, dxs, xs, pc, gam, E};
-  const unsigned nb = nblk(n, 256);
-  if (g_rec) { AnyArgs a{}; a.e = A; record(LK_EDGES, dir + 4 * (int)mixed, nb, G, a); return hipSuccess; }
+  const int bt = slot_threads(G, TS_EDGES);
+  const unsigned nb = nblk(n, bt);
+  if (g_rec) { AnyArgs a{}; a.e = A; record(LK_EDGES, dir + 4 * (int)mixed, bt, nb, G, a); return hipSuccess; }
   if (n == 0) return hipSuccess;
-#define L(Dd, T) hipLaunchKernelGGL((k_ppm_edges<Dd, T>), dim3(nb), dim3(256), 0, st, G, A)
+#define L(Dd, T) hipLaunchKernelGGL((k_ppm_edges<Dd, T>), dim3(nb), dim3(bt), 0, st, G, A)
   if (mixed) { if (dir == 1) L(1, float); else if (dir == 2) L(2, float); else L(3, float); }
   else { if (dir == 1) L(1, double); else if (dir == 2) L(2, double); else L(3, double); }
 #undef L
@@ -1417,10 +1418,16 @@ hipError_t launch_sweep(const Geo& G, int dir, int irec, const double* W, const 
                         double gam, double* F, double* CTsrc, bool mixed, hipStream_t st) {
   const long long n = box(G.nm0, G.ni - G.nm0) * box(G.nm0, G.nj - G.nm0) * box(G.nm0, G.nk - G.nm0);
   SweepArgs A{W, metF, E, gam, F, CTsrc};
-  const unsigned nb = nblk(n, 256);
-  if (g_rec) { AnyArgs a{}; a.s = A; record(LK_SWEEP, dir + 4 * (int)mixed + 8 * irec, nb, G, a); return hipSuccess; }
+  const int bt = slot_threads(G, TS_SWEEP1 + dir - 1);
+  const unsigned nb = nblk(n, bt);
+  if (g_rec) {
+    AnyArgs a{};
+    a.s = A;
+    record(LK_SWEEP, dir + 4 * (int)mixed + 8 * irec, bt, nb, G, a);
+    return hipSuccess;
+  }
   if (n == 0) return hipSuccess;
-#define L(Dd, Rc, T) hipLaunchKernelGGL((k_sweep<Dd, Rc, T>), dim3(nb), dim3(256), 0, st, G, A)
+#define L(Dd, Rc, T) hipLaunchKernelGGL((k_sweep<Dd, Rc, T>), dim3(nb), dim3(bt), 0, st, G, A)
 #define LT(Dd, Rc) \
   do { if (mixed) L(Dd, Rc, float); else L(Dd, Rc, double); } while (0)
   if (irec == 5) { if (dir == 1) LT(1, 5); else if (dir == 2) LT(2, 5); else LT(3, 5); }
@@ -1434,20 +1441,22 @@ hipError_t launch_sweep(const Geo& G, int dir, int irec, const double* W, const 
 hipError_t launch_fluxct(const Geo& G, const double* CTsrc, double* F1, double* F2, double* F3, hipStream_t st) {
   const long long n = box(G.nm1, G.ni - G.nm1) * box(G.nm1, G.nj - G.nm1) * box(G.nm1, G.nk - G.nm1);
   CtArgs A{CTsrc, F1, F2, F3};
-  if (g_rec) { AnyArgs a{}; a.c = A; record(LK_FLUXCT, 0, nblk(n, 256), G, a); return hipSuccess; }
+  const int bt = slot_threads(G, TS_FLUXCT);
+  if (g_rec) { AnyArgs a{}; a.c = A; record(LK_FLUXCT, 0, bt, nblk(n, bt), G, a); return hipSuccess; }
   if (n == 0) return hipSuccess;
-  hipLaunchKernelGGL(k_fluxct, dim3(nblk(n, 256)), dim3(256), 0, st, G, A);
+  hipLaunchKernelGGL(k_fluxct, dim3(nblk(n, bt)), dim3(bt), 0, st, G, A);
   return hipGetLastError();
 }
 
 hipError_t launch_update(const Geo& G, int stage, const StageArgs& A, hipStream_t st) {
   const long long n = box(G.nm1 + 1, G.ni - G.nm1) * box(G.nm1 + 1, G.nj - G.nm1) * box(G.nm1 + 1, G.nk - G.nm1);
-  const unsigned nb = nblk(n, 256);
-  if (g_rec) { AnyArgs a{}; a.st = A; record(LK_RK, stage + 8 * (int)A.chr_sym, nb, G, a); return hipSuccess; }
+  const int bt = slot_threads(G, TS_RK);
+  const unsigned nb = nblk(n, bt);
+  if (g_rec) { AnyArgs a{}; a.st = A; record(LK_RK, stage + 8 * (int)A.chr_sym, bt, nb, G, a); return hipSuccess; }
   if (n == 0) return hipSuccess;
 #define L(S) \
-  do { if (A.chr_sym) hipLaunchKernelGGL((k_rk<S, true>), dim3(nb), dim3(256), 0, st, G, A); \
-       else hipLaunchKernelGGL((k_rk<S, false>), dim3(nb), dim3(256), 0, st, G, A); } while (0)
+  do { if (A.chr_sym) hipLaunchKernelGGL((k_rk<S, true>), dim3(nb), dim3(bt), 0, st, G, A); \
+       else hipLaunchKernelGGL((k_rk<S, false>), dim3(nb), dim3(bt), 0, st, G, A); } while (0)
   if (stage == 1) L(1);
   else if (stage == 2) L(2);
   else if (stage == 3) L(3);
@@ -1458,14 +1467,15 @@ hipError_t launch_update(const Geo& G, int stage, const StageArgs& A, hipStream_
 
 hipError_t launch_recov(const Geo& G, const StageArgs& A, hipStream_t st) {
   const long long n = box(G.nm1 + 1, G.ni - G.nm1) * box(G.nm1 + 1, G.nj - G.nm1) * box(G.nm1 + 1, G.nk - G.nm1);
-  if (g_rec) { AnyArgs a{}; a.st = A; record(LK_RECOV, 0, nblk(n, 256), G, a); return hipSuccess; }
+  const int bt = slot_threads(G, TS_RECOV);
+  if (g_rec) { AnyArgs a{}; a.st = A; record(LK_RECOV, 0, bt, nblk(n, bt), G, a); return hipSuccess; }
   if (n == 0) return hipSuccess;
-  hipLaunchKernelGGL(k_recov, dim3(nblk(n, 256)), dim3(256), 0, st, G, A);
+  hipLaunchKernelGGL(k_recov, dim3(nblk(n, bt)), dim3(bt), 0, st, G, A);
   return hipGetLastError();
 }
 
 hipError_t launch_recov_carry(const Geo& G, const FixArgs& A, hipStream_t st) {
-  if (g_rec) { AnyArgs a{}; a.fx = A; record(LK_CARRY, 0, 1, G, a); return hipSuccess; }
+  if (g_rec) { AnyArgs a{}; a.fx = A; record(LK_CARRY, 0, 1024, 1, G, a); return hipSuccess; }
   hipLaunchKernelGGL(k_recov_carry, dim3(1), dim3(1024), 0, st, G, A);
   return hipGetLastError();
 }
@@ -1473,15 +1483,16 @@ hipError_t launch_recov_carry(const Geo& G, const FixArgs& A, hipStream_t st) {
 hipError_t launch_post(const Geo& G, const PostArgs& A, hipStream_t st) {
   const long long g = G.nm1;
   const long long ng = G.N - (long long)(G.ni - 2 * g) * (G.nj - 2 * g) * (G.nk - 2 * g);
+  const int bt = slot_threads(G, TS_POST);
   if (g_rec) {
     AnyArgs a{};
     a.p = A;
-    record(LK_BND, 0, ng > 0 ? nblk(ng, 256) : 0, G, a);
-    record(LK_POST, 0, nblk(G.N, 256), G, a);
+    record(LK_BND, 0, bt, ng > 0 ? nblk(ng, bt) : 0, G, a);
+    record(LK_POST, 0, bt, nblk(G.N, bt), G, a);
     return hipSuccess;
   }
-  if (ng > 0) hipLaunchKernelGGL(k_bnd, dim3(nblk(ng, 256)), dim3(256), 0, st, G, A);
-  hipLaunchKernelGGL(k_post, dim3(nblk(G.N, 256)), dim3(256), 0, st, G, A);
+  if (ng > 0) hipLaunchKernelGGL(k_bnd, dim3(nblk(ng, bt)), dim3(bt), 0, st, G, A);
+  hipLaunchKernelGGL(k_post, dim3(nblk(G.N, bt)), dim3(bt), 0, st, G, A);
   return hipGetLastError();
 }
 
@@ -1502,8 +1513,8 @@ hipError_t launch_cfl(const Geo& G, const CflArgs& A0, hipStream_t st) {
   if (g_rec) {
     AnyArgs a{};
     a.cf = A;
-    record(LK_CFL, 0, nb, G, a);
-    record(LK_CFL_FINISH, 0, 1, G, a);
+    record(LK_CFL, 0, 256, nb, G, a);
+    record(LK_CFL_FINISH, 0, 1024, 1, G, a);
     return hipSuccess;
   }
   hipLaunchKernelGGL(k_cfl, dim3(nb), dim3(256), 0, st, G, A);
@@ -1514,7 +1525,7 @@ hipError_t launch_cfl(const Geo& G, const CflArgs& A0, hipStream_t st) {
 hipError_t launch_advance_time(const Geo& G, const unsigned long long* dtbits, double cfl, double* tdt,
                                hipStream_t st) {
   AdvArgs A{dtbits, cfl, tdt};
-  if (g_rec) { AnyArgs a{}; a.a = A; record(LK_ADVANCE, 0, 1, G, a); return hipSuccess; }
+  if (g_rec) { AnyArgs a{}; a.a = A; record(LK_ADVANCE, 0, 64, 1, G, a); return hipSuccess; }
   hipLaunchKernelGGL(k_advance_time, dim3(1), dim3(64), 0, st, A);
   return hipGetLastError();
 }
@@ -1523,17 +1534,18 @@ hipError_t launch_copy(const Geo& G, double* dst, const double* src, long long n
   if (g_rec) {
     AnyArgs a{};
     a.cp = CopyArgs{src, dst, n};
-    record(LK_COPY, 0, nblk(n, 256), G, a);
+    record(LK_COPY, 0, 256, nblk(n, 256), G, a);
     return hipSuccess;
   }
   return hipMemcpyAsync(dst, src, (size_t)n * sizeof(double), hipMemcpyDeviceToDevice, st);
 }
 
-hipError_t launch_batched(int kind, int variant, const BatchItem* it, int n, long long total_blocks,
+hipError_t launch_batched(int kind, int variant, int threads, const BatchItem* it, int n, long long total_blocks,
                           hipStream_t st) {
   if (n <= 0 || total_blocks <= 0) return hipSuccess;
   if (total_blocks > 0x7fffffffLL) return hipErrorInvalidValue;
-  const dim3 g((unsigned)total_blocks), b(256);
+  if (!one_group_per_model(kind) && (threads < 64 || threads > 256 || threads % 64)) return hipErrorInvalidValue;
+  const dim3 g((unsigned)total_blocks), b((unsigned)threads);
   switch (kind) {
     case LK_EDGES: {
       const int d = variant & 3;

@@ -88,6 +88,14 @@ struct CopyArgs {
   long long n;
 };
 
+// Launch slots whose workgroup size is tuned per model (Geo::bs): the PPM
+// edges (all directions), the three sweeps, flux-CT, the RK update, recov1a,
+// and the ghost fill with floors and caluu1.  The CFL reduction keeps 256
+// (its per-block partial results are sized for it).
+enum TuneSlot { TS_EDGES, TS_SWEEP1, TS_SWEEP2, TS_SWEEP3, TS_FLUXCT, TS_RK, TS_RECOV, TS_POST, TS_N };
+constexpr int TUNE_CANDIDATES[3] = {64, 128, 256};
+inline int slot_threads(const Geo& G, int slot) { return G.bs[slot] ? (int)G.bs[slot] : 256; }
+
 // Launch kinds of one step, in the order run_step makes them.
 enum LaunchKind {
   LK_EDGES, LK_SWEEP, LK_FLUXCT, LK_RK, LK_RECOV, LK_CARRY, LK_BND, LK_POST, LK_CFL, LK_CFL_FINISH,
@@ -110,6 +118,7 @@ union AnyArgs {
 // 4*mixed; sweep: dir + 4*mixed + 8*irec; rk: stage + 8*sym), workgroups.
 struct LaunchRec {
   int kind, variant;
+  int threads;                     // workgroup size
   long long nb;
   Geo G;
   AnyArgs A;
@@ -128,7 +137,7 @@ struct BatchItem {
 void set_recorder(std::vector<LaunchRec>* r);
 // One launch of a kind and variant over a table of n models (total_blocks
 // workgroups; for one-workgroup-per-model kinds, n).
-hipError_t launch_batched(int kind, int variant, const BatchItem* items, int n, long long total_blocks,
+hipError_t launch_batched(int kind, int variant, int threads, const BatchItem* items, int n, long long total_blocks,
                           hipStream_t st);
 int block_threads(int kind);
 // dst[q] = src[q][0] for q < n (the members' times of a batch)

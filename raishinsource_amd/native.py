@@ -57,8 +57,11 @@ EXPORTS = (
     "raishin_model_debug_stage", "raishin_model_set_precision", "raishin_model_step_graph",
     "raishin_models_step_graph", "raishin_model_device_bytes", "raishin_batch_create", "raishin_batch_destroy",
     "raishin_batch_set", "raishin_batch_step", "raishin_batch_step_graph", "raishin_batch_times",
-    "raishin_batch_sync", "raishin_mdshoctub",
+    "raishin_batch_sync", "raishin_mdshoctub", "raishin_model_set_block_sizes", "raishin_model_get_block_sizes",
+    "raishin_model_autotune",
 )
+
+TUNE_SLOTS = ("ppm_edges", "sweep_i", "sweep_j", "sweep_k", "fluxct", "update", "recov", "post")
 
 # reference-named entry points (include/raishin_ref.h)
 REF_EXPORTS = (
@@ -115,6 +118,11 @@ def lib():
         L.raishin_model_set_precision.argtypes = [vp, C.c_int]
         L.raishin_model_step_graph.argtypes = [vp, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_double)]
         L.raishin_models_step_graph.argtypes = [C.POINTER(vp), C.c_int, C.c_int]
+        L.raishin_model_set_block_sizes.argtypes = [vp, C.POINTER(C.c_int), C.c_int]
+        L.raishin_model_get_block_sizes.argtypes = [vp, C.POINTER(C.c_int), C.c_int]
+        L.raishin_model_autotune.argtypes = [vp, vp, vp, C.c_double, C.c_int]
+        for f in ("raishin_model_set_block_sizes", "raishin_model_get_block_sizes", "raishin_model_autotune"):
+            getattr(L, f).restype = C.c_int
         L.raishin_model_device_bytes.argtypes = [vp]
         L.raishin_model_device_bytes.restype = C.c_longlong
         L.raishin_batch_create.restype = vp
@@ -394,6 +402,30 @@ class Model:
         lib().raishin_model_get_timing(self.h, ms, cnt, n)
         check()
         return {k: (ms[q], int(cnt[q])) for q, k in enumerate(KERNEL_KINDS)}
+
+    def block_sizes(self):
+        """{launch slot: workgroup size} in use."""
+        a = (C.c_int * len(TUNE_SLOTS))()
+        if lib().raishin_model_get_block_sizes(self.h, a, len(TUNE_SLOTS)) != 0:
+            check()
+        return dict(zip(TUNE_SLOTS, list(a)))
+
+    def set_block_sizes(self, sizes):
+        """sizes: dict by slot name or sequence of 8 (0 = 256)."""
+        if isinstance(sizes, dict):
+            sizes = [sizes.get(k, 0) for k in TUNE_SLOTS]
+        a = (C.c_int * len(TUNE_SLOTS))(*sizes)
+        if lib().raishin_model_set_block_sizes(self.h, a, len(TUNE_SLOTS)) != 0:
+            check()
+
+    def autotune(self, uri, uu=None, time=0.0, steps=2):
+        """Pick the fastest workgroup size per launch slot on this model
+        (then the state is uri again and the counters are clear)."""
+        if lib().raishin_model_autotune(self.h, _p(np.asfortranarray(uri)),
+                                        _p(None if uu is None else np.asfortranarray(uu)), time, steps) != 0:
+            check()
+        check()
+        return self.block_sizes()
 
     def device_bytes(self):
         return int(lib().raishin_model_device_bytes(self.h))

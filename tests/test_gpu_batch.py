@@ -127,3 +127,29 @@ def test_batch_rejects_mixed_schemes():
         b.set([a, a])
     b.close()
     close_all([a, c])
+
+
+def test_workgroup_sizes_do_not_change_results():
+    """The autotuner's choice, explicit odd choices, and members of one batch
+    with different sizes: bit-identical to the default launch shapes."""
+    members = [(configs.T2D, False, 0, 100.0), (configs.U_PPM, True, 1, 100.0),
+               (replace(configs.U_PPM, name="uppm16", jmax=16), False, 2, 100.0)]
+    ref = [make(p, mx, sd, bt) for p, mx, sd, bt in members]
+    tuned = [make(p, mx, sd, bt) for p, mx, sd, bt in members]
+    for (p, mx, sd, bt), mm in zip(members, tuned):
+        pr = p.resolved()
+        g = native.setup_grid(pr)
+        m = native.setup_metric(pr, g)
+        uri = native.setup_torus(pr, g, m, seed=sd, noise=0.02, beta=bt)
+        sizes = mm.autotune(uri, steps=1)
+        assert set(sizes.values()) <= {64, 128, 256}, sizes
+    tuned[0].set_block_sizes([64, 192, 128, 64, 192, 128, 64, 192])
+    for mm in ref:
+        mm.step(3)
+    for mm in tuned:
+        mm.step(1)
+    b = native.Batch(tuned)
+    b.step(2)
+    assert_same(ref, tuned, "tuned")
+    b.close()
+    close_all(ref, tuned)
