@@ -496,7 +496,16 @@ RH_DEV void recov_cell(const double u[9], const double wold[9], const Met& M, do
   const double vsqo = vsq;
   double xxn = wg, yyn = vsq, w3, gtmp;
   // 2D Newton-Raphson on (W, v^2): recov.f90:877-960.  Trips after convergence
-  // change nothing in the reference, so the loop stops there.
+  // change nothing in the reference, so the loop stops there.  A cell whose
+  // iteration never meets the tolerance runs all iter trips in the reference;
+  // such an iteration usually falls into a short exact cycle of states.  Once
+  // a state repeats (Brent's cycle detection) while the tolerance still fails,
+  // every later trip repeats the cycle, so whole periods are skipped (with
+  // their det=0 events) and the last trips run as usual: the loop ends in the
+  // state, and with the count, of the reference's full run.
+  unsigned long long cyx = (unsigned long long)__double_as_longlong(xxn);
+  unsigned long long cyy = (unsigned long long)__double_as_longlong(yyn);
+  int cy_pow = 1, cy_len = 0, cy_det0 = 0;
   for (int nnn = 1; nnn <= iter; nnn++) {
     if (!(fabs(delta) >= deltaend && irecov == 0)) break;
     if (nnn == iter) { irecov = 1; break; }
@@ -514,7 +523,7 @@ RH_DEV void recov_cell(const double u[9], const double wold[9], const Met& M, do
     const double dgx = -1.0 - (qdotbsq * (1.0 / w3)) + dpdw;
     const double dgy = -0.5 * bsq + dpdvsq;
     double det = dfx * dgy - dfy * dgx;
-    if (det == 0.0) { o.det0 = 1; det = 1.0; }
+    if (det == 0.0) { o.det0 += 1; det = 1.0; }
     const double dx = (-dgy * f1 + dfy * g1) * (1.0 / det);
     const double dy = (dgx * f1 - dfx * g1) * (1.0 / det);
     delta = 0.5 * (fabs(dx) + fabs(dy));
@@ -524,6 +533,17 @@ RH_DEV void recov_cell(const double u[9], const double wold[9], const Met& M, do
     if (xxn > saftybig) xxn = saftybig;
     if (yyn < 0.0) yyn = fabs(yyn);
     if (fabs(yyn) > 1.0) yyn = 1.0 - dv;
+    const unsigned long long bx = (unsigned long long)__double_as_longlong(xxn);
+    const unsigned long long by = (unsigned long long)__double_as_longlong(yyn);
+    cy_len++;
+    if (bx == cyx && by == cyy && fabs(delta) >= deltaend) {
+      const int periods = (iter - 1 - nnn) / cy_len;
+      nnn += periods * cy_len;
+      o.det0 += periods * (o.det0 - cy_det0);
+      cy_pow = 0x40000000;
+    } else if (cy_len == cy_pow) {
+      cyx = bx; cyy = by; cy_pow <<= 1; cy_len = 0; cy_det0 = o.det0;
+    }
   }
   if (irecov == 0) {
     wg = xxn; vsq = yyn;
@@ -548,6 +568,9 @@ RH_DEV void recov_cell(const double u[9], const double wold[9], const Met& M, do
     o.fallback = 1;
     delta = 1.0; deltaend = 1.0e-6; irecov = 0;
     xxn = vsqo;
+    // same cycle skipping as the 2D iteration; the state is v^2 alone
+    unsigned long long cyv = (unsigned long long)__double_as_longlong(xxn);
+    int ey_pow = 1, ey_len = 0;
     for (int nnn = 1; nnn <= iter; nnn++) {
       if (!(fabs(delta) >= deltaend && irecov == 0)) break;
       if (nnn == iter) { irecov = 1; break; }
@@ -573,6 +596,14 @@ RH_DEV void recov_cell(const double u[9], const double wold[9], const Met& M, do
       xxn = vsq - delta;
       if (xxn < 0.0) xxn = fabs(xxn);
       if (fabs(xxn) >= 1.0) xxn = 1.0 - dv;
+      const unsigned long long bv = (unsigned long long)__double_as_longlong(xxn);
+      ey_len++;
+      if (bv == cyv && fabs(delta) >= deltaend) {
+        nnn += ((iter - 1 - nnn) / ey_len) * ey_len;
+        ey_pow = 0x40000000;
+      } else if (ey_len == ey_pow) {
+        cyv = bv; ey_pow <<= 1; ey_len = 0;
+      }
     }
     if (irecov == 0) {
       vsq = xxn;
@@ -721,7 +752,7 @@ RH_DEV void b_recov(const Geo& G, const StageArgs& A, long long blk) {
   count_wave(A.cpart, blk, C_NEGRHO, ro.negrho && !dep);
   count_wave(A.cpart, blk, C_NEGP, ro.negp && !dep);
   count_wave(A.cpart, blk, C_VSQGT1, ro.vsqgt1 && !dep);
-  count_wave(A.cpart, blk, C_DET0, ro.det0 && !dep);
+  if (ro.det0 && !dep) atomicAdd(&A.cnt[C_DET0], (unsigned long long)ro.det0);  // rare: one event per det=0 trip
   count_wave(A.cpart, blk, C_WGCARRY, carry);
   count_wave(A.cpart, blk, C_GUARD, ro.guard && !dep);
   if (dep) {
@@ -1190,7 +1221,7 @@ RH_DEV void b_recov_carry(const Geo& G, const FixArgs& A) {
       if (ro.negrho) atomicAdd(&A.cnt[C_NEGRHO], 1ULL);
       if (ro.negp) atomicAdd(&A.cnt[C_NEGP], 1ULL);
       if (ro.vsqgt1) atomicAdd(&A.cnt[C_VSQGT1], 1ULL);
-      if (ro.det0) atomicAdd(&A.cnt[C_DET0], 1ULL);
+      if (ro.det0) atomicAdd(&A.cnt[C_DET0], (unsigned long long)ro.det0);
       if (ro.guard) atomicAdd(&A.cnt[C_GUARD], 1ULL);
       pend[e] = 2;
       s_progress = 1;

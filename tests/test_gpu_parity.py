@@ -277,7 +277,7 @@ def test_steps_match_oracle(name, state, nsteps, oracle_lib):
 # derivatives); the device pow (ROCm ocml) and the host's glibc pow differ in
 # the last bit for some arguments, and those differences grow over the steps.
 # Same counters, same step sequence; per-field agreement to this tolerance.
-TUBE_TOL = 1e-9
+TUBE_TOL = 1e-8
 
 
 @pytest.mark.parametrize("name,seed,nsteps", [("utube", 1, 40), ("utube", 2, 40), ("tube", 3, 12)])
@@ -294,6 +294,28 @@ def test_tube_matches_oracle(name, seed, nsteps, oracle_lib):
     assert_close(name + " dt", dtg[None], dto[None], TUBE_TOL)
     assert_close(name + " uri", wg, wo, TUBE_TOL)
     assert_close(name + " uu", ug, uo, TUBE_TOL)
+    assert_counters(cg, co)
+
+
+def test_tube_failing_inversions_match_oracle(oracle_lib):
+    """A strongly magnetised tube whose recovery often fails the 2D Newton
+    iteration (all iter trips in the reference) and sometimes the entropy
+    iteration too: the device skips whole periods of cycling iterations and
+    must end in the reference's states and counts."""
+    from raishinsource_amd import serving
+    req = serving.make_requests(40, 1.0, seed=1)[33]
+    assert req.kind == "tube"
+    p = configs.TUBE.resolved()
+    g = native.setup_grid(p)
+    m = native.setup_metric(p, g)
+    uri = native.setup_tube(p, req.init["left"], req.init["right"])
+    wo, uo, to, dto, co = oracle_run(p, g, m, uri, 20)
+    wg, ug, tg, dtg, cg = gpu_run(p, g, m, uri, 20)
+    print("oracle counters", co, "gpu counters", cg)
+    assert co["recov_fallback"] > 0 and co["recov_fail"] > 0
+    assert_close("tube dt", dtg[None], dto[None], TUBE_TOL)
+    assert_close("tube uri", wg, wo, TUBE_TOL)
+    assert_close("tube uu", ug, uo, TUBE_TOL)
     assert_counters(cg, co)
 
 
