@@ -112,9 +112,9 @@ def main():
                     help="models of the 2D-torus ensemble run (0: skip)")
     ap.add_argument("--ensemble-steps", type=int, default=500, help="steps per ensemble model (even)")
     ap.add_argument("--no-autotune", action="store_true", help="keep 256-thread workgroups everywhere")
-    ap.add_argument("--serve-rates", default="10,500",
+    ap.add_argument("--serve-rates", default="10,100,500",
                     help="Poisson arrival rates (requests/s) of the serving runs, comma-separated ('' skips)")
-    ap.add_argument("--serve-requests", type=int, default=12, help="requests per serving run")
+    ap.add_argument("--serve-requests", type=int, default=40, help="requests per serving run")
     ap.add_argument("--serve-budget-gib", type=float, default=64.0, help="HBM budget of the serving loop")
     args = ap.parse_args()
     if args.gpus != 1:

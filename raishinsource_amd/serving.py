@@ -190,7 +190,7 @@ def run_alone(req, keep_state=True, strict=True):
 class Server:
     """Continuous batching of arriving requests on one GPU."""
 
-    def __init__(self, hbm_budget=64 << 30, max_members=512, max_tick_steps=8, workers=4, keep_state=False):
+    def __init__(self, hbm_budget=64 << 30, max_members=512, max_tick_steps=32, workers=4, keep_state=False):
         self.budget = int(hbm_budget)
         self.max_members = max_members
         self.max_tick = max_tick_steps
