@@ -132,6 +132,15 @@ int raishin_model_get_block_sizes(raishin_model *m, int *bs, int n);
  * counters, so the run starts as if untuned.  Returns 0 or -1. */
 int raishin_model_autotune(raishin_model *m, const double *uri, const double *uu, double time, int steps);
 
+/* Axisymmetric models with one interior phi cell (jmax = 2*nm1+1), periodic
+ * phi and non-periodic r and theta keep every phi row identical at all
+ * times.  When raishin_model_set_state receives a state whose rows are
+ * bit-identical, the model stores and computes one row (results unchanged,
+ * get_state returns every row); allow = 0 keeps all rows.  stored_rows
+ * returns the phi rows in use (1 or jmax). */
+void raishin_model_allow_collapse(raishin_model *m, int allow);
+int raishin_model_stored_rows(raishin_model *m);
+
 /* Device memory the model holds, in bytes (state, fluxes, metric tables and
  * work arrays): what admission of a new model has to find free. */
 long long raishin_model_device_bytes(raishin_model *m);

@@ -51,6 +51,7 @@ struct raishin_batch {
   std::vector<Op> half[2];        // the launches of a step starting at parity p
   std::vector<Ptrs> at[2];        // each member's pointers at the start of half p
   std::vector<bool> at_mixed;
+  std::vector<unsigned> at_gen;   // each member's geometry generation
   std::vector<BatchItem> host_items;
   BatchItem* items = nullptr;
   size_t items_cap = 0;
@@ -74,7 +75,9 @@ void drop_graph(raishin_batch* b) {
 bool plan_valid(const raishin_batch* b) {
   if (!b->planned) return false;
   for (size_t q = 0; q < b->ms.size(); q++)
-    if (!same_ptrs(ptrs_of(b->ms[q]), b->at[b->parity][q]) || b->ms[q]->mixed != b->at_mixed[q]) return false;
+    if (!same_ptrs(ptrs_of(b->ms[q]), b->at[b->parity][q]) || b->ms[q]->mixed != b->at_mixed[q] ||
+        b->ms[q]->geo_gen != b->at_gen[q])
+      return false;
   return true;
 }
 
@@ -89,6 +92,7 @@ bool make_plan(raishin_batch* b) {
   b->at[0].resize(n);
   b->at[1].resize(n);
   b->at_mixed.resize(n);
+  b->at_gen.resize(n);
   for (size_t q = 0; q < n; q++) {
     raishin_model* m = b->ms[q];
     const bool timing = m->timing;
@@ -104,6 +108,7 @@ bool make_plan(raishin_batch* b) {
     if (!ok) return false;
     if (!same_ptrs(ptrs_of(m), b->at[0][q])) { set_error("raishin_batch: a member's buffers did not return after two steps"); return false; }
     b->at_mixed[q] = m->mixed;
+    b->at_gen[q] = m->geo_gen;
   }
   b->host_items.clear();
   for (int h = 0; h < 2; h++) {

@@ -16,6 +16,15 @@ extern std::mutex g_err_mu;
 extern std::string g_err;
 }
 
+#define HIPCHK_VOID(x)                                                              \
+  do {                                                                              \
+    hipError_t e_ = (x);                                                            \
+    if (e_ != hipSuccess) {                                                         \
+      ::rhm::set_error("%s:%d %s: %s", __FILE__, __LINE__, #x, hipGetErrorString(e_)); \
+      return;                                                                       \
+    }                                                                               \
+  } while (0)
+
 extern "C" {
 
 int raishin_last_error(char* buf, int len) {
@@ -66,13 +75,26 @@ void raishin_model_destroy(raishin_model* m) {
 }
 
 void raishin_model_set_state(raishin_model* m, const double* uri, const double* uu, double time) {
+  // one stored phi row when every row of the state is the same (and stays
+  // so: periodic phi with one interior cell, r and theta not periodic)
+  const int j0 = m->G.nm1 + 1;
+  const bool collapse = collapsible(m) && rows_equal(m->G, uri, j0) && (!uu || rows_equal(m->G, uu, j0));
+  HIPCHK_VOID(hipStreamSynchronize(m->st));
+  set_rows(m, collapse);
   const long long N = m->G.N;
+  std::vector<double> packed;
+  auto src = [&](const double* full) -> const double* {
+    if (!m->G.jfix) return full;
+    pack_row(m->G, full, packed);
+    return packed.data();
+  };
   auto go = [&]() -> bool {
     if (!m->stage_aos && !dalloc(&m->stage_aos, 9 * N)) return false;
-    if (!upload(m->stage_aos, uri, 9 * N, m->st)) return false;
+    if (!upload(m->stage_aos, src(uri), 9 * N, m->st)) return false;
     HIPCHK(launch_aos_to_soa(m->stage_aos, m->W, N, m->st));
+    HIPCHK(hipStreamSynchronize(m->st));  // the packed row is reused below
     if (uu) {
-      if (!upload(m->stage_aos, uu, 9 * N, m->st)) return false;
+      if (!upload(m->stage_aos, src(uu), 9 * N, m->st)) return false;
       HIPCHK(launch_aos_to_soa(m->stage_aos, m->Ucur, N, m->st));
     } else {
       HIPCHK(launch_caluu1(m->G, m->W, m->met[0], m->p.gam, m->Ucur, m->st));  // main.f90:530
@@ -90,18 +112,25 @@ void raishin_model_set_state(raishin_model* m, const double* uri, const double* 
 
 void raishin_model_get_state(raishin_model* m, double* uri, double* uu, double* time) {
   const long long N = m->G.N;
+  std::vector<double> packed;
+  // a collapsed model's row goes to every phi row of the host array
+  auto fetch = [&](const double* planes, double* full) -> bool {
+    HIPCHK(launch_soa_to_aos(planes, m->stage_aos, N, m->st));
+    if (!m->G.jfix) {
+      HIPCHK(hipMemcpyAsync(full, m->stage_aos, 9 * N * 8, hipMemcpyDeviceToHost, m->st));
+      HIPCHK(hipStreamSynchronize(m->st));
+      return true;
+    }
+    packed.resize(9 * (size_t)N);
+    HIPCHK(hipMemcpyAsync(packed.data(), m->stage_aos, 9 * N * 8, hipMemcpyDeviceToHost, m->st));
+    HIPCHK(hipStreamSynchronize(m->st));
+    unpack_row(m->G, packed, full);
+    return true;
+  };
   auto go = [&]() -> bool {
     if (!m->stage_aos && !dalloc(&m->stage_aos, 9 * N)) return false;
-    if (uri) {
-      HIPCHK(launch_soa_to_aos(m->W, m->stage_aos, N, m->st));
-      HIPCHK(hipMemcpyAsync(uri, m->stage_aos, 9 * N * 8, hipMemcpyDeviceToHost, m->st));
-      HIPCHK(hipStreamSynchronize(m->st));
-    }
-    if (uu) {
-      HIPCHK(launch_soa_to_aos(m->Ucur, m->stage_aos, N, m->st));
-      HIPCHK(hipMemcpyAsync(uu, m->stage_aos, 9 * N * 8, hipMemcpyDeviceToHost, m->st));
-      HIPCHK(hipStreamSynchronize(m->st));
-    }
+    if (uri && !fetch(m->W, uri)) return false;
+    if (uu && !fetch(m->Ucur, uu)) return false;
     if (time) {
       double tdt[2];
       HIPCHK(hipMemcpyAsync(tdt, m->tdt, 16, hipMemcpyDeviceToHost, m->st));
@@ -238,6 +267,12 @@ void* raishin_model_stream(raishin_model* m) { return (void*)m->st; }
 
 long long raishin_model_device_bytes(raishin_model* m) { return m ? (long long)m->dev_bytes : 0; }
 
+void raishin_model_allow_collapse(raishin_model* m, int allow) {
+  if (m) m->allow_collapse = allow != 0;
+}
+
+int raishin_model_stored_rows(raishin_model* m) { return m ? m->G.njs : 0; }
+
 void raishin_model_debug_stage(raishin_model* m, int stage, double* ww, double* wpre) {
   // stage 1 also runs the CFL reduction and time advance of a step
   StageTap tap;
@@ -288,6 +323,7 @@ int raishin_model_set_block_sizes(raishin_model* m, const int* bs, int n) {
       return -1;
     }
   for (int q = 0; q < TS_N; q++) m->G.bs[q] = (unsigned short)bs[q];
+  m->geo_gen++;
   // a captured graph holds the old launch shapes
   if (m->gexec) { (void)hipGraphExecDestroy(m->gexec); m->gexec = nullptr; }
   if (m->graph) { (void)hipGraphDestroy(m->graph); m->graph = nullptr; }

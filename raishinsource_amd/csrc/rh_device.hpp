@@ -29,8 +29,13 @@ RH_DEV float fsign1(float x) { return copysignf(1.0f, x); }
 struct Geo {
   int ni, nj, nk;        // imax, jmax, kmax (ghosts included)
   int nm0, nm1;          // ghost widths (main.f90:179-211)
-  long long N;           // ni*nj*nk
+  long long N;           // cells stored per plane: ni*njs*nk
   long long P2;          // ni*nk: plane size of the axisymmetric (i,k) tables
+  // Stored phi rows.  njs = nj, jfix = 0: every row.  njs = 1, jfix > 0:
+  // the model is axisymmetric with one interior phi cell and periodic phi,
+  // so every phi row holds the same values at all times; row jfix alone is
+  // stored and computed, and every j maps to it.
+  int njs, jfix;
   // workgroup size per tuned launch slot (TuneSlot in rh_kernels.hpp);
   // 0 means 256.  Chosen by raishin_model_autotune or set by the caller.
   unsigned short bs[8];

@@ -13,8 +13,15 @@
 namespace rh {
 
 RH_DEV long long cidx(int i, int j, int k, const Geo& G) {
-  return (long long)(i - 1) + (long long)G.ni * ((long long)(j - 1) + (long long)G.nj * (long long)(k - 1));
+  const long long jr = G.jfix ? 0 : (long long)(j - 1);
+  return (long long)(i - 1) + (long long)G.ni * (jr + (long long)G.njs * (long long)(k - 1));
 }
+// j range of a box: the stored row of a collapsed model, else [lo, hi]
+RH_DEV int jlo(const Geo& G, int lo) { return G.jfix ? G.jfix : lo; }
+RH_DEV int jhi(const Geo& G, int hi) { return G.jfix ? G.jfix : hi; }
+// strides of a step in j and in k in the stored planes
+RH_DEV long long jstride(const Geo& G) { return G.jfix ? 0LL : (long long)G.ni; }
+RH_DEV long long kstride(const Geo& G) { return (long long)G.ni * G.njs; }
 RH_DEV long long q2(int i, int k, const Geo& G) { return (long long)(i - 1) + (long long)G.ni * (long long)(k - 1); }
 
 // 1-based (i,j,k) of linear thread t over a box [lo_a, hi_a] per axis; false if outside.
@@ -60,11 +67,12 @@ enum { C_FALLBACK = 0, C_FAIL, C_NEGRHO, C_NEGP, C_VSQGT1, C_DET0, C_WGCARRY, C_
 // writer at a time; k_sum_counters folds the rows when counters are read.
 // (Device-scope atomics on a few addresses from every wave serialise across
 // the XCDs: they cost more than the kernels that raise the events.)
-RH_DEV void count_wave(unsigned long long* cpart, long long blk, int slot, bool flag) {
+RH_DEV void count_wave(unsigned long long* cpart, long long blk, int slot, bool flag,
+                       unsigned long long weight = 1) {
   const unsigned long long b = __ballot(flag);
   if (b && (threadIdx.x % warpSize) == (unsigned)(__ffsll((long long)b) - 1)) {
     const long long w = (blk * blockDim.x + threadIdx.x) / warpSize;
-    cpart[w * C_NCOUNT + slot] += (unsigned long long)__popcll(b);
+    cpart[w * C_NCOUNT + slot] += (unsigned long long)__popcll(b) * weight;
   }
 }
 
@@ -245,12 +253,13 @@ RH_DEV void b_ppm_edges(const Geo& G, const EdgeArgs& A, long long blk) {
   R* __restrict__ E = (R*)A.E;
   int i, j, k;
   const long long t = blk * blockDim.x + threadIdx.x;
-  const int hi = G.ni - G.nm0 + (D == 1), hj = G.nj - G.nm0 + (D == 2), hk = G.nk - G.nm0 + (D == 3);
-  const bool in = (D == 3) ? box_index_ikj(t, G.nm0, hi, G.nm0, hj, G.nm0, hk, i, j, k)
-                           : box_index(t, G.nm0, hi, G.nm0, hj, G.nm0, hk, i, j, k);
+  const int hi = G.ni - G.nm0 + (D == 1), hj = jhi(G, G.nj - G.nm0 + (D == 2)), hk = G.nk - G.nm0 + (D == 3);
+  const int lj = jlo(G, G.nm0);
+  const bool in = (D == 3) ? box_index_ikj(t, G.nm0, hi, lj, hj, G.nm0, hk, i, j, k)
+                           : box_index(t, G.nm0, hi, lj, hj, G.nm0, hk, i, j, k);
   if (!in) return;
   const long long N = G.N;
-  const long long s = (D == 1) ? 1LL : (D == 2) ? (long long)G.ni : (long long)G.ni * G.nj;
+  const long long s = (D == 1) ? 1LL : (D == 2) ? jstride(G) : kstride(G);
   const long long c = cidx(i, j, k, G);
   const int p = (D == 1) ? i : (D == 2) ? j : k;
   const long long npc = ((D == 1) ? G.ni : (D == 2) ? G.nj : G.nk) + 2;
@@ -300,11 +309,12 @@ RH_DEV void b_sweep(const Geo& G, const SweepArgs& A, long long blk) {
   const R gam = R(A.gam);
   int i, j, k;
   const long long t = blk * blockDim.x + threadIdx.x;
-  const bool in = (D == 3) ? box_index_ikj(t, G.nm0, G.ni - G.nm0, G.nm0, G.nj - G.nm0, G.nm0, G.nk - G.nm0, i, j, k)
-                           : box_index(t, G.nm0, G.ni - G.nm0, G.nm0, G.nj - G.nm0, G.nm0, G.nk - G.nm0, i, j, k);
+  const int lj = jlo(G, G.nm0), hj = jhi(G, G.nj - G.nm0);
+  const bool in = (D == 3) ? box_index_ikj(t, G.nm0, G.ni - G.nm0, lj, hj, G.nm0, G.nk - G.nm0, i, j, k)
+                           : box_index(t, G.nm0, G.ni - G.nm0, lj, hj, G.nm0, G.nk - G.nm0, i, j, k);
   if (!in) return;
   const long long N = G.N;
-  const long long s = (D == 1) ? 1LL : (D == 2) ? (long long)G.ni : (long long)G.ni * G.nj;
+  const long long s = (D == 1) ? 1LL : (D == 2) ? jstride(G) : kstride(G);
   const long long c = cidx(i, j, k, G);
   R wl[9], wr[9];
   if (IREC == 5) {
@@ -390,9 +400,9 @@ RH_DEV void b_fluxct(const Geo& G, const CtArgs& A, long long blk) {
   double* __restrict__ F3 = A.F3;
   int i, j, k;
   const long long t = blk * blockDim.x + threadIdx.x;
-  if (!box_index(t, G.nm1, G.ni - G.nm1, G.nm1, G.nj - G.nm1, G.nm1, G.nk - G.nm1, i, j, k)) return;
+  if (!box_index(t, G.nm1, G.ni - G.nm1, jlo(G, G.nm1), jhi(G, G.nj - G.nm1), G.nm1, G.nk - G.nm1, i, j, k)) return;
   const long long N = G.N, c = cidx(i, j, k, G);
-  const long long si = 1, sj = G.ni, sk = (long long)G.ni * G.nj;
+  const long long si = 1, sj = jstride(G), sk = kstride(G);
   const double* fyx = CTsrc + 0 * N;  // ww(2,7)
   const double* fzx = CTsrc + 1 * N;  // ww(3,7)
   const double* fxy = CTsrc + 2 * N;  // ww(1,8)
@@ -657,9 +667,11 @@ template <int STAGE, bool SYM>
 RH_DEV void b_rk(const Geo& G, const StageArgs& A, long long blk) {
   int i, j, k;
   const long long t = blk * blockDim.x + threadIdx.x;
-  if (!box_index(t, G.nm1 + 1, G.ni - G.nm1, G.nm1 + 1, G.nj - G.nm1, G.nm1 + 1, G.nk - G.nm1, i, j, k)) return;
+  if (!box_index(t, G.nm1 + 1, G.ni - G.nm1, jlo(G, G.nm1 + 1), jhi(G, G.nj - G.nm1), G.nm1 + 1, G.nk - G.nm1, i, j,
+                 k))
+    return;
   const long long N = G.N, c = cidx(i, j, k, G);
-  const long long sj = G.ni, sk = (long long)G.ni * G.nj;
+  const long long sj = jstride(G), sk = kstride(G);
   const double gam = A.gam;
   // dt and dt/dx (main.f90:760-780), recomputed per thread from the CFL minimum
   double dt, a1, a2, a3;
@@ -729,7 +741,9 @@ RH_DEV void b_rk(const Geo& G, const StageArgs& A, long long blk) {
 RH_DEV void b_recov(const Geo& G, const StageArgs& A, long long blk) {
   int i, j, k;
   const long long t = blk * blockDim.x + threadIdx.x;
-  if (!box_index(t, G.nm1 + 1, G.ni - G.nm1, G.nm1 + 1, G.nj - G.nm1, G.nm1 + 1, G.nk - G.nm1, i, j, k)) return;
+  if (!box_index(t, G.nm1 + 1, G.ni - G.nm1, jlo(G, G.nm1 + 1), jhi(G, G.nj - G.nm1), G.nm1 + 1, G.nk - G.nm1, i, j,
+                 k))
+    return;
   const long long N = G.N, c = cidx(i, j, k, G);
   const Met M{A.metC, G.P2, q2(i, k, G)};
   double w[9], u[9];
@@ -846,8 +860,27 @@ RH_DEV double bnd_value(const BndCtx& B, int m, int i, int j, int k) {
 // then the two phi slabs (interior k), then the two r slabs (interior j, k).
 RH_DEV bool ghost_index(long long t, const Geo& G, int& i, int& j, int& k) {
   const long long ni = G.ni, nj = G.nj, nk = G.nk, g = G.nm1;
-  const long long nkS = 2 * g * ni * nj, njS = 2 * g * ni * (nk - 2 * g), niS = 2 * g * (nj - 2 * g) * (nk - 2 * g);
   if (t < 0) return false;
+  if (G.jfix) {
+    // collapsed: the stored row's theta and r ghosts (no phi ghosts are stored)
+    const long long nkS1 = 2 * g * ni, niS1 = 2 * g * (nk - 2 * g);
+    j = G.jfix;
+    if (t < nkS1) {
+      i = (int)(t % ni) + 1;
+      const long long kk = t / ni;
+      k = (int)(kk < g ? kk : nk - 2 * g + kk) + 1;
+      return true;
+    }
+    t -= nkS1;
+    if (t < niS1) {
+      const long long ii = t % (2 * g);
+      i = (int)(ii < g ? ii : ni - 2 * g + ii) + 1;
+      k = (int)(g + t / (2 * g)) + 1;
+      return true;
+    }
+    return false;
+  }
+  const long long nkS = 2 * g * ni * nj, njS = 2 * g * ni * (nk - 2 * g), niS = 2 * g * (nj - 2 * g) * (nk - 2 * g);
   if (t < nkS) {
     i = (int)(t % ni) + 1;
     const long long r = t / ni;
@@ -894,7 +927,7 @@ RH_DEV void b_bnd(const Geo& G, const PostArgs& A, long long blk) {
 RH_DEV void b_post(const Geo& G, const PostArgs& A, long long blk) {
   const long long t = blk * blockDim.x + threadIdx.x;
   int i, j, k;
-  if (!box_index(t, 1, G.ni, 1, G.nj, 1, G.nk, i, j, k)) return;
+  if (!box_index(t, 1, G.ni, jlo(G, 1), jhi(G, G.nj), 1, G.nk, i, j, k)) return;
   const long long N = G.N, c = cidx(i, j, k, G);
   const double gam = A.gam;
   const Met M{A.metC, G.P2, q2(i, k, G)};
@@ -945,7 +978,9 @@ RH_DEV void b_post(const Geo& G, const PostArgs& A, long long blk) {
     }
     w[5] = 0.0;
   }
-  count_wave(A.cpart, blk, C_ATM, atm);
+  // a collapsed model's cell stands for its nj phi copies, each counted by the reference
+  const unsigned long long rows = G.jfix ? (unsigned long long)G.nj : 1ULL;
+  count_wave(A.cpart, blk, C_ATM, atm, rows);
   // pminmax2b: correction.f90:808-961
   bool scaled = false, hit = false;
   {
@@ -993,8 +1028,8 @@ RH_DEV void b_post(const Geo& G, const PostArgs& A, long long blk) {
     if (bbro > 50.0) { w[0] = s.bbsq / 50.0; hit = true; }
     if (bbpr > 2.0e3) { w[4] = s.bbsq / 2.0e3; hit = true; }
   }
-  count_wave(A.cpart, blk, C_PMMSCALE, scaled);
-  count_wave(A.cpart, blk, C_PMMFLOOR, hit);
+  count_wave(A.cpart, blk, C_PMMSCALE, scaled, rows);
+  count_wave(A.cpart, blk, C_PMMFLOOR, hit, rows);
 #pragma unroll
   for (int n = 0; n < 9; n++) A.Wout[(long long)n * N + c] = w[n];
   // caluu1 (caluu.f90:2-126): conserved variables of the final primitives
@@ -1043,7 +1078,7 @@ RH_DEV void b_cfl(const Geo& G, const CflArgs& A, long long blk) {
   const long long t = blk * blockDim.x + threadIdx.x;
   int i, j, k;
   double e[4] = {__builtin_inf(), __builtin_inf(), __builtin_inf(), 0.0};  // dtx1a, dtx2a, dtx3a
-  bool valid = box_index(t, 2, G.ni - 1, 2, G.nj - 1, 2, G.nk - 1, i, j, k);
+  bool valid = box_index(t, 2, G.ni - 1, jlo(G, 2), jhi(G, G.nj - 1), 2, G.nk - 1, i, j, k);
   if (valid) {
     const long long N = G.N, c = cidx(i, j, k, G);
     const Met M{A.metC, G.P2, q2(i, k, G)};
@@ -1191,14 +1226,15 @@ RH_DEV void b_recov_carry(const Geo& G, const FixArgs& A) {
   for (unsigned long long e = threadIdx.x; e < n; e += blockDim.x) pend[e] = 1;
   __syncthreads();
   const long long N = G.N;
-  const long long sj = G.ni, sk = (long long)G.ni * G.nj;
+  const long long sk = kstride(G);
   for (int pass = 0; pass < (1 << 22); pass++) {
     if (threadIdx.x == 0) { s_progress = 0; s_left = 0; }
     __syncthreads();
     for (unsigned long long e = threadIdx.x; e < n; e += blockDim.x) {
       if (pend[e] != 1) continue;
       const long long c = (long long)list[e];
-      const int i = (int)(c % G.ni) + 1, j = (int)((c / sj) % G.nj) + 1, k = (int)(c / sk) + 1;
+      const int i = (int)(c % G.ni) + 1, k = (int)(c / sk) + 1;
+      const int j = G.jfix ? G.jfix : (int)((c / G.ni) % G.nj) + 1;
       int pi = i - 1, pj = j, pk = k;
       if (pi < G.nm1 + 1) { pi = G.ni - G.nm1; pj = j - 1; if (pj < G.nm1 + 1) { pj = G.nj - G.nm1; pk = k - 1; } }
       double wg_prev = 0.0;
@@ -1243,7 +1279,7 @@ __global__ void __launch_bounds__(256) k_caluu1(Geo G, const double* __restrict_
                                                 double gam, double* __restrict__ U) {
   const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   int i, j, k;
-  if (!box_index(t, 1, G.ni, 1, G.nj, 1, G.nk, i, j, k)) return;
+  if (!box_index(t, 1, G.ni, jlo(G, 1), jhi(G, G.nj), 1, G.nk, i, j, k)) return;
   const long long N = G.N, c = cidx(i, j, k, G);
   const Met M{metC, G.P2, q2(i, k, G)};
   double w[9], u[9];
@@ -1408,6 +1444,8 @@ __global__ void __launch_bounds__(64) kb_advance_time(const BatchItem* __restric
 // ------------------------------------------------------------------------
 static inline unsigned nblk(long long n, int b) { return (unsigned)((n + b - 1) / b); }
 static inline long long box(int lo, int hi) { return hi >= lo ? (long long)(hi - lo + 1) : 0; }
+// rows of a box along j: one for a collapsed model (its stored row lies in every box)
+static inline long long jbox(const Geo& G, int lo, int hi) { return G.jfix ? 1 : box(lo, hi); }
 
 static thread_local std::vector<LaunchRec>* g_rec = nullptr;
 
@@ -1431,7 +1469,7 @@ bool one_group_per_model(int kind) { return kind == LK_CARRY || kind == LK_CFL_F
 
 hipError_t launch_ppm_edges(const Geo& G, int dir, const double* W, const double* dxs, const double* xs,
                             const double* pc, double gam, void* E, bool mixed, hipStream_t st) {
-  const long long n = box(G.nm0, G.ni - G.nm0 + (dir == 1)) * box(G.nm0, G.nj - G.nm0 + (dir == 2)) *
+  const long long n = box(G.nm0, G.ni - G.nm0 + (dir == 1)) * jbox(G, G.nm0, G.nj - G.nm0 + (dir == 2)) *
                       box(G.nm0, G.nk - G.nm0 + (dir == 3));
   EdgeArgs A{W, dxs, xs, pc, gam, E};
   const int bt = slot_threads(G, TS_EDGES);
@@ -1447,7 +1485,7 @@ hipError_t launch_ppm_edges(const Geo& G, int dir, const double* W, const double
 
 hipError_t launch_sweep(const Geo& G, int dir, int irec, const double* W, const double* metF, const void* E,
                         double gam, double* F, double* CTsrc, bool mixed, hipStream_t st) {
-  const long long n = box(G.nm0, G.ni - G.nm0) * box(G.nm0, G.nj - G.nm0) * box(G.nm0, G.nk - G.nm0);
+  const long long n = box(G.nm0, G.ni - G.nm0) * jbox(G, G.nm0, G.nj - G.nm0) * box(G.nm0, G.nk - G.nm0);
   SweepArgs A{W, metF, E, gam, F, CTsrc};
   const int bt = slot_threads(G, TS_SWEEP1 + dir - 1);
   const unsigned nb = nblk(n, bt);
@@ -1470,7 +1508,7 @@ hipError_t launch_sweep(const Geo& G, int dir, int irec, const double* W, const 
 }
 
 hipError_t launch_fluxct(const Geo& G, const double* CTsrc, double* F1, double* F2, double* F3, hipStream_t st) {
-  const long long n = box(G.nm1, G.ni - G.nm1) * box(G.nm1, G.nj - G.nm1) * box(G.nm1, G.nk - G.nm1);
+  const long long n = box(G.nm1, G.ni - G.nm1) * jbox(G, G.nm1, G.nj - G.nm1) * box(G.nm1, G.nk - G.nm1);
   CtArgs A{CTsrc, F1, F2, F3};
   const int bt = slot_threads(G, TS_FLUXCT);
   if (g_rec) { AnyArgs a{}; a.c = A; record(LK_FLUXCT, 0, bt, nblk(n, bt), G, a); return hipSuccess; }
@@ -1480,7 +1518,7 @@ hipError_t launch_fluxct(const Geo& G, const double* CTsrc, double* F1, double* 
 }
 
 hipError_t launch_update(const Geo& G, int stage, const StageArgs& A, hipStream_t st) {
-  const long long n = box(G.nm1 + 1, G.ni - G.nm1) * box(G.nm1 + 1, G.nj - G.nm1) * box(G.nm1 + 1, G.nk - G.nm1);
+  const long long n = box(G.nm1 + 1, G.ni - G.nm1) * jbox(G, G.nm1 + 1, G.nj - G.nm1) * box(G.nm1 + 1, G.nk - G.nm1);
   const int bt = slot_threads(G, TS_RK);
   const unsigned nb = nblk(n, bt);
   if (g_rec) { AnyArgs a{}; a.st = A; record(LK_RK, stage + 8 * (int)A.chr_sym, bt, nb, G, a); return hipSuccess; }
@@ -1497,7 +1535,7 @@ hipError_t launch_update(const Geo& G, int stage, const StageArgs& A, hipStream_
 }
 
 hipError_t launch_recov(const Geo& G, const StageArgs& A, hipStream_t st) {
-  const long long n = box(G.nm1 + 1, G.ni - G.nm1) * box(G.nm1 + 1, G.nj - G.nm1) * box(G.nm1 + 1, G.nk - G.nm1);
+  const long long n = box(G.nm1 + 1, G.ni - G.nm1) * jbox(G, G.nm1 + 1, G.nj - G.nm1) * box(G.nm1 + 1, G.nk - G.nm1);
   const int bt = slot_threads(G, TS_RECOV);
   if (g_rec) { AnyArgs a{}; a.st = A; record(LK_RECOV, 0, bt, nblk(n, bt), G, a); return hipSuccess; }
   if (n == 0) return hipSuccess;
@@ -1513,7 +1551,7 @@ hipError_t launch_recov_carry(const Geo& G, const FixArgs& A, hipStream_t st) {
 
 hipError_t launch_post(const Geo& G, const PostArgs& A, hipStream_t st) {
   const long long g = G.nm1;
-  const long long ng = G.N - (long long)(G.ni - 2 * g) * (G.nj - 2 * g) * (G.nk - 2 * g);
+  const long long ng = G.N - (long long)(G.ni - 2 * g) * (G.jfix ? 1 : G.nj - 2 * g) * (G.nk - 2 * g);
   const int bt = slot_threads(G, TS_POST);
   if (g_rec) {
     AnyArgs a{};
@@ -1533,7 +1571,7 @@ hipError_t launch_sum_counters(unsigned long long* cpart, long long rows, unsign
 }
 
 long long cfl_blocks(const Geo& G) {
-  const long long n = box(2, G.ni - 1) * box(2, G.nj - 1) * box(2, G.nk - 1);
+  const long long n = box(2, G.ni - 1) * jbox(G, 2, G.nj - 1) * box(2, G.nk - 1);
   return n > 0 ? (long long)nblk(n, 256) : 1;
 }
 

@@ -225,6 +225,8 @@ static bool build_model_(raishin_model* m, const raishin_pram* p, const double* 
   G.nm0 = ghost_nm0(p->irec); G.nm1 = G.nm0 + 1;
   G.N = (long long)G.ni * G.nj * G.nk;
   G.P2 = (long long)G.ni * G.nk;
+  G.njs = G.nj;
+  G.jfix = 0;
   const long long N = G.N;
   HIPCHK(hipStreamCreateWithFlags(&m->st, hipStreamNonBlocking));
   m->carry_cap = (long long)(G.ni - 2 * G.nm1) * (G.nj - 2 * G.nm1) * (G.nk - 2 * G.nm1);
@@ -326,14 +328,72 @@ static bool build_model_(raishin_model* m, const raishin_pram* p, const double* 
 
 bool copy_planes_out(raishin_model* m, const double* planes, int nplanes, int stride_out, int offset_out,
                      double* host) {
-  // planes[n*N + c] -> host[offset_out + stride_out*(n + nplanes*c)] via a staging buffer
-  const long long N = m->G.N;
+  // planes[n*N + c] -> host[offset_out + stride_out*(n + nplanes*c)] via a
+  // staging buffer; a collapsed model's row goes to every phi row
+  const Geo& G = m->G;
+  const long long N = G.N;
   std::vector<double> tmp((size_t)nplanes * N);
   HIPCHK(hipMemcpyAsync(tmp.data(), planes, (size_t)nplanes * N * 8, hipMemcpyDeviceToHost, m->st));
   HIPCHK(hipStreamSynchronize(m->st));
-  for (long long c = 0; c < N; c++)
-    for (int n = 0; n < nplanes; n++) host[offset_out + (size_t)stride_out * (n + (size_t)nplanes * c)] = tmp[(size_t)n * N + c];
+  const long long ni = G.ni, nj = G.nj, nk = G.nk;
+  for (long long k = 0; k < nk; k++)
+    for (long long j = 0; j < nj; j++)
+      for (long long i = 0; i < ni; i++) {
+        const long long cs = i + ni * ((G.jfix ? 0 : j) + (long long)G.njs * k);
+        const long long cf = i + ni * (j + nj * k);
+        for (int n = 0; n < nplanes; n++)
+          host[offset_out + (size_t)stride_out * (n + (size_t)nplanes * cf)] = tmp[(size_t)n * N + cs];
+      }
   return true;
+}
+
+bool collapsible(const raishin_model* m) {
+  const raishin_pram& p = m->p;
+  const Geo& G = m->G;
+  if (!m->allow_collapse || G.nj != 2 * G.nm1 + 1 || m->need_wo) return false;
+  for (int q = 0; q < 9; q++) {
+    // periodic phi; r and theta boundaries that are not periodic (two
+    // periodic axes leave corner ghosts that no pass writes)
+    if (p.iboux2in[q] != 1 || p.iboux2ot[q] != 1) return false;
+    if (p.iboux1in[q] == 1 || p.iboux1ot[q] == 1 || p.iboux3in[q] == 1 || p.iboux3ot[q] == 1) return false;
+  }
+  return true;
+}
+
+bool rows_equal(const Geo& G, const double* a, int j0) {
+  const size_t row = 9 * (size_t)G.ni;
+  for (size_t k = 0; k < (size_t)G.nk; k++) {
+    const double* r0 = a + row * ((size_t)(j0 - 1) + (size_t)G.nj * k);
+    for (size_t j = 0; j < (size_t)G.nj; j++)
+      if (std::memcmp(a + row * (j + (size_t)G.nj * k), r0, row * sizeof(double))) return false;
+  }
+  return true;
+}
+
+void set_rows(raishin_model* m, bool collapse) {
+  Geo& G = m->G;
+  const int jfix = collapse ? G.nm1 + 1 : 0;
+  if (G.jfix == jfix) return;
+  G.jfix = jfix;
+  G.njs = collapse ? 1 : G.nj;
+  G.N = (long long)G.ni * G.njs * G.nk;
+  m->geo_gen++;
+  if (m->gexec) { (void)hipGraphExecDestroy(m->gexec); m->gexec = nullptr; }
+  if (m->graph) { (void)hipGraphDestroy(m->graph); m->graph = nullptr; }
+}
+
+void pack_row(const Geo& G, const double* full, std::vector<double>& out) {
+  const size_t row = 9 * (size_t)G.ni;
+  out.resize(row * G.nk);
+  for (size_t k = 0; k < (size_t)G.nk; k++)
+    std::memcpy(&out[row * k], full + row * ((size_t)(G.jfix - 1) + (size_t)G.nj * k), row * sizeof(double));
+}
+
+void unpack_row(const Geo& G, const std::vector<double>& in, double* full) {
+  const size_t row = 9 * (size_t)G.ni;
+  for (size_t k = 0; k < (size_t)G.nk; k++)
+    for (size_t j = 0; j < (size_t)G.nj; j++)
+      std::memcpy(full + row * (j + (size_t)G.nj * k), &in[row * k], row * sizeof(double));
 }
 
 bool stage_sweeps(raishin_model* m, const double* W, double* F) {

@@ -63,6 +63,10 @@ struct raishin_model {
   const double* graph_U = nullptr;
   bool graph_mixed = false;
   size_t dev_bytes = 0;  // device memory allocated by build_model
+  // one stored phi row when the state allows it (raishin_model_set_state);
+  // geo_gen changes whenever G does, so batch plans can tell
+  bool allow_collapse = true;
+  unsigned geo_gen = 0;
 };
 
 namespace rhm {
@@ -83,6 +87,14 @@ bool dalloc(T** p, size_t n) {
 }
 void dfree_all(raishin_model* m);
 bool upload(double* dst, const double* src, size_t n, hipStream_t st);
+// One stored phi row (Geo::jfix): allowed for the model's configuration,
+// states whose rows are bit-identical, switching the geometry, and moving a
+// row between a full host array and a packed (9, ni, nk) one.
+bool collapsible(const raishin_model* m);
+bool rows_equal(const rh::Geo& G, const double* a, int j0);
+void set_rows(raishin_model* m, bool collapse);
+void pack_row(const rh::Geo& G, const double* full, std::vector<double>& out);
+void unpack_row(const rh::Geo& G, const std::vector<double>& in, double* full);
 void resolve_timing(raishin_model* m);
 bool build_model(raishin_model* m, const raishin_pram* p, const double* x1, const double* x2, const double* x3,
                  const double* dx1b, const double* dx2b, const double* dx3b, const double* const gcv[4],

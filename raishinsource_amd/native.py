@@ -58,7 +58,7 @@ EXPORTS = (
     "raishin_models_step_graph", "raishin_model_device_bytes", "raishin_batch_create", "raishin_batch_destroy",
     "raishin_batch_set", "raishin_batch_step", "raishin_batch_step_graph", "raishin_batch_times",
     "raishin_batch_sync", "raishin_mdshoctub", "raishin_model_set_block_sizes", "raishin_model_get_block_sizes",
-    "raishin_model_autotune",
+    "raishin_model_autotune", "raishin_model_allow_collapse", "raishin_model_stored_rows",
 )
 
 TUNE_SLOTS = ("ppm_edges", "sweep_i", "sweep_j", "sweep_k", "fluxct", "update", "recov", "post")
@@ -123,6 +123,9 @@ def lib():
         L.raishin_model_autotune.argtypes = [vp, vp, vp, C.c_double, C.c_int]
         for f in ("raishin_model_set_block_sizes", "raishin_model_get_block_sizes", "raishin_model_autotune"):
             getattr(L, f).restype = C.c_int
+        L.raishin_model_allow_collapse.argtypes = [vp, C.c_int]
+        L.raishin_model_stored_rows.argtypes = [vp]
+        L.raishin_model_stored_rows.restype = C.c_int
         L.raishin_model_device_bytes.argtypes = [vp]
         L.raishin_model_device_bytes.restype = C.c_longlong
         L.raishin_batch_create.restype = vp
@@ -426,6 +429,14 @@ class Model:
             check()
         check()
         return self.block_sizes()
+
+    def allow_collapse(self, allow=True):
+        """Whether set_state may store one phi row for a state whose rows are
+        all the same (axisymmetric models with one interior phi cell)."""
+        lib().raishin_model_allow_collapse(self.h, 1 if allow else 0)
+
+    def stored_rows(self):
+        return int(lib().raishin_model_stored_rows(self.h))
 
     def device_bytes(self):
         return int(lib().raishin_model_device_bytes(self.h))
