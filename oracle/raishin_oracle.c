@@ -580,6 +580,44 @@ static void five_point(const double *uri, double *uriir, double *urijr, double *
 }
 
 /* ------------------------------------------------------------------ */
+/* mclim: rec.f90:208-367 (irec=1), MC-limited linear reconstruction.  */
+/* Needs the cell-face positions x1a..x3a (orc_set_faces).             */
+/* ------------------------------------------------------------------ */
+static const double *FX[3] = {0, 0, 0};
+
+void orc_set_faces(const double *x1a, const double *x2a, const double *x3a) {
+  FX[0] = x1a; FX[1] = x2a; FX[2] = x3a;
+}
+
+static void mclim(const double *uri, const double *x1, const double *x2, const double *x3,
+                  double *uriir, double *urijr, double *urikr, double *uriil, double *urijl, double *urikl) {
+  const double small = 1.0e-12;
+  const double *xc_[3] = {x1, x2, x3};
+  double *lft[3] = {uriil, urijl, urikl}, *rgt[3] = {uriir, urijr, urikr};
+  for (int k = 2; k <= NK - 1; k++)
+    for (int j = 2; j <= NJ - 1; j++)
+      for (int i = 2; i <= NI - 1; i++)
+        for (int n = 1; n <= 9; n++) {
+          const int id[3] = {i, j, k};
+          for (int d = 0; d < 3; d++) {
+            const int p = id[d];
+            const double *x = xc_[d], *xa = FX[d];
+            const double xp = x[p], xc = x[p - 1], xm = x[p - 2], xap = xa[p - 1], xam = xa[p - 2];
+            const int dp[3] = {d == 0, d == 1, d == 2};
+            const double u = V(uri, n, i, j, k);
+            const double up = V(uri, n, i + dp[0], j + dp[1], k + dp[2]);
+            const double um = V(uri, n, i - dp[0], j - dp[1], k - dp[2]);
+            const double sp = (up - u) * (1.0 / (xp - xc + small));
+            const double sm = (u - um) * (1.0 / (xc - xm + small));
+            const double tmp = (sp < 0.0) ? -1.0 : 1.0;
+            const double dk = tmp * FMAX(0.0, FMIN(FMIN(2.0 * fabs(sp), tmp * 2.0 * sm), tmp * 0.5 * (sp + sm)));
+            V(lft[d], n, i, j, k) = u + dk * (xap - xc);
+            V(rgt[d], n, i - dp[0], j - dp[1], k - dp[2]) = u - dk * (xc - xam);
+          }
+        }
+}
+
+/* ------------------------------------------------------------------ */
 /* rec2: rec.f90:2-142                                                 */
 /* ------------------------------------------------------------------ */
 void orc_rec2(const double *uri, const double *x1, const double *x2, const double *x3,
@@ -605,6 +643,8 @@ void orc_rec2(const double *uri, const double *x1, const double *x2, const doubl
     free(dma); free(tmp);
   } else if (P.irec == 6 || P.irec == 7) {
     five_point(uri, uriir, urijr, urikr, uriil, urijl, urikl, P.irec);
+  } else if (P.irec == 1 && FX[0]) {
+    mclim(uri, x1, x2, x3, uriir, urijr, urikr, uriil, urijl, urikl);
   } else {
     abort();
   }
@@ -1380,7 +1420,7 @@ static void stage(orc_state *s, int st, double dt, const double *a1, const doubl
 double orc_step(orc_state *s, double *time) {
   const size_t N = (size_t)NI * NJ * NK;
   work(N);
-  int nm0 = (P.irec == 5) ? 4 : 3;
+  int nm0 = (P.irec == 5) ? 4 : (P.irec == 1) ? 2 : 3; /* main.f90:179-211 */
   int nm1 = nm0 + 1;
   memcpy(W_uo, s->uu, sizeof(double) * 9 * N);
   memcpy(W_urio, s->uri, sizeof(double) * 9 * N);

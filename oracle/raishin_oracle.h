@@ -50,6 +50,8 @@ void orc_reset_counters(void);
 /* hot-path routines (reference argument meaning, 1-based ghost widths) */
 void orc_caluu1(const double *uri, double *uu, const double *gcov, const double *gcon,
                 const double *detg);
+/* cell-face positions x1a, x2a, x3a (grid.f90) for the MC reconstruction (irec=1) */
+void orc_set_faces(const double *x1a, const double *x2a, const double *x3a);
 void orc_rec2(const double *uri, const double *x1, const double *x2, const double *x3,
               const double *dx1b, const double *dx2b, const double *dx3b,
               double *uriir, double *urijr, double *urikr,

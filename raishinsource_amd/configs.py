@@ -119,9 +119,13 @@ def tube(name, nzones, nm1=5):
                 iboux3in=(1,) * 9, iboux3ot=(1,) * 9)
 
 
+# MC-limited linear reconstruction (irec=1, rec.f90:208): nm0=2, nm1=3
+U_MC = Pram("umc", 16, 10, 18, irec=1, irkt=2, cfl=0.4)
+T2D_MC = Pram("t2dmc", 64, 7, 64, irec=1, irkt=2, cfl=0.4)
+
 # the serving mix's tube (1024 zones) and a small one for the oracle
 TUBE = tube("tube", 1024)
 U_TUBE = tube("utube", 64)
 
 ALL = {p.name: p for p in (C1, C2, C2_PPM, C2_WENO, U_PPM, U_MP5, U_WENO, T2D, T3D, G_PPM, G_MP5, G_WENO, TUBE,
-                           U_TUBE)}
+                           U_TUBE, U_MC, T2D_MC)}

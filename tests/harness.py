@@ -132,6 +132,11 @@ class Oracle:
         self._pram = s
         self.lib.orc_set_pram(C.byref(s))
 
+    def set_faces(self, g):
+        """Cell-face positions for the MC reconstruction (irec=1)."""
+        self._faces = [np.ascontiguousarray(g[k]) for k in ("x1a", "x2a", "x3a")]
+        self.lib.orc_set_faces(*[ptr(a) for a in self._faces])
+
     def counters(self):
         c = self.Counters()
         self.lib.orc_get_counters(C.byref(c))
@@ -156,6 +161,8 @@ class Oracle:
         getattr(self.lib, "orc_" + fn)(*cargs)
 
     def step(self, uu, uri, g, m, time=0.0):
+        if "x1a" in g:
+            self.set_faces(g)
         st = self.State()
         keep = dict(uu=uu, uri=uri, **{k: m[k] for k in (
             "gcov", "gcon", "gcovi", "gconi", "gcovj", "gconj", "gcovk", "gconk", "detg", "detgi",
