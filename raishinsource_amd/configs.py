@@ -21,7 +21,7 @@ class Pram:
     imax: int
     jmax: int
     kmax: int
-    irec: int = 5          # 5 PPM, 6 MP5, 7 WENO5
+    irec: int = 5          # 1 MC, 5 PPM, 6 MP5, 7 WENO5
     irkt: int = 2
     ict: int = 1
     ihll: int = 1

@@ -288,6 +288,43 @@ RH_DEV void b_ppm_edges(const Geo& G, const EdgeArgs& A, long long blk) {
   }
 }
 
+// MC-limited linear reconstruction of every cell along D (mclim,
+// rec.f90:208-367, irec=1): slopes to both neighbours over the centre
+// distances (+1e-12), the MC limiter, edges at the cell's faces; stored like
+// the PPM edges.  A.pc holds the face positions x*a of the direction.
+template <int D, class R>
+RH_DEV void b_mc_edges(const Geo& G, const EdgeArgs& A, long long blk) {
+  const double* __restrict__ W = A.W;
+  const double* __restrict__ xs = A.xs;
+  const double* __restrict__ xa = A.pc;
+  R* __restrict__ E = (R*)A.E;
+  int i, j, k;
+  const long long t = blk * blockDim.x + threadIdx.x;
+  const int hi = G.ni - G.nm0 + (D == 1), hj = jhi(G, G.nj - G.nm0 + (D == 2)), hk = G.nk - G.nm0 + (D == 3);
+  const int lj = jlo(G, G.nm0);
+  const bool in = (D == 3) ? box_index_ikj(t, G.nm0, hi, lj, hj, G.nm0, hk, i, j, k)
+                           : box_index(t, G.nm0, hi, lj, hj, G.nm0, hk, i, j, k);
+  if (!in) return;
+  const long long N = G.N;
+  const long long s = (D == 1) ? 1LL : (D == 2) ? jstride(G) : kstride(G);
+  const long long c = cidx(i, j, k, G);
+  const int p = (D == 1) ? i : (D == 2) ? j : k;
+  const R small = R(1.0e-12);
+  const R xp = R(xs[p]), xc = R(xs[p - 1]), xm = R(xs[p - 2]), xap = R(xa[p - 1]), xam = R(xa[p - 2]);
+  const R ip = R(1) / (xp - xc + small), im = R(1) / (xc - xm + small);
+#pragma unroll 3
+  for (int n = 0; n < 9; n++) {
+    const double* Pn = W + (long long)n * N + c;
+    const R u = R(Pn[0]), up = R(Pn[s]), um = R(Pn[-s]);
+    const R sp = (up - u) * ip;
+    const R sm = (u - um) * im;
+    const R tmp = (sp < R(0)) ? R(-1) : R(1);
+    const R dk = tmp * fmaxf_(R(0), fminf_(fminf_(R(2) * absr(sp), tmp * R(2) * sm), tmp * R(0.5) * (sp + sm)));
+    E[(long long)n * N + c - s] = u - dk * (xc - xam);   // left edge, at the interface c-1/2
+    E[(long long)(9 + n) * N + c] = u + dk * (xap - xc); // right edge
+  }
+}
+
 // ------------------------------------------------------------------------
 // Direction sweep: reconstruction + face fluxes + wave speeds + HLLE.
 // rec2 (rec.f90:2-142) -> caluu3/calflx3* -> calcha (calcha.f90:2-126) -> hlle
@@ -317,7 +354,7 @@ RH_DEV void b_sweep(const Geo& G, const SweepArgs& A, long long blk) {
   const long long s = (D == 1) ? 1LL : (D == 2) ? jstride(G) : kstride(G);
   const long long c = cidx(i, j, k, G);
   R wl[9], wr[9];
-  if (IREC == 5) {
+  if (IREC == 5 || IREC == 1) {
     // right edge of cell c (uri*l(c)) and left edge of cell c+1 (uri*r(c)),
     // both stored at interface c+1/2's index
 #pragma unroll
@@ -1351,6 +1388,8 @@ RH_DEV void b_copy(const CopyArgs& A, long long blk) {
 
 template <int D, class R>
 __global__ void __launch_bounds__(256) k_ppm_edges(Geo G, EdgeArgs A) { b_ppm_edges<D, R>(G, A, xcd_block()); }
+template <int D, class R>
+__global__ void __launch_bounds__(256) k_mc_edges(Geo G, EdgeArgs A) { b_mc_edges<D, R>(G, A, xcd_block()); }
 template <int D, int IREC, class R>
 __global__ void __launch_bounds__(256) k_sweep(Geo G, SweepArgs A) { b_sweep<D, IREC, R>(G, A, xcd_block()); }
 __global__ void __launch_bounds__(256) k_fluxct(Geo G, CtArgs A) { b_fluxct(G, A, xcd_block()); }
@@ -1374,6 +1413,12 @@ __global__ void __launch_bounds__(256) kb_ppm_edges(const BatchItem* __restrict_
   RH_BATCH_PROLOGUE
   const EdgeArgs A = it[q].A.e;
   b_ppm_edges<D, R>(G, A, blk);
+}
+template <int D, class R>
+__global__ void __launch_bounds__(256) kb_mc_edges(const BatchItem* __restrict__ it, int n) {
+  RH_BATCH_PROLOGUE
+  const EdgeArgs A = it[q].A.e;
+  b_mc_edges<D, R>(G, A, blk);
 }
 template <int D, int IREC, class R>
 __global__ void __launch_bounds__(256) kb_sweep(const BatchItem* __restrict__ it, int n) {
@@ -1467,18 +1512,26 @@ int block_threads(int kind) {
 
 bool one_group_per_model(int kind) { return kind == LK_CARRY || kind == LK_CFL_FINISH || kind == LK_ADVANCE; }
 
-hipError_t launch_ppm_edges(const Geo& G, int dir, const double* W, const double* dxs, const double* xs,
-                            const double* pc, double gam, void* E, bool mixed, hipStream_t st) {
+hipError_t launch_edges(const Geo& G, int dir, int irec, const double* W, const double* dxs, const double* xs,
+                        const double* tab, double gam, void* E, bool mixed, hipStream_t st) {
   const long long n = box(G.nm0, G.ni - G.nm0 + (dir == 1)) * jbox(G, G.nm0, G.nj - G.nm0 + (dir == 2)) *
                       box(G.nm0, G.nk - G.nm0 + (dir == 3));
-  EdgeArgs A{W, dxs, xs, pc, gam, E};
+  EdgeArgs A{W, dxs, xs, tab, gam, E};
   const int bt = slot_threads(G, TS_EDGES);
   const unsigned nb = nblk(n, bt);
-  if (g_rec) { AnyArgs a{}; a.e = A; record(LK_EDGES, dir + 4 * (int)mixed, bt, nb, G, a); return hipSuccess; }
+  const bool mc = irec == 1;
+  if (g_rec) {
+    AnyArgs a{};
+    a.e = A;
+    record(LK_EDGES, dir + 4 * (int)mixed + 16 * (int)mc, bt, nb, G, a);
+    return hipSuccess;
+  }
   if (n == 0) return hipSuccess;
-#define L(Dd, T) hipLaunchKernelGGL((k_ppm_edges<Dd, T>), dim3(nb), dim3(bt), 0, st, G, A)
-  if (mixed) { if (dir == 1) L(1, float); else if (dir == 2) L(2, float); else L(3, float); }
-  else { if (dir == 1) L(1, double); else if (dir == 2) L(2, double); else L(3, double); }
+#define L(K, Dd, T) hipLaunchKernelGGL((K<Dd, T>), dim3(nb), dim3(bt), 0, st, G, A)
+#define LD(K, T) do { if (dir == 1) L(K, 1, T); else if (dir == 2) L(K, 2, T); else L(K, 3, T); } while (0)
+  if (mc) { if (mixed) LD(k_mc_edges, float); else LD(k_mc_edges, double); }
+  else { if (mixed) LD(k_ppm_edges, float); else LD(k_ppm_edges, double); }
+#undef LD
 #undef L
   return hipGetLastError();
 }
@@ -1500,6 +1553,7 @@ hipError_t launch_sweep(const Geo& G, int dir, int irec, const double* W, const 
 #define LT(Dd, Rc) \
   do { if (mixed) L(Dd, Rc, float); else L(Dd, Rc, double); } while (0)
   if (irec == 5) { if (dir == 1) LT(1, 5); else if (dir == 2) LT(2, 5); else LT(3, 5); }
+  else if (irec == 1) { if (dir == 1) LT(1, 1); else if (dir == 2) LT(2, 1); else LT(3, 1); }
   else if (irec == 6) { if (dir == 1) LT(1, 6); else if (dir == 2) LT(2, 6); else LT(3, 6); }
   else { if (dir == 1) LT(1, 7); else if (dir == 2) LT(2, 7); else LT(3, 7); }
 #undef LT
@@ -1618,10 +1672,12 @@ hipError_t launch_batched(int kind, int variant, int threads, const BatchItem* i
   switch (kind) {
     case LK_EDGES: {
       const int d = variant & 3;
-      const bool mx = (variant & 4) != 0;
-#define L(Dd, T) hipLaunchKernelGGL((kb_ppm_edges<Dd, T>), g, b, 0, st, it, n)
-      if (mx) { if (d == 1) L(1, float); else if (d == 2) L(2, float); else L(3, float); }
-      else { if (d == 1) L(1, double); else if (d == 2) L(2, double); else L(3, double); }
+      const bool mx = (variant & 4) != 0, mc = (variant & 16) != 0;
+#define L(K, Dd, T) hipLaunchKernelGGL((K<Dd, T>), g, b, 0, st, it, n)
+#define LD(K, T) do { if (d == 1) L(K, 1, T); else if (d == 2) L(K, 2, T); else L(K, 3, T); } while (0)
+      if (mc) { if (mx) LD(kb_mc_edges, float); else LD(kb_mc_edges, double); }
+      else { if (mx) LD(kb_ppm_edges, float); else LD(kb_ppm_edges, double); }
+#undef LD
 #undef L
       break;
     }
@@ -1632,6 +1688,7 @@ hipError_t launch_batched(int kind, int variant, int threads, const BatchItem* i
 #define LT(Dd, Rc) \
   do { if (mx) L(Dd, Rc, float); else L(Dd, Rc, double); } while (0)
       if (irec == 5) { if (d == 1) LT(1, 5); else if (d == 2) LT(2, 5); else LT(3, 5); }
+      else if (irec == 1) { if (d == 1) LT(1, 1); else if (d == 2) LT(2, 1); else LT(3, 1); }
       else if (irec == 6) { if (d == 1) LT(1, 6); else if (d == 2) LT(2, 6); else LT(3, 6); }
       else { if (d == 1) LT(1, 7); else if (d == 2) LT(2, 7); else LT(3, 7); }
 #undef LT

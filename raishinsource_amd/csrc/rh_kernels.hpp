@@ -59,7 +59,7 @@ struct FixArgs {
 };
 
 struct EdgeArgs {
-  const double *W, *dxs, *xs, *pc;
+  const double *W, *dxs, *xs, *pc;  // pc: PPM width coefficients, or the face positions x*a for MC
   double gam;
   void *E;                         // float or double planes, by the launch's variant
 };
@@ -144,12 +144,13 @@ int block_threads(int kind);
 hipError_t launch_gather_first(const double* const* src, double* dst, int n, hipStream_t st);
 bool one_group_per_model(int kind);
 
-// PPM face values of every cell along dir into E (18 planes: left edges, right edges).
-// pc: the width-only interp coefficients of the direction (7 planes of n+2, ppm_coefficients).
-// mixed: the reconstruction and the face arithmetic in float (E holds
-// floats), else double.
-hipError_t launch_ppm_edges(const Geo& G, int dir, const double* W, const double* dxs, const double* xs,
-                            const double* pc, double gam, void* E, bool mixed, hipStream_t st);
+// Face values of every cell along dir into E (18 planes: left edges, right
+// edges): PPM (irec 5; tab = the width-only interp coefficients of the
+// direction, 7 planes of n+2, ppm_coefficients) or the MC limiter (irec 1;
+// tab = the face positions x*a).  mixed: the reconstruction and the face
+// arithmetic in float (E holds floats), else double.
+hipError_t launch_edges(const Geo& G, int dir, int irec, const double* W, const double* dxs, const double* xs,
+                        const double* tab, double gam, void* E, bool mixed, hipStream_t st);
 // Fluxes at the interfaces along dir; E = the PPM edges (irec 5), unused otherwise.
 hipError_t launch_sweep(const Geo& G, int dir, int irec, const double* W, const double* metF, const void* E,
                         double gam, double* F, double* CTsrc, bool mixed, hipStream_t st);

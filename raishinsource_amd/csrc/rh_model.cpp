@@ -40,7 +40,10 @@ int ghost_nm0(int irec) {
 
 bool check_pram(const raishin_pram& p) {
   if (p.imax < 1 || p.jmax < 1 || p.kmax < 1) { set_error("bad grid %d %d %d", p.imax, p.jmax, p.kmax); return false; }
-  if (p.irec != 5 && p.irec != 6 && p.irec != 7) { set_error("irec=%d not built (5 PPM, 6 MP5, 7 WENO5)", p.irec); return false; }
+  if (p.irec != 1 && p.irec != 5 && p.irec != 6 && p.irec != 7) {
+    set_error("irec=%d not built (1 MC, 5 PPM, 6 MP5, 7 WENO5)", p.irec);
+    return false;
+  }
   if (p.irkt != 2 && p.irkt != 3) { set_error("irkt=%d not supported", p.irkt); return false; }
   if (p.ict != 1) { set_error("ict=%d not built (1 = flux-CT)", p.ict); return false; }
   if (p.ihll != 1) { set_error("ihll=%d not built (1 = HLLE)", p.ihll); return false; }
@@ -79,7 +82,8 @@ void dfree_all(raishin_model* m) {
     if (m->xs[a]) (void)hipFree(m->xs[a]);
     if (m->dxb[a]) (void)hipFree(m->dxb[a]);
     if (m->dxs[a]) (void)hipFree(m->dxs[a]);
-    m->xs[a] = m->dxb[a] = m->dxs[a] = nullptr;
+    if (m->xa[a]) (void)hipFree(m->xa[a]);
+    m->xs[a] = m->dxb[a] = m->dxs[a] = m->xa[a] = nullptr;
   }
   if (m->bc) (void)hipFree(m->bc);
   if (m->dtbits) (void)hipFree(m->dtbits);
@@ -245,7 +249,7 @@ static bool build_model_(raishin_model* m, const raishin_pram* p, const double* 
       !dalloc(&m->cfl_min, 4 * cfl_blocks(G)))
     return false;
   if (m->need_wo && !dalloc(&m->Wo, 9 * N)) return false;
-  if (p->irec == 5 && !dalloc(&m->E, 18 * N)) return false;  // PPM face values
+  if ((p->irec == 5 || p->irec == 1) && !dalloc(&m->E, 18 * N)) return false;  // face values
   if (!dalloc(&m->stage_aos, 9 * N)) return false;            // set_state / get_state staging
   m->crows = ((N + 255) / 256) * 4;  // wavefronts of the largest launch (every cell, 256 per workgroup)
   if (!dalloc(&m->cpart, (size_t)m->crows * NCOUNT)) return false;
@@ -295,7 +299,13 @@ static bool build_model_(raishin_model* m, const raishin_pram* p, const double* 
   const int nn[3] = {G.ni, G.nj, G.nk};
   for (int a = 0; a < 3; a++) {
     const int n = nn[a];
-    if (!dalloc(&m->xs[a], n) || !dalloc(&m->dxb[a], n) || !dalloc(&m->dxs[a], n + 2)) return false;
+    if (!dalloc(&m->xs[a], n) || !dalloc(&m->dxb[a], n) || !dalloc(&m->dxs[a], n + 2) || !dalloc(&m->xa[a], n))
+      return false;
+    // x*a(i) = (x*(i) + x*(i+1))/2 as grid.f90 builds it (the last entry is never read)
+    std::vector<double> fa(n, 0.0);
+    for (int q = 0; q + 1 < n; q++) fa[q] = 0.5 * (xsrc[a][q] + xsrc[a][q + 1]);
+    if (n > 1) fa[n - 1] = fa[n - 2] + (xsrc[a][n - 1] - xsrc[a][n - 2]);
+    if (!h2d_sync(m->xa[a], fa.data(), n * 8, m->st)) return false;
     std::vector<double> db(n, 0.0), ds(n + 2, 0.0);
     for (int q = 0; q < n - 1; q++) db[q] = dsrc[a][q];
     for (int d = 0; d <= n + 1; d++) {
@@ -401,9 +411,10 @@ bool stage_sweeps(raishin_model* m, const double* W, double* F) {
   const long long N = G.N;
   const int irec = m->p.irec;
   for (int d = 1; d <= 3; d++) {
-    if (irec == 5) {
+    if (irec == 5 || irec == 1) {
       TimedLaunch t(m, K_EDGES);
-      HIPCHK(launch_ppm_edges(G, d, W, m->dxs[d - 1], m->xs[d - 1], m->ppmc[d - 1], m->p.gam, m->E, m->mixed, m->st));
+      const double* tab = irec == 5 ? m->ppmc[d - 1] : m->xa[d - 1];
+      HIPCHK(launch_edges(G, d, irec, W, m->dxs[d - 1], m->xs[d - 1], tab, m->p.gam, m->E, m->mixed, m->st));
     }
     TimedLaunch t(m, K_SWEEP1 + d - 1);
     HIPCHK(launch_sweep(G, d, irec, W, m->met[d], m->E, m->p.gam, F + (d - 1) * 9 * N, m->CT, m->mixed, m->st));

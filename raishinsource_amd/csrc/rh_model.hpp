@@ -39,12 +39,13 @@ struct raishin_model {
   double *Ucur = nullptr, *Uh = nullptr, *Us = nullptr, *Unext = nullptr;
   // F: ww(3,9,...) (27 planes); CT: the six transverse B flux copies ct.f90 averages
   double *F = nullptr, *CT = nullptr, *wgout = nullptr, *stage_aos = nullptr;
-  double* E = nullptr;  // PPM face values of one direction (18 planes), irec = 5 only
+  double* E = nullptr;  // face values of one direction (18 planes): PPM (irec 5) and MC (irec 1)
   double* ppmc[3] = {nullptr, nullptr, nullptr};  // PPM width coefficients per direction (irec = 5)
   // geometry
   double *met[4] = {nullptr, nullptr, nullptr, nullptr};  // centre, i-face, j-face, k-face
   double *chr = nullptr;
   double *xs[3] = {nullptr, nullptr, nullptr}, *dxb[3] = {nullptr, nullptr, nullptr}, *dxs[3] = {nullptr, nullptr, nullptr};
+  double* xa[3] = {nullptr, nullptr, nullptr};  // cell-face positions x*a (MC reconstruction)
   double *r15 = nullptr, *r25 = nullptr;
   int* bc = nullptr;
   unsigned long long *dtbits = nullptr, *cnt = nullptr, *carry = nullptr;

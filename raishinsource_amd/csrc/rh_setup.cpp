@@ -388,7 +388,8 @@ int raishin_mdtorus1(const raishin_pram* pp, const raishin_torus* tp, double* ur
   // normalise density to 1 and perturb the pressure.  The noise is drawn in the
   // reference's (k,j,i) order from a seeded splitmix64 stream; with one interior
   // phi cell the model is axisymmetric and one value is drawn per (i,k).
-  const bool axisym = (nj - 2 * ((p.irec == 5 ? 4 : 3) + 1)) <= 1;
+  const int nm0 = (p.irec == 5) ? 4 : (p.irec == 1 || p.irec == 2 || p.irec == 3) ? 2 : 3;  // main.f90:179-211
+  const bool axisym = (nj - 2 * (nm0 + 1)) <= 1;
   uint64_t seed = (uint64_t)t.seed;
   std::vector<double> noise((size_t)ni * nk, 0.0);
   for (int k = 1; k <= nk; k++)

@@ -82,14 +82,14 @@ def footprint(p):
     need_wo = any(2 in getattr(p, a) for a in ("iboux1in", "iboux1ot", "iboux2in", "iboux2ot", "iboux3in",
                                                "iboux3ot"))
     ncfl = max((max(ni - 2, 0) * max(nj - 2, 0) * max(nk - 2, 0) + 255) // 256, 1)
-    planes = 7 * 9 + 27 + 6 + 1 + 9 + (9 if need_wo else 0) + (18 if p.irec == 5 else 0)
+    planes = 7 * 9 + 27 + 6 + 1 + 9 + (9 if need_wo else 0) + (18 if p.irec in (1, 5) else 0)
     b = 8 * n * planes
     b += n + 8 * (1 + cap) + cap                     # cflag, carry list, pend
     b += 8 * 64 * p2 + 2 * 8 * ni + 4 * 54 + 8 * 4 + 8 * 11 + 8 * 2
     b += 2 * 8 * 4 * ncfl                            # CFL per-block results
     b += 8 * 11 * ((n + 255) // 256) * 4             # per-wavefront counter rows
     b += 4 * 8 * 33 * p2                             # metric tables
-    b += sum(8 * (3 * d + 2) for d in (ni, nj, nk))  # xs, dxb, dxs
+    b += sum(8 * (4 * d + 2) for d in (ni, nj, nk))  # xs, dxb, dxs, xa
     if p.irec == 5:
         b += sum(8 * 7 * (d + 2) for d in (ni, nj, nk))
     return b

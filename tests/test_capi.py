@@ -72,7 +72,7 @@ def test_error_path_without_gpu_work():
     """A rejected configuration reports through the sticky error, not a crash."""
     from dataclasses import replace
     from raishinsource_amd import configs
-    p = replace(configs.G_MP5, irec=1)   # MC limiter: not built
+    p = replace(configs.G_MP5, irec=2)   # minmod limiter: not built
     ps = native.pram_struct(p)
     lib = native.lib()
     buf = C.create_string_buffer(256)
@@ -81,5 +81,5 @@ def test_error_path_without_gpu_work():
     h = lib.raishin_model_create(C.byref(ps), *([C.cast(z, C.c_void_p)] * 19))
     assert not h
     assert lib.raishin_last_error(buf, 256) == 1
-    assert b"irec=1" in buf.value
+    assert b"irec=2" in buf.value
     lib.raishin_clear_error()

@@ -117,6 +117,19 @@ def test_batch_mp5_rk3():
     close_all(alone, batched)
 
 
+def test_batch_mc():
+    members = [(configs.U_MC, False, 0, 100.0), (configs.T2D_MC, True, 1, 10.0), (configs.U_MC, True, 2, 100.0)]
+    alone, batched = pairs(members)
+    b = native.Batch(batched)
+    for mm in alone:
+        mm.step(3)
+    b.step(1)
+    b.step_graph(1)  # one replay: two steps
+    assert_same(alone, batched, "mc")
+    b.close()
+    close_all(alone, batched)
+
+
 def test_batch_rejects_mixed_schemes():
     a = make(configs.U_PPM)
     c = make(configs.U_MP5)

@@ -20,7 +20,7 @@ def run(p, g, m, uri, nsteps, mixed):
     return w, t, cnt
 
 
-@pytest.mark.parametrize("name,nsteps", [("t2d", 20), ("uppm", 20), ("c1", 10)])
+@pytest.mark.parametrize("name,nsteps", [("t2d", 20), ("uppm", 20), ("c1", 10), ("umc", 20)])
 def test_mixed_density_l1_within_1e5(name, nsteps):
     p = configs.ALL[name].resolved()
     g = native.setup_grid(p)

@@ -251,6 +251,8 @@ ORACLE_CASES = [
     ("ump5", "torus", 8),
     ("uweno", "torus", 8),
     ("t3d", "torus", 4),
+    ("umc", "torus", 8),
+    ("t2dmc", "torus", 6),
     ("gppm", "synthetic", 4),
     ("gmp5", "synthetic", 4),
     ("gweno", "synthetic", 4),
@@ -342,8 +344,8 @@ def test_edge_grids_match_oracle(tag, p, nsteps, oracle_lib):
 
 def test_rejects_unbuilt_configuration():
     p, g, m, _ = make("gmp5", "synthetic")
-    with pytest.raises(native.RaishinError, match="irec=1"):
-        native.Model(replace(p, irec=1), g, m)
+    with pytest.raises(native.RaishinError, match="irec=2"):
+        native.Model(replace(p, irec=2), g, m)
     with pytest.raises(native.RaishinError, match="mirror"):
         q = replace(configs.G_PPM, kmax=14).resolved()
         gq = native.setup_grid(q)

@@ -49,7 +49,7 @@ def small_requests(n, rate, seed):
 
 
 def test_footprint_matches_allocation():
-    for p in (configs.U_TUBE, configs.T2D, configs.U_PPM, configs.U_MP5,
+    for p in (configs.U_TUBE, configs.T2D, configs.U_PPM, configs.U_MP5, configs.U_MC,
               replace(configs.C1, imax=74, jmax=74, kmax=74, ymax=None)):
         p = p.resolved()
         g = native.setup_grid(p)
