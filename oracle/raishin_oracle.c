@@ -657,8 +657,31 @@ void orc_rec2(const double *uri, const double *x1, const double *x2, const doubl
 }
 
 /* ------------------------------------------------------------------ */
-/* calcha4i/j/k: calcha.f90:370-906 (face speeds, icha=2)               */
+/* calcha4i/j/k: calcha.f90:370-906 (face speeds, icha 0/1/2)           */
 /* ------------------------------------------------------------------ */
+/* The speed pair of direction d before the lapse/shift correction:
+   icha 0 calcha.f90:489-494 (Gammie), 1 :496-509 (Leismann), 2 :511-527
+   (Del Zanna); gii = gcon3a(d,d), vb = vcov.bcon (calcha.f90:478). */
+static void speed_pair(double vd, double vsq, double omsq, double cssq, double vb, double rohb,
+                       double gfl, double gii, double *p, double *m) {
+  if (P.icha == 0) {
+    *p = vd + sqrt(omsq);
+    *m = vd - sqrt(omsq);
+    return;
+  }
+  double f1 = vd * (1.0 - omsq), f2, f3;
+  if (P.icha == 1) {
+    double r1 = cssq * (vb * vb) * (1.0 / (rohb * (gfl * gfl)));
+    f2 = 1.0 - vsq * omsq - r1;
+    f3 = ((vsq - 1.0) * omsq + r1) * ((gii * vsq - vd * vd) * omsq + vd * vd - gii + r1);
+  } else {
+    f2 = 1.0 - vsq * omsq;
+    f3 = omsq * (1.0 - vsq) * ((1.0 - vsq * omsq) * gii - (1.0 - omsq) * vd * vd);
+  }
+  *p = (f1 + sqrt(f3)) * (1.0 / f2);
+  *m = (f1 - sqrt(f3)) * (1.0 / f2);
+}
+
 static void face_speed(const double *w, const double *g, const double *gi, int d,
                        double *vfp, double *vfm) {
   prel_t s;
@@ -682,11 +705,9 @@ static void face_speed(const double *w, const double *g, const double *gi, int d
   if (d == 2) vasq = s.bbsq / (s.roh + s.bbsq);
   else vasq = s.bbsq * (1.0 / (s.roh + s.bbsq));
   double omsq = vasq + cssq * (1.0 - vasq);
-  double f1 = vcon[d] * (1.0 - omsq);
-  double f2 = 1.0 - vsq * omsq;
-  double f3 = omsq * (1.0 - vsq) * ((1.0 - vsq * omsq) * gi3[(d - 1) * 4] - (1.0 - omsq) * vcon[d] * vcon[d]);
-  double p = (f1 + sqrt(f3)) * (1.0 / f2);
-  double m = (f1 - sqrt(f3)) * (1.0 / f2);
+  double vb = vcov[1] * s.bcon[1] + vcov[2] * s.bcon[2] + vcov[3] * s.bcon[3];
+  double p, m;
+  speed_pair(vcon[d], vsq, omsq, cssq, vb, s.roh + s.bbsq, s.gfl, gi3[(d - 1) * 4], &p, &m);
   p = alpha1 * p - beta_d;
   m = alpha1 * m - beta_d;
   if (fabs(p) > 1.0) p = 1.0;
@@ -701,7 +722,7 @@ void orc_calcha(const double *uriir, const double *urijr, const double *urikr,
                 const double *gconi, const double *gconj, const double *gconk,
                 double *cmaxi, double *cmini, double *cmaxj, double *cminj,
                 double *cmaxk, double *cmink, int nm0) {
-  if (P.icha != 2) abort();
+  if (P.icha < 0 || P.icha > 2) abort();
   for (int k = nm0; k <= NK - nm0; k++)
     for (int j = nm0; j <= NJ - nm0; j++)
       for (int i = nm0; i <= NI - nm0; i++) {
@@ -1308,13 +1329,9 @@ static void centre_speeds(const double *w, const double *g, const double *gi, do
   double cssq = P.gam * s.pr / s.roh;
   double vasq = s.bbsq * (1.0 / (s.roh + s.bbsq));
   double omsq = vasq + cssq * (1.0 - vasq);
-  double f2 = 1.0 - vsq * omsq;
-  for (int d = 1; d <= 3; d++) {
-    double f1 = vcon[d] * (1.0 - omsq);
-    double f3 = omsq * (1.0 - vsq) * ((1.0 - vsq * omsq) * gi3[(d - 1) * 4] - (1.0 - omsq) * vcon[d] * vcon[d]);
-    vp[d] = (f1 + sqrt(f3)) * (1.0 / f2);
-    vm[d] = (f1 - sqrt(f3)) * (1.0 / f2);
-  }
+  double vb = vcov[1] * s.bcon[1] + vcov[2] * s.bcon[2] + vcov[3] * s.bcon[3];
+  for (int d = 1; d <= 3; d++)
+    speed_pair(vcon[d], vsq, omsq, cssq, vb, s.roh + s.bbsq, s.gfl, gi3[(d - 1) * 4], &vp[d], &vm[d]);
   for (int d = 1; d <= 3; d++) {
     vp[d] = alpha1 * vp[d] - beta1[d];
     vm[d] = alpha1 * vm[d] - beta1[d];
@@ -1326,7 +1343,7 @@ static void centre_speeds(const double *w, const double *g, const double *gi, do
 void orc_cdtcfl4(const double *uri, const double *gcov, const double *gcon, double *dtcfl,
                  const double *dx1, const double *dx2, const double *dx3,
                  double *dt1, double *dt2, double *dt3, int nm1) {
-  if (P.icha != 2) abort();
+  if (P.icha < 0 || P.icha > 2) abort();
   const int nmax = nm1 * 2 + (nm1 - 1);
   const double small = 1.0e-12;
   double dtc = 1.0, d1 = 1.0, d2 = 1.0, d3 = 1.0;

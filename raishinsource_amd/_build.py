@@ -5,6 +5,7 @@ keep IEEE FP64 semantics that the parity with the reference relies on:
 -ffp-contract=off (no FMA contraction), no fast-math.  Packed FP32 ops are
 disabled for all device code.
 """
+import concurrent.futures
 import os
 import subprocess
 import sys
@@ -40,18 +41,20 @@ def _run(cmd):
 
 def build(verbose=False):
     os.makedirs(OUT, exist_ok=True)
-    objs = []
+    objs, cmds = [], []
     for src, kind in UNITS:
         s = os.path.join(CSRC, src)
         o = os.path.join(OUT, src + ".o")
         if kind == "hip":
-            cmd = [HIPCC] + HIP_FLAGS + ["-c", s, "-o", o]
+            cmds.append([HIPCC] + HIP_FLAGS + ["-c", s, "-o", o])
         else:
-            cmd = ["g++"] + HOST_FLAGS + ["-c", s, "-o", o]
-        out = _run(cmd)
-        if verbose and out:
-            print(out)
+            cmds.append(["g++"] + HOST_FLAGS + ["-c", s, "-o", o])
         objs.append(o)
+    # the units are independent: compile them side by side
+    with concurrent.futures.ThreadPoolExecutor(max_workers=min(len(cmds), 4)) as ex:
+        for out in ex.map(_run, cmds):
+            if verbose and out:
+                print(out)
     _run([HIPCC, "-shared", "-fPIC", f"--offload-arch={ARCH}", "-o", LIB] + objs + ["-fopenmp"])
     build_fortran(verbose)
     return LIB

@@ -25,7 +25,7 @@ class Pram:
     irkt: int = 2
     ict: int = 1
     ihll: int = 1
-    icha: int = 2
+    icha: int = 2          # 0 Gammie, 1 Leismann, 2 Del Zanna wave speeds
     iwvec: int = 1
     ieos: int = 0
     model: int = 7
@@ -122,10 +122,14 @@ def tube(name, nzones, nm1=5):
 # MC-limited linear reconstruction (irec=1, rec.f90:208): nm0=2, nm1=3
 U_MC = Pram("umc", 16, 10, 18, irec=1, irkt=2, cfl=0.4)
 T2D_MC = Pram("t2dmc", 64, 7, 64, irec=1, irkt=2, cfl=0.4)
+# the other wave-speed formulas (calcha.f90:489-509): Leismann et al. (icha=1,
+# with MC and RK3 as pram_3Dtorus.f90:66-70 pairs them) and Gammie et al. (icha=0)
+U_LEIS = Pram("uleis", 16, 10, 18, irec=1, irkt=3, cfl=0.3, icha=1)
+T2D_GAM = Pram("t2dgam", 64, 11, 64, irec=5, irkt=2, cfl=0.4, icha=0)
 
 # the serving mix's tube (1024 zones) and a small one for the oracle
 TUBE = tube("tube", 1024)
 U_TUBE = tube("utube", 64)
 
 ALL = {p.name: p for p in (C1, C2, C2_PPM, C2_WENO, U_PPM, U_MP5, U_WENO, T2D, T3D, G_PPM, G_MP5, G_WENO, TUBE,
-                           U_TUBE, U_MC, T2D_MC)}
+                           U_TUBE, U_MC, T2D_MC, U_LEIS, T2D_GAM)}

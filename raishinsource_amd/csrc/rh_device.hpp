@@ -36,6 +36,7 @@ struct Geo {
   // so every phi row holds the same values at all times; row jfix alone is
   // stored and computed, and every j maps to it.
   int njs, jfix;
+  int icha;              // wave-speed formula (calcha.f90): 0 Gammie, 1 Leismann, 2 Del Zanna
   // workgroup size per tuned launch slot (TuneSlot in rh_kernels.hpp);
   // 0 means 256.  Chosen by raishin_model_autotune or set by the caller.
   unsigned short bs[8];
@@ -183,10 +184,46 @@ RH_DEV void tmunu_row(const PrelT<R>& s, int d, R detg, R powdg, R out[9]) {
   for (int n = 0; n < 9; n++) out[n] = detg * out[n];
 }
 
-// Fast magnetosonic speeds (icha=2, Del Zanna et al. 2007) along direction d
-// for a face state: calcha4i/j/k (calcha.f90:370-906).  The three directions
-// write cs^2 and vA^2 differently (calcha.f90:480,648,826 and 486,654,832).
+// The (f1 +- sqrt f3) / f2 speed pair of one direction d for the three
+// formulas of calcha.f90 (icha 0: Gammie et al. 2003, vcon +- sqrt(omsq);
+// icha 1: Leismann et al. 2005 with r1 = cs^2 (v.B)^2 / ((rho h + b^2) W^2);
+// icha 2: Del Zanna et al. 2007), before the lapse/shift correction.
+// gii = gcon3a(d,d); vb = v_i B^i; the evaluation order is the reference's
+// (calcha.f90:250-310, 489-527 and the j/k twins).
+template <class R, int ICHA>
+RH_DEV void speed_pair(R vd, R vsq, R omsq, R cssq, R vb, R rohb, R gfl, R gii, R& p, R& m) {
+  if constexpr (ICHA == 0) {
+    p = vd + sqrtr(omsq);
+    m = vd - sqrtr(omsq);
+    return;
+  }
+  const R f1 = vd * (R(1) - omsq);
+  R f2, f3;
+  if constexpr (ICHA == 1) {
+    const R r1 = cssq * (vb * vb) * (R(1) / (rohb * (gfl * gfl)));
+    f2 = R(1) - vsq * omsq - r1;
+    f3 = ((vsq - R(1)) * omsq + r1) * ((gii * vsq - vd * vd) * omsq + vd * vd - gii + r1);
+  } else {
+    f2 = R(1) - vsq * omsq;
+    f3 = omsq * (R(1) - vsq) * ((R(1) - vsq * omsq) * gii - (R(1) - omsq) * vd * vd);
+  }
+  p = (f1 + sqrtr(f3)) * (R(1) / f2);
+  m = (f1 - sqrtr(f3)) * (R(1) / f2);
+}
+
+// v_i B^i with v_i = gcov3a v^j (lower1, physfunc.f90:233-258)
 template <class R>
+RH_DEV R v_dot_b(const R* g3, const R vcon[4], const R bcon[4]) {
+  R vcov[4];
+#pragma unroll
+  for (int m = 1; m <= 3; m++) vcov[m] = g3[(m - 1) + 0] * vcon[1] + g3[(m - 1) + 3] * vcon[2] + g3[(m - 1) + 6] * vcon[3];
+  return vcov[1] * bcon[1] + vcov[2] * bcon[2] + vcov[3] * bcon[3];
+}
+
+// Fast magnetosonic speeds along direction d for a face state:
+// calcha4i/j/k (calcha.f90:370-906).  The three directions write cs^2 and
+// vA^2 differently (calcha.f90:480,648,826 and 486,654,832).
+template <class R, int ICHA>
 RH_DEV void face_speed(const PrelT<R>& s, int d, R gam, const Met& M, R& vfp, R& vfm) {
   const R alpha1 = R(1) / sqrtr(-R(M.gi(0, 0)));
   const R beta_d = alpha1 * alpha1 * R(M.gi(0, d));
@@ -207,11 +244,9 @@ RH_DEV void face_speed(const PrelT<R>& s, int d, R gam, const Met& M, R& vfp, R&
   if (d == 2) vasq = s.bbsq / (s.roh + s.bbsq);
   else vasq = s.bbsq * (R(1) / (s.roh + s.bbsq));
   const R omsq = vasq + cssq * (R(1) - vasq);
-  const R f1 = vcon[d] * (R(1) - omsq);
-  const R f2 = R(1) - vsq * omsq;
-  const R f3 = omsq * (R(1) - vsq) * ((R(1) - vsq * omsq) * gi3[(d - 1) * 4] - (R(1) - omsq) * vcon[d] * vcon[d]);
-  R p = (f1 + sqrtr(f3)) * (R(1) / f2);
-  R m = (f1 - sqrtr(f3)) * (R(1) / f2);
+  const R vb = ICHA == 1 ? v_dot_b<R>(g3, vcon, s.bcon) : R(0);
+  R p, m;
+  speed_pair<R, ICHA>(vcon[d], vsq, omsq, cssq, vb, s.roh + s.bbsq, s.gfl, gi3[(d - 1) * 4], p, m);
   p = alpha1 * p - beta_d;
   m = alpha1 * m - beta_d;
   if (absr(p) > R(1)) p = R(1);
@@ -220,6 +255,7 @@ RH_DEV void face_speed(const PrelT<R>& s, int d, R gam, const Met& M, R& vfp, R&
 }
 
 // Centre speeds for the CFL condition: calcha4 (calcha.f90:129-367).
+template <int ICHA>
 RH_DEV void centre_speeds(const Prel& s, double gam, const Met& M, double vp[4], double vm[4]) {
   const double alpha1 = 1.0 / sqrtr(-M.gi(0, 0));
   double beta1[4];
@@ -240,14 +276,10 @@ RH_DEV void centre_speeds(const Prel& s, double gam, const Met& M, double vp[4],
   const double cssq = gam * s.pr / s.roh;
   const double vasq = s.bbsq * (1.0 / (s.roh + s.bbsq));
   const double omsq = vasq + cssq * (1.0 - vasq);
-  const double f2 = 1.0 - vsq * omsq;
+  const double vb = ICHA == 1 ? v_dot_b<double>(g3, vcon, s.bcon) : 0.0;
 #pragma unroll
-  for (int d = 1; d <= 3; d++) {
-    const double f1 = vcon[d] * (1.0 - omsq);
-    const double f3 = omsq * (1.0 - vsq) * ((1.0 - vsq * omsq) * gi3[(d - 1) * 4] - (1.0 - omsq) * vcon[d] * vcon[d]);
-    vp[d] = (f1 + sqrtr(f3)) * (1.0 / f2);
-    vm[d] = (f1 - sqrtr(f3)) * (1.0 / f2);
-  }
+  for (int d = 1; d <= 3; d++)
+    speed_pair<double, ICHA>(vcon[d], vsq, omsq, cssq, vb, s.roh + s.bbsq, s.gfl, gi3[(d - 1) * 4], vp[d], vm[d]);
 #pragma unroll
   for (int d = 1; d <= 3; d++) {
     vp[d] = alpha1 * vp[d] - beta1[d];

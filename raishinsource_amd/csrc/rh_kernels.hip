@@ -7,6 +7,7 @@
 //                  so each location set is 33 planes of ni*nk values and the
 //                  Christoffel symbols 64 planes.
 // Each kernel cites the reference routines it restates.
+#include <type_traits>
 #include "rh_device.hpp"
 #include "rh_kernels.hpp"
 
@@ -336,7 +337,7 @@ RH_DEV void b_mc_edges(const Geo& G, const EdgeArgs& A, long long blk) {
 // averaging); inside the CT box fluxct then writes F itself.
 //
 // PPM's face values come from k_ppm_edges (E); MP5 and WENO5 reconstruct here.
-template <int D, int IREC, class R>
+template <int D, int IREC, class R, int ICHA>
 RH_DEV void b_sweep(const Geo& G, const SweepArgs& A, long long blk) {
   const double* __restrict__ W = A.W;
   const double* __restrict__ metF = A.metF;
@@ -384,7 +385,7 @@ RH_DEV void b_sweep(const Geo& G, const SweepArgs& A, long long blk) {
     const R pw = powr(sl.de, gam);
     tmunu_row<R>(sl, 0, detg, pw, Ul);
     tmunu_row<R>(sl, D, detg, pw, Fl);
-    face_speed<R>(sl, D, gam, M, vpl, vml);
+    face_speed<R, ICHA>(sl, D, gam, M, vpl, vml);
   }
   R Ur[9], Fr[9], vpr, vmr;
   {
@@ -393,7 +394,7 @@ RH_DEV void b_sweep(const Geo& G, const SweepArgs& A, long long blk) {
     const R pw = powr(sr.de, gam);
     tmunu_row<R>(sr, 0, detg, pw, Ur);
     tmunu_row<R>(sr, D, detg, pw, Fr);
-    face_speed<R>(sr, D, gam, M, vpr, vmr);
+    face_speed<R, ICHA>(sr, D, gam, M, vpr, vmr);
   }
   const R cmax = fmaxf_(fmaxf_(R(0), vpr), vpl);
   const R cmin = (D == 3) ? fminf_(fminf_(R(0), vpr), vml)   // calcha.f90:107
@@ -1109,6 +1110,7 @@ RH_DEV long long wave_max_ll(long long v) {
   return v;
 }
 
+template <int ICHA>
 RH_DEV void b_cfl(const Geo& G, const CflArgs& A, long long blk) {
   __shared__ long long s_last[4][4];
   __shared__ double s_min[4][4];
@@ -1125,7 +1127,7 @@ RH_DEV void b_cfl(const Geo& G, const CflArgs& A, long long blk) {
     Prel s;
     prelude(w, A.gam, M, s);
     double vp[4], vm[4];
-    centre_speeds(s, A.gam, M, vp, vm);
+    centre_speeds<ICHA>(s, A.gam, M, vp, vm);
     const int nmax = G.nm1 * 2 + (G.nm1 - 1);
     const int nd[4] = {0, G.ni, G.nj, G.nk};
     const double dxv[4] = {0.0, A.dx1b[i - 1], A.dx2b[j - 1], A.dx3b[k - 1]};
@@ -1390,8 +1392,8 @@ template <int D, class R>
 __global__ void __launch_bounds__(256) k_ppm_edges(Geo G, EdgeArgs A) { b_ppm_edges<D, R>(G, A, xcd_block()); }
 template <int D, class R>
 __global__ void __launch_bounds__(256) k_mc_edges(Geo G, EdgeArgs A) { b_mc_edges<D, R>(G, A, xcd_block()); }
-template <int D, int IREC, class R>
-__global__ void __launch_bounds__(256) k_sweep(Geo G, SweepArgs A) { b_sweep<D, IREC, R>(G, A, xcd_block()); }
+template <int D, int IREC, class R, int ICHA>
+__global__ void __launch_bounds__(256) k_sweep(Geo G, SweepArgs A) { b_sweep<D, IREC, R, ICHA>(G, A, xcd_block()); }
 __global__ void __launch_bounds__(256) k_fluxct(Geo G, CtArgs A) { b_fluxct(G, A, xcd_block()); }
 template <int STAGE, bool SYM>
 __global__ void __launch_bounds__(256) k_rk(Geo G, StageArgs A) { b_rk<STAGE, SYM>(G, A, xcd_block()); }
@@ -1399,7 +1401,8 @@ __global__ void __launch_bounds__(256) k_recov(Geo G, StageArgs A) { b_recov(G, 
 __global__ void __launch_bounds__(1024) k_recov_carry(Geo G, FixArgs A) { b_recov_carry(G, A); }
 __global__ void __launch_bounds__(256) k_bnd(Geo G, PostArgs A) { b_bnd(G, A, blockIdx.x); }
 __global__ void __launch_bounds__(256) k_post(Geo G, PostArgs A) { b_post(G, A, xcd_block()); }
-__global__ void __launch_bounds__(256) k_cfl(Geo G, CflArgs A) { b_cfl(G, A, blockIdx.x); }
+template <int ICHA>
+__global__ void __launch_bounds__(256) k_cfl(Geo G, CflArgs A) { b_cfl<ICHA>(G, A, blockIdx.x); }
 __global__ void __launch_bounds__(1024) k_cfl_finish(CflArgs A) { b_cfl_finish(A); }
 __global__ void __launch_bounds__(64) k_advance_time(AdvArgs A) { b_advance_time(A); }
 
@@ -1420,11 +1423,11 @@ __global__ void __launch_bounds__(256) kb_mc_edges(const BatchItem* __restrict__
   const EdgeArgs A = it[q].A.e;
   b_mc_edges<D, R>(G, A, blk);
 }
-template <int D, int IREC, class R>
+template <int D, int IREC, class R, int ICHA>
 __global__ void __launch_bounds__(256) kb_sweep(const BatchItem* __restrict__ it, int n) {
   RH_BATCH_PROLOGUE
   const SweepArgs A = it[q].A.s;
-  b_sweep<D, IREC, R>(G, A, blk);
+  b_sweep<D, IREC, R, ICHA>(G, A, blk);
 }
 __global__ void __launch_bounds__(256) kb_fluxct(const BatchItem* __restrict__ it, int n) {
   RH_BATCH_PROLOGUE
@@ -1452,10 +1455,11 @@ __global__ void __launch_bounds__(256) kb_post(const BatchItem* __restrict__ it,
   const PostArgs A = it[q].A.p;
   b_post(G, A, blk);
 }
+template <int ICHA>
 __global__ void __launch_bounds__(256) kb_cfl(const BatchItem* __restrict__ it, int n) {
   RH_BATCH_PROLOGUE
   const CflArgs A = it[q].A.cf;
-  b_cfl(G, A, blk);
+  b_cfl<ICHA>(G, A, blk);
 }
 __global__ void __launch_bounds__(256) kb_copy(const BatchItem* __restrict__ it, int n) {
   RH_BATCH_PROLOGUE
@@ -1536,6 +1540,43 @@ hipError_t launch_edges(const Geo& G, int dir, int irec, const double* W, const 
   return hipGetLastError();
 }
 
+// Sweep kernels are instantiated per (direction, reconstruction, arithmetic
+// type, wave-speed formula); the variant number packs the four so that batch
+// plans group launches of the same kernel:
+//   dir + 4*mixed + 8*irec + 64*icha.
+template <int V> using IC = std::integral_constant<int, V>;
+template <class T> struct TypeTag { using type = T; };
+
+static int sweep_variant(int dir, bool mixed, int irec, int icha) { return dir + 4 * (int)mixed + 8 * irec + 64 * icha; }
+
+template <class F>
+static void with_icha(int icha, F&& f) {
+  if (icha == 2) f(IC<2>{});
+  else if (icha == 1) f(IC<1>{});
+  else f(IC<0>{});
+}
+
+template <class F>
+static void with_sweep(int variant, F&& f) {
+  const int d = variant & 3, irec = (variant >> 3) & 7;
+  const bool mx = (variant & 4) != 0;
+  with_icha(variant >> 6, [&](auto C) {
+    auto t = [&](auto D, auto I) {
+      if (mx) f(D, I, TypeTag<float>{}, C);
+      else f(D, I, TypeTag<double>{}, C);
+    };
+    auto r = [&](auto I) {
+      if (d == 1) t(IC<1>{}, I);
+      else if (d == 2) t(IC<2>{}, I);
+      else t(IC<3>{}, I);
+    };
+    if (irec == 5) r(IC<5>{});
+    else if (irec == 1) r(IC<1>{});
+    else if (irec == 6) r(IC<6>{});
+    else r(IC<7>{});
+  });
+}
+
 hipError_t launch_sweep(const Geo& G, int dir, int irec, const double* W, const double* metF, const void* E,
                         double gam, double* F, double* CTsrc, bool mixed, hipStream_t st) {
   const long long n = box(G.nm0, G.ni - G.nm0) * jbox(G, G.nm0, G.nj - G.nm0) * box(G.nm0, G.nk - G.nm0);
@@ -1545,19 +1586,14 @@ hipError_t launch_sweep(const Geo& G, int dir, int irec, const double* W, const 
   if (g_rec) {
     AnyArgs a{};
     a.s = A;
-    record(LK_SWEEP, dir + 4 * (int)mixed + 8 * irec, bt, nb, G, a);
+    record(LK_SWEEP, sweep_variant(dir, mixed, irec, G.icha), bt, nb, G, a);
     return hipSuccess;
   }
   if (n == 0) return hipSuccess;
-#define L(Dd, Rc, T) hipLaunchKernelGGL((k_sweep<Dd, Rc, T>), dim3(nb), dim3(bt), 0, st, G, A)
-#define LT(Dd, Rc) \
-  do { if (mixed) L(Dd, Rc, float); else L(Dd, Rc, double); } while (0)
-  if (irec == 5) { if (dir == 1) LT(1, 5); else if (dir == 2) LT(2, 5); else LT(3, 5); }
-  else if (irec == 1) { if (dir == 1) LT(1, 1); else if (dir == 2) LT(2, 1); else LT(3, 1); }
-  else if (irec == 6) { if (dir == 1) LT(1, 6); else if (dir == 2) LT(2, 6); else LT(3, 6); }
-  else { if (dir == 1) LT(1, 7); else if (dir == 2) LT(2, 7); else LT(3, 7); }
-#undef LT
-#undef L
+  with_sweep(sweep_variant(dir, mixed, irec, G.icha), [&](auto D, auto I, auto T, auto C) {
+    hipLaunchKernelGGL((k_sweep<D.value, I.value, typename decltype(T)::type, C.value>), dim3(nb), dim3(bt), 0, st,
+                       G, A);
+  });
   return hipGetLastError();
 }
 
@@ -1636,11 +1672,11 @@ hipError_t launch_cfl(const Geo& G, const CflArgs& A0, hipStream_t st) {
   if (g_rec) {
     AnyArgs a{};
     a.cf = A;
-    record(LK_CFL, 0, 256, nb, G, a);
+    record(LK_CFL, G.icha, 256, nb, G, a);
     record(LK_CFL_FINISH, 0, 1024, 1, G, a);
     return hipSuccess;
   }
-  hipLaunchKernelGGL(k_cfl, dim3(nb), dim3(256), 0, st, G, A);
+  with_icha(G.icha, [&](auto C) { hipLaunchKernelGGL((k_cfl<C.value>), dim3(nb), dim3(256), 0, st, G, A); });
   hipLaunchKernelGGL(k_cfl_finish, dim3(1), dim3(1024), 0, st, A);
   return hipGetLastError();
 }
@@ -1681,20 +1717,11 @@ hipError_t launch_batched(int kind, int variant, int threads, const BatchItem* i
 #undef L
       break;
     }
-    case LK_SWEEP: {
-      const int d = variant & 3, irec = variant >> 3;
-      const bool mx = (variant & 4) != 0;
-#define L(Dd, Rc, T) hipLaunchKernelGGL((kb_sweep<Dd, Rc, T>), g, b, 0, st, it, n)
-#define LT(Dd, Rc) \
-  do { if (mx) L(Dd, Rc, float); else L(Dd, Rc, double); } while (0)
-      if (irec == 5) { if (d == 1) LT(1, 5); else if (d == 2) LT(2, 5); else LT(3, 5); }
-      else if (irec == 1) { if (d == 1) LT(1, 1); else if (d == 2) LT(2, 1); else LT(3, 1); }
-      else if (irec == 6) { if (d == 1) LT(1, 6); else if (d == 2) LT(2, 6); else LT(3, 6); }
-      else { if (d == 1) LT(1, 7); else if (d == 2) LT(2, 7); else LT(3, 7); }
-#undef LT
-#undef L
+    case LK_SWEEP:
+      with_sweep(variant, [&](auto D, auto I, auto T, auto C) {
+        hipLaunchKernelGGL((kb_sweep<D.value, I.value, typename decltype(T)::type, C.value>), g, b, 0, st, it, n);
+      });
       break;
-    }
     case LK_FLUXCT: hipLaunchKernelGGL(kb_fluxct, g, b, 0, st, it, n); break;
     case LK_RK: {
       const int stg = variant & 7;
@@ -1712,7 +1739,9 @@ hipError_t launch_batched(int kind, int variant, int threads, const BatchItem* i
     case LK_RECOV: hipLaunchKernelGGL(kb_recov, g, b, 0, st, it, n); break;
     case LK_BND: hipLaunchKernelGGL(kb_bnd, g, b, 0, st, it, n); break;
     case LK_POST: hipLaunchKernelGGL(kb_post, g, b, 0, st, it, n); break;
-    case LK_CFL: hipLaunchKernelGGL(kb_cfl, g, b, 0, st, it, n); break;
+    case LK_CFL:
+      with_icha(variant, [&](auto C) { hipLaunchKernelGGL((kb_cfl<C.value>), g, b, 0, st, it, n); });
+      break;
     case LK_COPY: hipLaunchKernelGGL(kb_copy, g, b, 0, st, it, n); break;
     case LK_CARRY: hipLaunchKernelGGL(kb_recov_carry, dim3(n), dim3(1024), 0, st, it, n); break;
     case LK_CFL_FINISH: hipLaunchKernelGGL(kb_cfl_finish, dim3(n), dim3(1024), 0, st, it, n); break;

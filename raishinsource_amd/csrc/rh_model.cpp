@@ -47,7 +47,10 @@ bool check_pram(const raishin_pram& p) {
   if (p.irkt != 2 && p.irkt != 3) { set_error("irkt=%d not supported", p.irkt); return false; }
   if (p.ict != 1) { set_error("ict=%d not built (1 = flux-CT)", p.ict); return false; }
   if (p.ihll != 1) { set_error("ihll=%d not built (1 = HLLE)", p.ihll); return false; }
-  if (p.icha != 2) { set_error("icha=%d not built (2 = Del Zanna speeds)", p.icha); return false; }
+  if (p.icha < 0 || p.icha > 2) {
+    set_error("icha=%d not built (0 Gammie, 1 Leismann, 2 Del Zanna speeds)", p.icha);
+    return false;
+  }
   if (p.iwvec != 1) { set_error("iwvec=%d not built (1 = recov1a)", p.iwvec); return false; }
   if (p.ieos != 0) { set_error("ieos=%d not built (0 = gamma law)", p.ieos); return false; }
   const int nm1 = ghost_nm0(p.irec) + 1;
@@ -227,6 +230,7 @@ static bool build_model_(raishin_model* m, const raishin_pram* p, const double* 
   Geo& G = m->G;
   G.ni = p->imax; G.nj = p->jmax; G.nk = p->kmax;
   G.nm0 = ghost_nm0(p->irec); G.nm1 = G.nm0 + 1;
+  G.icha = p->icha;
   G.N = (long long)G.ni * G.nj * G.nk;
   G.P2 = (long long)G.ni * G.nk;
   G.njs = G.nj;

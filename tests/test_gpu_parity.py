@@ -253,6 +253,8 @@ ORACLE_CASES = [
     ("t3d", "torus", 4),
     ("umc", "torus", 8),
     ("t2dmc", "torus", 6),
+    ("uleis", "torus", 6),
+    ("t2dgam", "torus", 6),
     ("gppm", "synthetic", 4),
     ("gmp5", "synthetic", 4),
     ("gweno", "synthetic", 4),

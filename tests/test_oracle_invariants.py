@@ -269,3 +269,60 @@ def test_tube_uniform_state_stays_uniform(oracle_lib):
     assert np.isclose(line[5, 0], 0.3 * 0.2 / 0.3 ** (p.gam - 1.0), rtol=1e-7)
     # a tube of cube cells at |v| < 1: dt = cfl * dx / (fastest speed) with speed < 1
     assert dt > 0.4 * g["dx1b"][1]
+
+
+def _speeds(p, m, uri):
+    o = Oracle(p)
+    sh = (p.imax, p.jmax, p.kmax)
+    c = [np.asfortranarray(np.zeros(sh)) for _ in range(6)]
+    o.call("calcha", uri, uri, uri, uri, uri, uri, m["gcovi"], m["gcovj"], m["gcovk"], m["gconi"], m["gconj"],
+           m["gconk"], *c, p.nm0)
+    n = p.nm0
+    sl = (slice(n - 1, p.imax - n), slice(n - 1, p.jmax - n), slice(n - 1, p.kmax - n))
+    return np.stack([x[sl] for x in c])
+
+
+def _state(p, rng, field, velocity):
+    uri = fzeros(9, p.imax, p.jmax, p.kmax)
+    uri[0] = 0.5 + rng.random(uri[0].shape)
+    uri[4] = 0.1 + rng.random(uri[0].shape)
+    if velocity:
+        uri[1:4] = 0.3 * rng.standard_normal(uri[1:4].shape)
+    if field:
+        uri[6:9] = 0.5 * rng.standard_normal(uri[6:9].shape)
+    return uri
+
+
+def test_wave_speed_formulas_agree_where_they_must(oracle_lib):
+    """calcha.f90:489-527.  Without a field (v.B = 0, so r1 = 0) Leismann's
+    f2 and f3 equal Del Zanna's algebraically: the speeds agree to round-off
+    in the Kerr-Schild metric.  At rest in flat space all three formulas give
+    +-sqrt(omsq) (Del Zanna's f3 = omsq * gcon3a(d,d) with gcon3a = 1).  With
+    a field and motion the Leismann term r1 changes the speeds."""
+    p = configs.U_PPM.resolved()
+    g = native.setup_grid(p)
+    m = native.setup_metric(p, g)
+    rng = np.random.default_rng(4)
+    uri = _state(p, rng, field=False, velocity=True)
+    s1, s2 = _speeds(replace(p, icha=1), m, uri), _speeds(replace(p, icha=2), m, uri)
+    # the few states that leave f3 < 0 give NaN in both, at the same places
+    assert np.array_equal(np.isnan(s1), np.isnan(s2))
+    assert np.allclose(s1, s2, rtol=1e-12, atol=1e-14, equal_nan=True)
+    assert not np.array_equal(_speeds(replace(p, icha=0), m, uri), s2, equal_nan=True)
+    uri = _state(p, rng, field=True, velocity=True)
+    s1, s2 = _speeds(replace(p, icha=1), m, uri), _speeds(replace(p, icha=2), m, uri)
+    assert np.nanmax(np.abs(s1 - s2)) > 1e-3
+    q = flat(configs.G_MP5, imax=12, jmax=10, kmax=11)
+    gq = native.setup_grid(q)
+    mq = native.setup_metric(q, gq)
+    uri = _state(q, rng, field=True, velocity=False)
+    s = [_speeds(replace(q, icha=i), mq, uri) for i in (0, 1, 2)]
+    roh = uri[0] + (q.gam / (q.gam - 1.0)) * uri[4]
+    bsq = (uri[6] ** 2 + uri[7] ** 2 + uri[8] ** 2)
+    va = bsq / (roh + bsq)
+    om = np.sqrt(va + q.gam * uri[4] / roh * (1.0 - va))
+    n = q.nm0
+    om = om[n - 1:q.imax - n, n - 1:q.jmax - n, n - 1:q.kmax - n]
+    for x in s:
+        assert np.allclose(x[0::2], np.minimum(om, 1.0)[None], rtol=1e-14, atol=0)
+        assert np.allclose(x[1::2], -np.minimum(om, 1.0)[None], rtol=1e-14, atol=0)
