@@ -28,7 +28,7 @@ FORTRAN_MAIN = os.path.join(HERE, "fortran", "raishin_main")
 
 UNITS = [("rh_kernels.hip", "hip"), ("rh_model.cpp", "hip"), ("rh_capi.cpp", "hip"), ("rh_ref.cpp", "hip"),
          ("rh_batch.cpp", "hip"),
-         ("rh_setup.cpp", "host")]
+         ("rh_setup.cpp", "host"), ("rh_io.cpp", "host")]
 
 
 def _run(cmd):

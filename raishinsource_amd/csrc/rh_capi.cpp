@@ -267,6 +267,10 @@ void* raishin_model_stream(raishin_model* m) { return (void*)m->st; }
 
 long long raishin_model_device_bytes(raishin_model* m) { return m ? (long long)m->dev_bytes : 0; }
 
+void raishin_model_get_pram(raishin_model* m, raishin_pram* p) {
+  if (m && p) *p = m->p;
+}
+
 void raishin_model_allow_collapse(raishin_model* m, int allow) {
   if (m) m->allow_collapse = allow != 0;
 }

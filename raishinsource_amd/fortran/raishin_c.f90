@@ -1,5 +1,5 @@
 ! iso_c_binding interfaces of libraishin_hip.so (include/raishin_hip.h,
-! include/raishin_ref.h, include/raishin_setup.h).  Integers and arrays are
+! include/raishin_ref.h, include/raishin_setup.h, include/raishin_io.h).  Integers and arrays are
 ! passed by address, as Fortran passes them, so the reference-named wrappers
 ! in raishin_wrappers.f90 forward their dummy arguments unchanged.
 module raishin_c
@@ -58,6 +58,28 @@ module raishin_c
       real(c_double) :: a(*)
     end subroutine
     subroutine raishin_sync() bind(C, name="raishin_sync")
+    end subroutine
+    ! ---- restart and analysis files (include/raishin_io.h); path is NUL-terminated
+    subroutine raishin_store1(path, uu, u0, uri, uri0, it0, ih0, time, nd, npe, myrank, &
+                              is1, ie1, js1, je1, ks1, ke1) bind(C, name="raishin_store1")
+      import :: c_char, c_double, c_int
+      character(kind=c_char) :: path(*)
+      real(c_double) :: uu(*), u0(*), uri(*), uri0(*), time
+      integer(c_int) :: it0, ih0, nd, npe, myrank, is1, ie1, js1, je1, ks1, ke1
+    end subroutine
+    subroutine raishin_restar2(path, uu, u0, uri, uri0, it0, ih0, time, nd, npe, myrank, &
+                               is1, ie1, js1, je1, ks1, ke1) bind(C, name="raishin_restar2")
+      import :: c_char, c_double, c_int
+      character(kind=c_char) :: path(*)
+      real(c_double) :: uu(*), u0(*), uri(*), uri0(*), time
+      integer(c_int) :: it0, ih0, nd, npe, myrank, is1, ie1, js1, je1, ks1, ke1
+    end subroutine
+    subroutine raishin_output1(path, uri, detg, x1, x2, x3, time, nm1, nd, npe, myrank, myranki, myrankj, &
+                               myrankk, is1, ie1, js1, je1, ks1, ke1) bind(C, name="raishin_output1")
+      import :: c_char, c_double, c_int
+      character(kind=c_char) :: path(*)
+      real(c_double) :: uri(*), detg(*), x1(*), x2(*), x3(*), time
+      integer(c_int) :: nm1, nd, npe, myrank, myranki, myrankj, myrankk, is1, ie1, js1, je1, ks1, ke1
     end subroutine
     subroutine raishin_reset_counters() bind(C, name="raishin_reset_counters")
     end subroutine

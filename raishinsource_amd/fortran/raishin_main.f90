@@ -3,16 +3,20 @@
 ! the device library.  Set-up uses the library's host routines (grid, Kerr-
 ! Schild metric, Christoffel symbols, magnetised torus).
 !
-! usage: raishin_main imax jmax kmax irec irkt cfl nsteps out.bin [noise seed]
+! usage: raishin_main imax jmax kmax irec irkt cfl nsteps out.bin [noise seed [files]]
 ! out.bin (stream, native endian): int32 imax,jmax,kmax,nsteps; real64
 ! dt(nsteps), time; uri(9,imax,jmax,kmax); uu(9,imax,jmax,kmax);
 ! int64 counters(11).
+! files (bits): 2 = start from restart000.outdat through restar2 (icres=1,
+! main.f90:560-571) instead of the torus; 1 = after the last step write
+! restart000.outdat (store1) and structr0000-0001.outdat (output1), as the
+! output branch of main.f90:642-716 does.
 program raishin_main
   use iso_c_binding
   use raishin_c
   implicit none
   integer, parameter :: nv = 9
-  integer :: imax, jmax, kmax, irec, irkt, nsteps, nm0, nm1, it, i, j, k
+  integer :: imax, jmax, kmax, irec, irkt, nsteps, nm0, nm1, it, i, j, k, files, it0, ihi, nd, npe, myrank
   integer :: is1, ie1, js1, je1, ks1, ke1, myranki, myrankj, myrankk, icputable(1)
   real(8) :: cfl, noise, dtcfl, dtcflg, dt1, dt2, dt3, dt, time
   integer(c_long_long) :: seed
@@ -28,7 +32,7 @@ program raishin_main
   real(8), allocatable :: gcovj(:,:,:,:,:), gconj(:,:,:,:,:), gcovk(:,:,:,:,:), gconk(:,:,:,:,:)
   real(8), allocatable :: detg(:,:,:), detgi(:,:,:), detgj(:,:,:), detgk(:,:,:), christ(:,:,:,:,:,:)
   real(8), allocatable :: uri(:,:,:,:), urio(:,:,:,:), uu(:,:,:,:), uo(:,:,:,:), uh(:,:,:,:)
-  real(8), allocatable :: us(:,:,:,:), un(:,:,:,:), ww(:,:,:,:,:), wwo(:)
+  real(8), allocatable :: us(:,:,:,:), un(:,:,:,:), ww(:,:,:,:,:), wwo(:), u0(:,:,:,:), uri0(:,:,:,:)
   ! face states, speeds, tenr and sf live inside the library: never written
   real(8) :: face(1), cmx(1), tenr(1), sf(1)
   real(8), allocatable :: dts(:)
@@ -48,14 +52,21 @@ program raishin_main
   call get_command_argument(8, outfile)
   noise = 0.02d0
   seed = 0
+  files = 0
   if (command_argument_count() >= 10) then
     call get_command_argument(9, arg); read(arg, *) noise
     call get_command_argument(10, arg); read(arg, *) seed
   end if
+  if (command_argument_count() >= 11) then
+    call get_command_argument(11, arg); read(arg, *) files
+  end if
+  npe = 1; myrank = 0; nd = 1; it0 = 1; ihi = 0
 
   ! ghost widths, main.f90:179-211 (flux-CT adds one)
   if (irec == 5) then
     nm0 = 4
+  else if (irec <= 3) then
+    nm0 = 2
   else
     nm0 = 3
   end if
@@ -92,7 +103,7 @@ program raishin_main
   allocate(christ(4,4,4,imax,jmax,kmax))
   allocate(uri(nv,imax,jmax,kmax), urio(nv,imax,jmax,kmax), uu(nv,imax,jmax,kmax), uo(nv,imax,jmax,kmax))
   allocate(uh(nv,imax,jmax,kmax), us(nv,imax,jmax,kmax), un(nv,imax,jmax,kmax), ww(3,nv,imax,jmax,kmax), wwo(1))
-  allocate(dts(nsteps))
+  allocate(dts(nsteps), u0(nv,imax,jmax,kmax), uri0(nv,imax,jmax,kmax))
   x1ab = 0.d0; x2ab = 0.d0; x3ab = 0.d0
   ww = 0.d0                                       ! main.f90:372-381
   uh = 0.d0; us = 0.d0; un = 0.d0; uo = 0.d0; urio = 0.d0
@@ -109,7 +120,14 @@ program raishin_main
   call raishin_check('raishin_geometry')
 
   call caluu1(uri, uu, gcov, gcon, detg, x1, is1, ie1, js1, je1, ks1, ke1)     ! main.f90:530
+  call ident(uu, u0, is1, ie1, js1, je1, ks1, ke1)                             ! main.f90:535-538
+  call ident(uri, uri0, is1, ie1, js1, je1, ks1, ke1)
   time = 0.d0
+  if (files /= 0) open(9, file='restart000.outdat', status='unknown', form='unformatted')  ! main.f90:339-343
+  if (iand(files, 2) /= 0) then
+    call restar2(uu, u0, uri, uri0, it0, ihi, time, nd, npe, myrank, is1, ie1, js1, je1, ks1, ke1)
+    call raishin_check('restar2')
+  end if
 
   do it = 1, nsteps
     call ident(uu, uo, is1, ie1, js1, je1, ks1, ke1)
@@ -206,6 +224,14 @@ program raishin_main
     time = time + dt
   end do
 
+  if (iand(files, 1) /= 0) then
+    ihi = ihi + nsteps
+    call store1(uu, u0, uri, uri0, it0 + nsteps, ihi, time, nd, npe, myrank, is1, ie1, js1, je1, ks1, ke1)
+    call output1(uri, detg, x1, x2, x3, time, nm1, nd, npe, myrank, myranki, myrankj, myrankk, &
+                 is1, ie1, js1, je1, ks1, ke1)
+    call raishin_check('store1/output1')
+  end if
+  if (files /= 0) close(9)
   call raishin_pull(uri)
   call raishin_pull(uu)
   call raishin_get_counters(cnt)

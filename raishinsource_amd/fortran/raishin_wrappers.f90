@@ -185,3 +185,65 @@ subroutine mpiex(uri, nm1, is1, ie1, js1, je1, ks1, ke1, myranki, myrankj, myran
   real(8) :: uri(*)
   integer :: nm1, is1, ie1, js1, je1, ks1, ke1, myranki, myrankj, myrankk, icputable(*)
 end subroutine mpiex
+
+! ---- restart and analysis files (restar.f90) ------------------------------
+! main.f90 keeps unit 9 open on restartRRR.outdat (main.f90:339-343); the
+! wrappers write or read that file through the library and leave the unit
+! open on it again.  The state arrays are pulled from the device first
+! (store1, output1) or pushed to it after reading (restar2).
+
+subroutine unit9_name(myrank, fname, was_open)
+  integer :: myrank
+  character(len=*) :: fname
+  logical :: was_open
+  inquire(unit=9, opened=was_open, name=fname)
+  if (was_open) then
+    close(9)
+  else
+    write(fname, '(a, i3.3, a)') 'restart', myrank, '.outdat'
+  end if
+end subroutine unit9_name
+
+subroutine store1(uu, u0, uri, uri0, it0, ih0, time, nd, npe, myrank, is1, ie1, js1, je1, ks1, ke1)
+  use raishin_c                                                  ! restar.f90:5
+  real(8) :: uu(*), u0(*), uri(*), uri0(*), time
+  integer :: it0, ih0, nd, npe, myrank, is1, ie1, js1, je1, ks1, ke1
+  character(len=1024) :: fname
+  logical :: was_open
+  call raishin_pull(uu)
+  call raishin_pull(u0)
+  call raishin_pull(uri)
+  call raishin_pull(uri0)
+  call unit9_name(myrank, fname, was_open)
+  call raishin_store1(trim(fname) // c_null_char, uu, u0, uri, uri0, it0, ih0, time, nd, npe, myrank, &
+                      is1, ie1, js1, je1, ks1, ke1)
+  if (was_open) open(9, file=trim(fname), form='unformatted', status='old')
+end subroutine store1
+
+subroutine restar2(uu, u0, uri, uri0, it0, ih0, time, nd, npe, myrank, is1, ie1, js1, je1, ks1, ke1)
+  use raishin_c                                                  ! restar.f90:273
+  real(8) :: uu(*), u0(*), uri(*), uri0(*), time
+  integer :: it0, ih0, nd, npe, myrank, is1, ie1, js1, je1, ks1, ke1
+  character(len=1024) :: fname
+  logical :: was_open
+  call unit9_name(myrank, fname, was_open)
+  call raishin_restar2(trim(fname) // c_null_char, uu, u0, uri, uri0, it0, ih0, time, nd, npe, myrank, &
+                       is1, ie1, js1, je1, ks1, ke1)
+  call raishin_push(uu)
+  call raishin_push(u0)
+  call raishin_push(uri)
+  call raishin_push(uri0)
+  if (was_open) open(9, file=trim(fname), form='unformatted', status='old')
+end subroutine restar2
+
+subroutine output1(uri, detg, x1, x2, x3, time, nm1, nd, npe, myrank, myranki, myrankj, myrankk, &
+                   is1, ie1, js1, je1, ks1, ke1)
+  use raishin_c                                                  ! restar.f90:91
+  real(8) :: uri(*), detg(*), x1(*), x2(*), x3(*), time
+  integer :: nm1, nd, npe, myrank, myranki, myrankj, myrankk, is1, ie1, js1, je1, ks1, ke1
+  character(len=64) :: fname
+  call raishin_pull(uri)
+  write(fname, '(a, i4.4, a, i4.4, a)') 'structr', myrank, '-', nd, '.outdat'
+  call raishin_output1(trim(fname) // c_null_char, uri, detg, x1, x2, x3, time, nm1, nd, npe, myrank, &
+                       myranki, myrankj, myrankk, is1, ie1, js1, je1, ks1, ke1)
+end subroutine output1

@@ -59,6 +59,8 @@ EXPORTS = (
     "raishin_batch_set", "raishin_batch_step", "raishin_batch_step_graph", "raishin_batch_times",
     "raishin_batch_sync", "raishin_mdshoctub", "raishin_model_set_block_sizes", "raishin_model_get_block_sizes",
     "raishin_model_autotune", "raishin_model_allow_collapse", "raishin_model_stored_rows",
+    "raishin_store1", "raishin_restar2", "raishin_output1", "raishin_model_store1", "raishin_model_restar2",
+    "raishin_model_output1", "raishin_model_get_pram",
 )
 
 TUNE_SLOTS = ("ppm_edges", "sweep_i", "sweep_j", "sweep_k", "fluxct", "update", "recov", "post")
@@ -138,6 +140,17 @@ def lib():
         for f in ("raishin_batch_set", "raishin_batch_step", "raishin_batch_step_graph", "raishin_batch_times",
                   "raishin_batch_sync"):
             getattr(L, f).restype = C.c_int
+        ip = C.POINTER(C.c_int)
+        dp = C.POINTER(C.c_double)
+        L.raishin_store1.argtypes = [C.c_char_p] + [vp] * 4 + [ip, ip, dp, ip, ip, ip] + [ip] * 6
+        L.raishin_restar2.argtypes = [C.c_char_p] + [vp] * 4 + [ip, ip, dp, ip, ip, ip] + [ip] * 6
+        L.raishin_output1.argtypes = [C.c_char_p] + [vp] * 5 + [dp] + [ip] * 12
+        L.raishin_model_store1.argtypes = [vp, C.c_char_p, vp, vp, C.c_int, C.c_int, C.c_int, C.c_int]
+        L.raishin_model_restar2.argtypes = [vp, C.c_char_p, vp, vp, ip, ip, ip, C.c_int]
+        L.raishin_model_output1.argtypes = [vp, C.c_char_p, vp, vp, vp, vp, C.c_int, C.c_int, C.c_int]
+        for f in ("raishin_model_store1", "raishin_model_restar2", "raishin_model_output1"):
+            getattr(L, f).restype = C.c_int
+        L.raishin_model_get_pram.argtypes = [vp, C.POINTER(PramS)]
         _lib = L
     return _lib
 
@@ -171,6 +184,46 @@ def _p(a):
 
 def fzeros(*shape):
     return np.zeros(shape, dtype=np.float64, order="F")
+
+
+def _ints(*v):
+    return [C.byref(C.c_int(int(x))) for x in v]
+
+
+def _path(path):
+    return None if path is None else str(path).encode()
+
+
+def store1(path, uu, u0, uri, uri0, it0, ih0, time, nd, myrank=0, npe=1):
+    """Restart file of host arrays (restar.f90:5-88 layout)."""
+    ni, nj, nk = uu.shape[1:]
+    arr = [np.asfortranarray(a) for a in (uu, u0, uri, uri0)]
+    lib().raishin_store1(_path(path), *[_p(a) for a in arr], *_ints(it0, ih0), C.byref(C.c_double(time)),
+                         *_ints(nd, npe, myrank, 1, ni, 1, nj, 1, nk))
+    check()
+
+
+def restar2(path, shape, myrank=0, npe=1):
+    """Read a restart file (restar.f90:273-346): dict of uu, u0, uri, uri0,
+    it0, ih0, time, nd."""
+    ni, nj, nk = shape
+    arr = {k: fzeros(9, ni, nj, nk) for k in ("uu", "u0", "uri", "uri0")}
+    it0, ih0, nd = C.c_int(0), C.c_int(0), C.c_int(0)
+    t = C.c_double(0)
+    lib().raishin_restar2(_path(path), _p(arr["uu"]), _p(arr["u0"]), _p(arr["uri"]), _p(arr["uri0"]), C.byref(it0),
+                          C.byref(ih0), C.byref(t), C.byref(nd), *_ints(npe, myrank, 1, ni, 1, nj, 1, nk))
+    check()
+    arr.update(it0=it0.value, ih0=ih0.value, nd=nd.value, time=t.value)
+    return arr
+
+
+def output1(path, uri, detg, x1, x2, x3, time, nd, myrank=0, npe=1, nm1=5):
+    """Analysis file of host arrays (restar.f90:91-185 layout)."""
+    ni, nj, nk = uri.shape[1:]
+    lib().raishin_output1(_path(path), _p(np.asfortranarray(uri)), _p(np.asfortranarray(detg)),
+                          *[_p(np.ascontiguousarray(x, dtype=np.float64)) for x in (x1, x2, x3)],
+                          C.byref(C.c_double(time)), *_ints(nm1, nd, npe, myrank, 0, 0, 0, 1, ni, 1, nj, 1, nk))
+    check()
 
 
 def setup_grid(p: Pram):
@@ -440,6 +493,29 @@ class Model:
 
     def device_bytes(self):
         return int(lib().raishin_model_device_bytes(self.h))
+
+    def store1(self, path=None, u0=None, uri0=None, it0=0, ih0=0, nd=0, myrank=0):
+        """Restart file of the device-resident state (restar.f90:5-88 layout)."""
+        a = [None if x is None else np.asfortranarray(x) for x in (u0, uri0)]
+        if lib().raishin_model_store1(self.h, _path(path), _p(a[0]), _p(a[1]), it0, ih0, nd, myrank):
+            check()
+
+    def restar2(self, path=None, myrank=0):
+        """Load a restart file into the model; returns (u0, uri0, it0, ih0, nd)."""
+        p = self.p
+        u0, uri0 = fzeros(9, p.imax, p.jmax, p.kmax), fzeros(9, p.imax, p.jmax, p.kmax)
+        it0, ih0, nd = C.c_int(0), C.c_int(0), C.c_int(0)
+        if lib().raishin_model_restar2(self.h, _path(path), _p(u0), _p(uri0), C.byref(it0), C.byref(ih0),
+                                       C.byref(nd), myrank):
+            check()
+        return u0, uri0, it0.value, ih0.value, nd.value
+
+    def output1(self, detg, x1, x2, x3, path=None, nd=0, myrank=0, npe=1):
+        """Analysis file of the device-resident state (restar.f90:91-185 layout)."""
+        xs = [np.ascontiguousarray(x, dtype=np.float64) for x in (x1, x2, x3)]
+        if lib().raishin_model_output1(self.h, _path(path), _p(np.asfortranarray(detg)), *[_p(x) for x in xs], nd,
+                                       myrank, npe):
+            check()
 
     @property
     def stream(self):

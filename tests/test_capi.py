@@ -18,7 +18,7 @@ def declared(header):
     return sorted(set(names))
 
 
-@pytest.mark.parametrize("header", ["raishin_hip.h", "raishin_setup.h", "raishin_ref.h"])
+@pytest.mark.parametrize("header", ["raishin_hip.h", "raishin_setup.h", "raishin_ref.h", "raishin_io.h"])
 def test_library_exports_header(header):
     lib = C.CDLL(native.LIB_PATH)
     names = declared(header)
@@ -28,7 +28,7 @@ def test_library_exports_header(header):
 
 
 def test_python_binding_covers_exports():
-    names = set(declared("raishin_hip.h")) | set(declared("raishin_setup.h"))
+    names = set(declared("raishin_hip.h")) | set(declared("raishin_setup.h")) | set(declared("raishin_io.h"))
     assert names <= set(native.EXPORTS), names - set(native.EXPORTS)
     assert set(declared("raishin_ref.h")) == set(native.REF_EXPORTS)
 
@@ -46,6 +46,9 @@ def test_fortran_interfaces_bind_every_reference_entry_point():
     routines = set(native.REF_EXPORTS) - {"raishin_init", "raishin_geometry", "raishin_finalize", "raishin_push",
                                           "raishin_pull", "raishin_sync", "raishin_get_counters",
                                           "raishin_reset_counters"}
+    io = {"raishin_store1", "raishin_restar2", "raishin_output1"}
+    assert io <= bound and io <= called, io - (bound & called)
+    called -= io | {"raishin_pull", "raishin_push"}
     assert routines == called, routines ^ called
 
 
