@@ -1084,6 +1084,174 @@ void orc_recov1a(double *uu, double *uri, const double *gcov, const double *gcon
 }
 
 /* ------------------------------------------------------------------ */
+/* recov2: recov.f90:1919-2454 (Mignone & McKinney 2007: one Newton-     */
+/* Raphson unknown W' = W - D, gamma-law EoS)                            */
+/* ------------------------------------------------------------------ */
+void orc_recov2(double *uu, double *uri, const double *gcov, const double *gcon,
+                const double *detg, const double *x1, int nm1) {
+  const double safty = 1.0e-5, dv = 1.0e-4;  /* recov.f90:1962-1964 */
+  const double gam = P.gam;
+  const double gam1 = gam / (gam - 1.0);
+  const double gam2 = 1.0 / gam1;
+  const int iter = P.iter;
+  double wg = 0.0; /* recov.f90:1968: carried from cell to cell when no root is admissible */
+  for (int k = 1 + nm1; k <= NK - nm1; k++)
+    for (int j = 1 + nm1; j <= NJ - nm1; j++)
+      for (int i = 1 + nm1; i <= NI - nm1; i++) {
+        double delta = 1.0;
+        const double deltaend = 1.0e-8;
+        int irecov = 0;
+        const double *g = G16(gcov, i, j, k), *gi = G16(gcon, i, j, k);
+        double alpha1 = 1.0 / sqrt(-GM(gi, 0, 0));
+        double detg1 = S3(detg, i, j, k);
+        double *u = &V(uu, 1, i, j, k), *w = &V(uri, 1, i, j, k);
+        double dro = alpha1 * u[0] / detg1;
+        double qcov[4], qcon[4], bcon[4], bcov[4], oncov[4], oncon[4];
+        qcov[0] = alpha1 * (u[4] - u[0]) * (1.0 / detg1);
+        qcov[1] = alpha1 * u[1] * (1.0 / detg1);
+        qcov[2] = alpha1 * u[2] * (1.0 / detg1);
+        qcov[3] = alpha1 * u[3] * (1.0 / detg1);
+        bcon[0] = 0.0;
+        bcon[1] = alpha1 * u[6] * (1.0 / detg1);
+        bcon[2] = alpha1 * u[7] * (1.0 / detg1);
+        bcon[3] = alpha1 * u[8] * (1.0 / detg1);
+        double util[4], ro = 0.0, pp = 0.0;
+        util[1] = w[1]; util[2] = w[2]; util[3] = w[3];
+        const double ro1 = w[0], pp1 = w[4];
+        const double util1[4] = {0.0, w[1], w[2], w[3]};
+        lower(bcon, bcov, g);
+        upper(qcov, qcon, gi);
+        double bsq = bcon[1] * bcov[1] + bcon[2] * bcov[2] + bcon[3] * bcov[3];
+        double qdotb = qcov[0] * bcon[0] + qcov[1] * bcon[1] + qcov[2] * bcon[2] + qcov[3] * bcon[3];
+        double qdotbsq = qdotb * qdotb;
+        oncov[0] = -1.0 * alpha1; oncov[1] = 0.0; oncov[2] = 0.0; oncov[3] = 0.0;
+        upper(oncov, oncon, gi);
+        double qdotn = qcon[0] * oncov[0];
+        double qsq = qcov[0] * qcon[0] + qcov[1] * qcon[1] + qcov[2] * qcon[2] + qcov[3] * qcon[3];
+        double qtsq = qsq + (qdotn * qdotn);
+        double utsq = 0.0;
+        for (int m = 1; m <= 3; m++)
+          for (int n = 1; n <= 3; n++) utsq = utsq + GM(g, m, n) * util[m] * util[n];
+        if (utsq < 0.0 && fabs(utsq) <= safty) utsq = fabs(utsq);
+        if (utsq < 0.0 || fabs(utsq) > 1.0e4) utsq = 1.0e4;
+        double gfsq = 1.0 + utsq;
+        double gf = sqrt(gfsq);
+        double vsq = (gfsq - 1.0) * (1.0 / gfsq);
+        /* initial W: the larger admissible root of the quadratic (recov.f90:2125-2146) */
+        double t1a = 4.0 - vsq;
+        double t1b = 4.0 * (bsq + qdotn);
+        double t1c = qtsq + bsq * bsq + 2.0 * bsq * qdotn;
+        double t1d = (t1b * t1b) - 4.0 * t1a * t1c;
+        if (t1d <= 0.0) t1d = fabs(t1d);
+        double wgp = (-t1b + sqrt(t1d)) * (1.0 / (2.0 * t1a));
+        double wgm = (-t1b - sqrt(t1d)) * (1.0 / (2.0 * t1a));
+        if (wgp > 0.0) wg = wgp;
+        else if (wgm >= 0.0) wg = wgm;
+        else {
+          CNT.recov_wg_carry++;
+          if (DBG_CARRY && DBG_N < DBG_CAP) DBG_CARRY[DBG_N] = (long long)CELL(i, j, k);
+          DBG_N++;
+        }
+        /* recov.f90:2150-2172: "goto 95" adjustment loops */
+        int guard = 0;
+        for (; guard < 100000; guard++) {
+          pp = gam2 * (wg * gf - dro * sqrt(gf));
+          double t3a = (wg * wg * wg) * (wg + 2.0 * bsq) - qdotbsq * (2.0 * wg + bsq);
+          double t3b = (wg * wg) * (qtsq - bsq * bsq);
+          double t3c = (wg * wg * wg) + (wg * wg) * (0.5 * bsq + qdotn - pp) - 0.5 * qdotbsq;
+          if (t3a <= t3b) { wg = fabs(1.5 * wg); continue; }
+          if (t3c > 0.0) { wg = fabs((double)0.9f * wg); continue; }
+          break;
+        }
+        if (guard >= 100000) CNT.guard++;
+        double wgd = wg - dro;
+        double xxn = wgd;
+        /* Newton-Raphson on W' (recov.f90:2180-2300) */
+        for (int nnn = 1; nnn <= iter; nnn++) {
+          if (!(fabs(delta) >= deltaend && irecov == 0)) break; /* remaining trips do nothing */
+          if (nnn == iter) { irecov = 1; break; }
+          wgd = xxn;
+          wg = wgd + dro;
+          double wsq = wg * wg;
+          double w3 = wsq * wg;
+          double xsq = (bsq + wg) * (bsq + wg);
+          vsq = fabs((wsq * qtsq + qdotbsq * (bsq + 2.0 * wg)) * (1.0 / (wsq * xsq)));
+          if (fabs(vsq) > 1.0) { CNT.recov_vsq_gt1++; vsq = 1.0 - dv; }
+          if (vsq < 0.0) vsq = fabs(vsq);
+          double gtmp = 1.0 - vsq;
+          if (gtmp < 0.0 || gtmp < safty) { gtmp = safty; vsq = 1.0 - gtmp; }
+          double usq = (1.0 - gtmp) / (gtmp);
+          gf = sqrt(1.0 + usq);
+          gfsq = gf * gf;
+          ro = dro / gf;
+          double xi = (wg - dro * gf) * (1.0 / gfsq);
+          pp = gam2 * xi;
+          const double dpdxi = gam2, dpdro = 0.0;
+          double t1a_ = (bsq + wg);
+          double t1b_ = (bsq + wg) * (bsq + wg) * (bsq + wg);
+          double t1c_ = (qtsq == 0.0) ? 0.0 : -2.0 * qtsq * (1.0 / t1b_);
+          double t1d_ = (qdotbsq == 0.0) ? 0.0 : -2.0 * qdotbsq * (3.0 * wg * t1a_ + bsq * bsq) * (1.0 / (t1b_ * w3));
+          double dvdwd = t1c_ + t1d_;
+          double dxidwd = (1.0 / gfsq) - 0.5 * gf * (dro + 2.0 * gf * xi) * dvdwd;
+          double drodwd = -0.5 * dro * gf * dvdwd;
+          double dpdwd = dpdxi * dxidwd + dpdro * drodwd;
+          double t5a = bsq * qtsq - qdotbsq;
+          double t5b = bsq + wgd + dro;
+          double t5c = 0.0, t5d = 0.0;
+          if (t5a != 0.0) {
+            t5c = 0.5 * t5a * (1.0 / (t5b * t5b));
+            t5d = t5a * (1.0 / (t5b * t5b * t5b));
+          }
+          double f = -(qdotn + dro) - wgd + pp - 0.5 * bsq - t5c;
+          double df = -1.0 + dpdwd + t5d;
+          delta = f / df;
+          xxn = wgd - delta;
+          if (xxn < 0.0) xxn = fabs(xxn);
+          if (xxn > 1.0e8) xxn = 1.0e8;
+        }
+        if (irecov == 0) {
+          /* recov.f90:2305-2370 */
+          wgd = xxn;
+          wg = wgd + dro;
+          double wsq = wg * wg;
+          double xsq = (bsq + wg) * (bsq + wg);
+          vsq = fabs((wsq * qtsq + qdotbsq * (bsq + 2.0 * wg)) * (1.0 / (wsq * xsq)));
+          if (vsq < 0.0) vsq = fabs(vsq);
+          if (vsq > 1.0) { CNT.recov_vsq_gt1++; vsq = 1.0 - dv; }
+          double gtmp = 1.0 - vsq;
+          if (gtmp < safty) gtmp = safty;
+          double usq = (1.0 - gtmp) / gtmp;
+          gf = sqrt(1.0 + usq);
+          gfsq = gf * gf;
+          ro = dro / gf;
+          double xi = wg * (1.0 / gfsq) - ro;
+          pp = gam2 * xi;
+          if (ro < 0.0) { CNT.recov_neg_rho++; ro = P.dmin * pow(x1[i - 1], -1.5); }
+          if (pp < 0.0) { CNT.recov_neg_p++; pp = P.pmin * (gam - 1.0) * pow(x1[i - 1], -2.5); }
+          for (int m = 1; m <= 3; m++) {
+            double qt = qcon[m] + oncon[m] * qdotn;
+            util[m] = (gf * (1.0 / (wg + bsq))) * (qt + qdotb * bcon[m] * (1.0 / wg));
+          }
+          w[0] = ro; w[1] = util[1]; w[2] = util[2]; w[3] = util[3]; w[4] = pp;
+          w[5] = pp / pow(ro, gam - 1.0);
+        } else {
+          CNT.recov_fail++;
+          w[0] = ro1; w[1] = util1[1]; w[2] = util1[2]; w[3] = util1[3]; w[4] = pp1;
+          w[5] = pp1 / pow(ro1, gam - 1.0);
+        }
+        w[6] = u[6] * (1.0 / detg1); w[7] = u[7] * (1.0 / detg1); w[8] = u[8] * (1.0 / detg1);
+      }
+}
+
+/* recov: recov.f90:2-47, the inversion iwvec selects */
+void orc_recov(double *uu, double *uri, const double *gcov, const double *gcon,
+               const double *detg, const double *x1, int nm1) {
+  if (P.iwvec == 1) orc_recov1a(uu, uri, gcov, gcon, detg, x1, nm1);
+  else if (P.iwvec == 2) orc_recov2(uu, uri, gcov, gcon, detg, x1, nm1);
+  else abort();
+}
+
+/* ------------------------------------------------------------------ */
 /* bnd4 = bndpefb4 + bndryfb4: bnd.f90:4-1059, single-rank branches     */
 /* ------------------------------------------------------------------ */
 static void bnd_side(double *uu, const double *uo, int m, int dir, int outer, int type, int nm1) {
@@ -1425,7 +1593,7 @@ static void stage(orc_state *s, int st, double dt, const double *a1, const doubl
     orc_rk3trd(out, W_ww, W_uo, W_us, nm1, a1, a2, a3);
     orc_rkadsf(out, W_sf, dt, nm1, 3);
   }
-  orc_recov1a(out, s->uri, s->gcov, s->gcon, s->detg, s->x1, nm1);
+  orc_recov(out, s->uri, s->gcov, s->gcon, s->detg, s->x1, nm1);
   orc_bnd4(s->uri, W_urio, nm1);
   if (P.model >= 7 && P.model <= 20) orc_infchk(s->uri, s->gcov, s->gcon, nm1);
   if (P.model >= 7 && P.model <= 15) orc_atmosphere1a(s->uri, s->gcov, s->gcon, s->x1);

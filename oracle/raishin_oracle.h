@@ -89,6 +89,10 @@ void orc_rk3trd(double *uu, const double *ww, const double *uo, const double *us
 void orc_rkadsf(double *uu, const double *sf, double dt, int nm1, int which);
 void orc_recov1a(double *uu, double *uri, const double *gcov, const double *gcon,
                  const double *detg, const double *x1, int nm1);
+void orc_recov2(double *uu, double *uri, const double *gcov, const double *gcon,
+                 const double *detg, const double *x1, int nm1);
+void orc_recov(double *uu, double *uri, const double *gcov, const double *gcon,
+                 const double *detg, const double *x1, int nm1);
 void orc_bnd4(double *uri, const double *uo, int nm1);
 void orc_infchk(double *uri, const double *gcov, const double *gcon, int nm1);
 void orc_atmosphere1a(double *uri, const double *gcov, const double *gcon, const double *x1);

@@ -26,7 +26,7 @@ class Pram:
     ict: int = 1
     ihll: int = 1
     icha: int = 2          # 0 Gammie, 1 Leismann, 2 Del Zanna wave speeds
-    iwvec: int = 1
+    iwvec: int = 1          # 1 recov1a, 2 recov2 (recov.f90:2-47)
     ieos: int = 0
     model: int = 7
     metric: int = 303
@@ -126,10 +126,14 @@ T2D_MC = Pram("t2dmc", 64, 7, 64, irec=1, irkt=2, cfl=0.4)
 # with MC and RK3 as pram_3Dtorus.f90:66-70 pairs them) and Gammie et al. (icha=0)
 U_LEIS = Pram("uleis", 16, 10, 18, irec=1, irkt=3, cfl=0.3, icha=1)
 T2D_GAM = Pram("t2dgam", 64, 11, 64, irec=5, irkt=2, cfl=0.4, icha=0)
+# recov2 (iwvec=2, recov.f90:1919-2454): alone, and with the whole scheme set
+# of pram_3Dtorus.f90:66-72 (MC, Leismann speeds, RK3)
+T2D_RC2 = Pram("t2drc2", 64, 11, 64, irec=5, irkt=2, cfl=0.4, iwvec=2)
+U_RC2 = Pram("urc2", 16, 10, 18, irec=1, irkt=3, cfl=0.3, icha=1, iwvec=2)
 
 # the serving mix's tube (1024 zones) and a small one for the oracle
 TUBE = tube("tube", 1024)
 U_TUBE = tube("utube", 64)
 
 ALL = {p.name: p for p in (C1, C2, C2_PPM, C2_WENO, U_PPM, U_MP5, U_WENO, T2D, T3D, G_PPM, G_MP5, G_WENO, TUBE,
-                           U_TUBE, U_MC, T2D_MC, U_LEIS, T2D_GAM)}
+                           U_TUBE, U_MC, T2D_MC, U_LEIS, T2D_GAM, T2D_RC2, U_RC2)}

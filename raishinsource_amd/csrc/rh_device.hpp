@@ -37,6 +37,7 @@ struct Geo {
   // stored and computed, and every j maps to it.
   int njs, jfix;
   int icha;              // wave-speed formula (calcha.f90): 0 Gammie, 1 Leismann, 2 Del Zanna
+  int iwvec;             // primitive recovery (recov.f90:2-47): 1 recov1a, 2 recov2
   // workgroup size per tuned launch slot (TuneSlot in rh_kernels.hpp);
   // 0 means 256.  Chosen by raishin_model_autotune or set by the caller.
   unsigned short bs[8];

@@ -687,6 +687,188 @@ RH_DEV void recov_cell(const double u[9], const double wold[9], const Met& M, do
 }
 
 // ------------------------------------------------------------------------
+// Recovery of primitives, recov2 (recov.f90:1919-2454: Mignone & McKinney
+// 2007, one Newton-Raphson unknown W' = W - D, gamma-law EoS), one interior
+// cell; the same interface as recov_cell.  The initial W of a cell whose
+// quadratic has no admissible root is the previous cell's final W.  recov2
+// prints "vsq > 1.0" at every trip that clamps v^2; vsqgt1 counts them.
+// ------------------------------------------------------------------------
+RH_DEV void recov2_cell(const double u[9], const double wold[9], const Met& M, double gam, double dmin,
+                        double pmin, int iter, double x1pow15, double x1pow25, double wg_in, bool& carry,
+                        RecovOut& o, double& wg_out) {
+  const double safty = 1.0e-5, dv = 1.0e-4;
+  const double gam1 = gam / (gam - 1.0);
+  const double gam2 = 1.0 / gam1;
+  o.fallback = o.fail = o.negrho = o.negp = o.vsqgt1 = o.det0 = o.guard = 0;
+  double delta = 1.0;
+  const double deltaend = 1.0e-8;
+  int irecov = 0;
+  const double alpha1 = 1.0 / sqrt(-M.gi(0, 0));
+  const double detg1 = M.detg();
+  const double dro = alpha1 * u[0] / detg1;
+  double qcov[4], qcon[4], bcon[4], bcov[4], oncov[4], oncon[4];
+  qcov[0] = alpha1 * (u[4] - u[0]) * (1.0 / detg1);
+  qcov[1] = alpha1 * u[1] * (1.0 / detg1);
+  qcov[2] = alpha1 * u[2] * (1.0 / detg1);
+  qcov[3] = alpha1 * u[3] * (1.0 / detg1);
+  bcon[0] = 0.0;
+  bcon[1] = alpha1 * u[6] * (1.0 / detg1);
+  bcon[2] = alpha1 * u[7] * (1.0 / detg1);
+  bcon[3] = alpha1 * u[8] * (1.0 / detg1);
+  double util[4] = {0.0, wold[1], wold[2], wold[3]};
+  const double ro1 = wold[0], pp1 = wold[4];
+  lower(bcon, bcov, M);
+  upper(qcov, qcon, M);
+  const double bsq = bcon[1] * bcov[1] + bcon[2] * bcov[2] + bcon[3] * bcov[3];
+  const double qdotb = qcov[0] * bcon[0] + qcov[1] * bcon[1] + qcov[2] * bcon[2] + qcov[3] * bcon[3];
+  const double qdotbsq = qdotb * qdotb;
+  oncov[0] = -1.0 * alpha1; oncov[1] = 0.0; oncov[2] = 0.0; oncov[3] = 0.0;
+  upper(oncov, oncon, M);
+  const double qdotn = qcon[0] * oncov[0];
+  const double qsq = qcov[0] * qcon[0] + qcov[1] * qcon[1] + qcov[2] * qcon[2] + qcov[3] * qcon[3];
+  const double qtsq = qsq + (qdotn * qdotn);
+  double utsq = 0.0;
+#pragma unroll
+  for (int m = 1; m <= 3; m++)
+#pragma unroll
+    for (int n = 1; n <= 3; n++) utsq = utsq + M.g(m, n) * util[m] * util[n];
+  if (utsq < 0.0 && fabs(utsq) <= safty) utsq = fabs(utsq);
+  if (utsq < 0.0 || fabs(utsq) > 1.0e4) utsq = 1.0e4;
+  double gfsq = 1.0 + utsq;
+  double gf = sqrt(gfsq);
+  double vsq = (gfsq - 1.0) * (1.0 / gfsq);
+  const double t1a = 4.0 - vsq;
+  const double t1b = 4.0 * (bsq + qdotn);
+  const double t1c = qtsq + bsq * bsq + 2.0 * bsq * qdotn;
+  double t1d = (t1b * t1b) - 4.0 * t1a * t1c;
+  if (t1d <= 0.0) t1d = fabs(t1d);
+  const double wgp = (-t1b + sqrt(t1d)) * (1.0 / (2.0 * t1a));
+  const double wgm = (-t1b - sqrt(t1d)) * (1.0 / (2.0 * t1a));
+  double wg = wg_in;
+  carry = false;
+  // as in recov_cell: NaN inputs make the result independent of the guess
+  o.indep = isnan(dro) || isnan(qtsq) || isnan(qdotn) || isnan(qdotbsq) || isnan(bsq);
+  if (wgp > 0.0) wg = wgp;
+  else if (wgm >= 0.0) wg = wgm;
+  else carry = true;
+  double pp = 0.0, ro = 0.0;
+  // recov.f90:2150-2172 ("goto 95"); bounded here, bound hits are counted
+  int guard = 0;
+  for (; guard < 100000; guard++) {
+    pp = gam2 * (wg * gf - dro * sqrt(gf));
+    const double t3a = (wg * wg * wg) * (wg + 2.0 * bsq) - qdotbsq * (2.0 * wg + bsq);
+    const double t3b = (wg * wg) * (qtsq - bsq * bsq);
+    const double t3c = (wg * wg * wg) + (wg * wg) * (0.5 * bsq + qdotn - pp) - 0.5 * qdotbsq;
+    if (t3a <= t3b) { wg = fabs(1.5 * wg); continue; }
+    if (t3c > 0.0) { wg = fabs((double)0.9f * wg); continue; }
+    break;
+  }
+  if (guard >= 100000) o.guard = 1;
+  double wgd = wg - dro;
+  double xxn = wgd;
+  // Newton-Raphson on W' (recov.f90:2180-2300); a trip is a function of xxn
+  // alone, so a repeated xxn while the tolerance fails is an exact cycle and
+  // whole periods are skipped (with their "vsq > 1" events), as in recov_cell.
+  unsigned long long cyx = (unsigned long long)__double_as_longlong(xxn);
+  int cy_pow = 1, cy_len = 0, cy_vsq = 0;
+  for (int nnn = 1; nnn <= iter; nnn++) {
+    if (!(fabs(delta) >= deltaend && irecov == 0)) break;
+    if (nnn == iter) { irecov = 1; break; }
+    wgd = xxn;
+    wg = wgd + dro;
+    const double wsq = wg * wg;
+    const double w3 = wsq * wg;
+    const double xsq = (bsq + wg) * (bsq + wg);
+    vsq = fabs((wsq * qtsq + qdotbsq * (bsq + 2.0 * wg)) * (1.0 / (wsq * xsq)));
+    if (fabs(vsq) > 1.0) { o.vsqgt1 += 1; vsq = 1.0 - dv; }
+    if (vsq < 0.0) vsq = fabs(vsq);
+    double gtmp = 1.0 - vsq;
+    if (gtmp < 0.0 || gtmp < safty) { gtmp = safty; vsq = 1.0 - gtmp; }
+    const double usq = (1.0 - gtmp) / (gtmp);
+    gf = sqrt(1.0 + usq);
+    gfsq = gf * gf;
+    ro = dro / gf;
+    const double xi = (wg - dro * gf) * (1.0 / gfsq);
+    pp = gam2 * xi;
+    const double dpdxi = gam2, dpdro = 0.0;
+    const double ta = (bsq + wg);
+    const double tb = (bsq + wg) * (bsq + wg) * (bsq + wg);
+    const double tc = (qtsq == 0.0) ? 0.0 : -2.0 * qtsq * (1.0 / tb);
+    const double td = (qdotbsq == 0.0) ? 0.0 : -2.0 * qdotbsq * (3.0 * wg * ta + bsq * bsq) * (1.0 / (tb * w3));
+    const double dvdwd = tc + td;
+    const double dxidwd = (1.0 / gfsq) - 0.5 * gf * (dro + 2.0 * gf * xi) * dvdwd;
+    const double drodwd = -0.5 * dro * gf * dvdwd;
+    const double dpdwd = dpdxi * dxidwd + dpdro * drodwd;
+    const double t5a = bsq * qtsq - qdotbsq;
+    const double t5b = bsq + wgd + dro;
+    double t5c = 0.0, t5d = 0.0;
+    if (t5a != 0.0) {
+      t5c = 0.5 * t5a * (1.0 / (t5b * t5b));
+      t5d = t5a * (1.0 / (t5b * t5b * t5b));
+    }
+    const double f = -(qdotn + dro) - wgd + pp - 0.5 * bsq - t5c;
+    const double df = -1.0 + dpdwd + t5d;
+    delta = f / df;
+    xxn = wgd - delta;
+    if (xxn < 0.0) xxn = fabs(xxn);
+    if (xxn > 1.0e8) xxn = 1.0e8;
+    const unsigned long long bx = (unsigned long long)__double_as_longlong(xxn);
+    cy_len++;
+    if (bx == cyx && fabs(delta) >= deltaend) {
+      const int periods = (iter - 1 - nnn) / cy_len;
+      nnn += periods * cy_len;
+      o.vsqgt1 += periods * (o.vsqgt1 - cy_vsq);
+      cy_pow = 0x40000000;
+    } else if (cy_len == cy_pow) {
+      cyx = bx; cy_pow <<= 1; cy_len = 0; cy_vsq = o.vsqgt1;
+    }
+  }
+  if (irecov == 0) {
+    wgd = xxn;
+    wg = wgd + dro;
+    const double wsq = wg * wg;
+    const double xsq = (bsq + wg) * (bsq + wg);
+    vsq = fabs((wsq * qtsq + qdotbsq * (bsq + 2.0 * wg)) * (1.0 / (wsq * xsq)));
+    if (vsq < 0.0) vsq = fabs(vsq);
+    if (vsq > 1.0) { o.vsqgt1 += 1; vsq = 1.0 - dv; }
+    double gtmp = 1.0 - vsq;
+    if (gtmp < safty) gtmp = safty;
+    const double usq = (1.0 - gtmp) / gtmp;
+    gf = sqrt(1.0 + usq);
+    gfsq = gf * gf;
+    ro = dro / gf;
+    const double xi = wg * (1.0 / gfsq) - ro;
+    pp = gam2 * xi;
+    if (ro < 0.0) { o.negrho = 1; ro = dmin * x1pow15; }
+    if (pp < 0.0) { o.negp = 1; pp = pmin * (gam - 1.0) * x1pow25; }
+#pragma unroll
+    for (int m = 1; m <= 3; m++) {
+      const double qt = qcon[m] + oncon[m] * qdotn;
+      util[m] = (gf * (1.0 / (wg + bsq))) * (qt + qdotb * bcon[m] * (1.0 / wg));
+    }
+    o.w[0] = ro; o.w[1] = util[1]; o.w[2] = util[2]; o.w[3] = util[3]; o.w[4] = pp;
+    o.w[5] = pp / pow(ro, gam - 1.0);
+  } else {
+    o.fail = 1;
+    o.w[0] = ro1; o.w[1] = wold[1]; o.w[2] = wold[2]; o.w[3] = wold[3]; o.w[4] = pp1;
+    o.w[5] = pp1 / pow(ro1, gam - 1.0);
+  }
+  o.w[6] = u[6] * (1.0 / detg1);
+  o.w[7] = u[7] * (1.0 / detg1);
+  o.w[8] = u[8] * (1.0 / detg1);
+  wg_out = wg;
+}
+
+// one cell of the recovery iwvec selects
+template <int IW>
+RH_DEV void recov_any(const double u[9], const double wold[9], const Met& M, double gam, double dmin, double pmin,
+                      int iter, double x1pow15, double x1pow25, double wg_in, bool& carry, RecovOut& o,
+                      double& wg_out) {
+  if constexpr (IW == 2) recov2_cell(u, wold, M, gam, dmin, pmin, iter, x1pow15, x1pow25, wg_in, carry, o, wg_out);
+  else recov_cell(u, wold, M, gam, dmin, pmin, iter, x1pow15, x1pow25, wg_in, carry, o, wg_out);
+}
+
+// ------------------------------------------------------------------------
 // Update of one interior cell: flux divergence (with CT'd B fluxes), the
 // geometric source (caltenr + calsf, calsf.f90:6-151), the TVD-RK stage
 // (rkt.f90:2-180) plus its source add (rkt.f90:183-320), then recov1a.
@@ -774,8 +956,10 @@ RH_DEV void b_rk(const Geo& G, const StageArgs& A, long long blk) {
   }
 }
 
-// recov1a (recov.f90:578-1228) on every interior cell from the stage's RK
-// result; cells whose initial W is inherited are flagged for k_recov_carry.
+// recov1a (recov.f90:578-1228) or recov2 (recov.f90:1919-2454) on every
+// interior cell from the stage's RK result; cells whose initial W is
+// inherited are flagged for k_recov_carry.
+template <int IW>
 RH_DEV void b_recov(const Geo& G, const StageArgs& A, long long blk) {
   int i, j, k;
   const long long t = blk * blockDim.x + threadIdx.x;
@@ -793,7 +977,8 @@ RH_DEV void b_recov(const Geo& G, const StageArgs& A, long long blk) {
   RecovOut ro;
   bool carry;
   double wgout;
-  recov_cell(u, w, M, A.gam, A.dmin, A.pmin, A.iter, A.r15[i - 1], A.r25[i - 1], __builtin_nan(""), carry, ro, wgout);
+  recov_any<IW>(u, w, M, A.gam, A.dmin, A.pmin, A.iter, A.r15[i - 1], A.r25[i - 1], __builtin_nan(""), carry, ro,
+                wgout);
   const bool dep = carry && !ro.indep;
 #pragma unroll
   for (int n = 0; n < 9; n++) A.Wpre[(long long)n * N + c] = ro.w[n];
@@ -804,6 +989,7 @@ RH_DEV void b_recov(const Geo& G, const StageArgs& A, long long blk) {
   count_wave(A.cpart, blk, C_NEGRHO, ro.negrho && !dep);
   count_wave(A.cpart, blk, C_NEGP, ro.negp && !dep);
   count_wave(A.cpart, blk, C_VSQGT1, ro.vsqgt1 && !dep);
+  if (ro.vsqgt1 > 1 && !dep) atomicAdd(&A.cnt[C_VSQGT1], (unsigned long long)(ro.vsqgt1 - 1));  // recov2: per trip
   if (ro.det0 && !dep) atomicAdd(&A.cnt[C_DET0], (unsigned long long)ro.det0);  // rare: one event per det=0 trip
   count_wave(A.cpart, blk, C_WGCARRY, carry);
   count_wave(A.cpart, blk, C_GUARD, ro.guard && !dep);
@@ -1252,6 +1438,7 @@ RH_DEV void b_advance_time(const AdvArgs& A) {
 // predecessor's final W is known: passes over the flagged list, each taking
 // the cells whose predecessor is resolved, until none is left.  Chains of
 // consecutive flagged cells take one pass per link.  One workgroup.
+template <int IW>
 RH_DEV void b_recov_carry(const Geo& G, const FixArgs& A) {
   __shared__ int s_progress, s_left;
   unsigned long long n = A.carry[0];
@@ -1288,14 +1475,14 @@ RH_DEV void b_recov_carry(const Geo& G, const FixArgs& A) {
       RecovOut ro;
       bool carry;
       double wgout;
-      recov_cell(u, w, M, A.gam, A.dmin, A.pmin, A.iter, A.r15[i - 1], A.r25[i - 1], wg_prev, carry, ro, wgout);
+      recov_any<IW>(u, w, M, A.gam, A.dmin, A.pmin, A.iter, A.r15[i - 1], A.r25[i - 1], wg_prev, carry, ro, wgout);
       for (int m = 0; m < 9; m++) A.Wpre[(long long)m * N + c] = ro.w[m];
       A.wgout[c] = wgout;
       if (ro.fallback) atomicAdd(&A.cnt[C_FALLBACK], 1ULL);
       if (ro.fail) atomicAdd(&A.cnt[C_FAIL], 1ULL);
       if (ro.negrho) atomicAdd(&A.cnt[C_NEGRHO], 1ULL);
       if (ro.negp) atomicAdd(&A.cnt[C_NEGP], 1ULL);
-      if (ro.vsqgt1) atomicAdd(&A.cnt[C_VSQGT1], 1ULL);
+      if (ro.vsqgt1) atomicAdd(&A.cnt[C_VSQGT1], (unsigned long long)ro.vsqgt1);
       if (ro.det0) atomicAdd(&A.cnt[C_DET0], (unsigned long long)ro.det0);
       if (ro.guard) atomicAdd(&A.cnt[C_GUARD], 1ULL);
       pend[e] = 2;
@@ -1397,8 +1584,10 @@ __global__ void __launch_bounds__(256) k_sweep(Geo G, SweepArgs A) { b_sweep<D, 
 __global__ void __launch_bounds__(256) k_fluxct(Geo G, CtArgs A) { b_fluxct(G, A, xcd_block()); }
 template <int STAGE, bool SYM>
 __global__ void __launch_bounds__(256) k_rk(Geo G, StageArgs A) { b_rk<STAGE, SYM>(G, A, xcd_block()); }
-__global__ void __launch_bounds__(256) k_recov(Geo G, StageArgs A) { b_recov(G, A, xcd_block()); }
-__global__ void __launch_bounds__(1024) k_recov_carry(Geo G, FixArgs A) { b_recov_carry(G, A); }
+template <int IW>
+__global__ void __launch_bounds__(256) k_recov(Geo G, StageArgs A) { b_recov<IW>(G, A, xcd_block()); }
+template <int IW>
+__global__ void __launch_bounds__(1024) k_recov_carry(Geo G, FixArgs A) { b_recov_carry<IW>(G, A); }
 __global__ void __launch_bounds__(256) k_bnd(Geo G, PostArgs A) { b_bnd(G, A, blockIdx.x); }
 __global__ void __launch_bounds__(256) k_post(Geo G, PostArgs A) { b_post(G, A, xcd_block()); }
 template <int ICHA>
@@ -1440,10 +1629,11 @@ __global__ void __launch_bounds__(256) kb_rk(const BatchItem* __restrict__ it, i
   const StageArgs A = it[q].A.st;
   b_rk<STAGE, SYM>(G, A, blk);
 }
+template <int IW>
 __global__ void __launch_bounds__(256) kb_recov(const BatchItem* __restrict__ it, int n) {
   RH_BATCH_PROLOGUE
   const StageArgs A = it[q].A.st;
-  b_recov(G, A, blk);
+  b_recov<IW>(G, A, blk);
 }
 __global__ void __launch_bounds__(256) kb_bnd(const BatchItem* __restrict__ it, int n) {
   RH_BATCH_PROLOGUE
@@ -1469,11 +1659,12 @@ __global__ void __launch_bounds__(256) kb_copy(const BatchItem* __restrict__ it,
 }
 #undef RH_BATCH_PROLOGUE
 // one workgroup per model
+template <int IW>
 __global__ void __launch_bounds__(1024) kb_recov_carry(const BatchItem* __restrict__ it, int n) {
   if ((int)blockIdx.x >= n) return;
   const Geo G = it[blockIdx.x].G;
   const FixArgs A = it[blockIdx.x].A.fx;
-  b_recov_carry(G, A);
+  b_recov_carry<IW>(G, A);
 }
 __global__ void __launch_bounds__(1024) kb_cfl_finish(const BatchItem* __restrict__ it, int n) {
   if ((int)blockIdx.x >= n) return;
@@ -1627,15 +1818,17 @@ hipError_t launch_update(const Geo& G, int stage, const StageArgs& A, hipStream_
 hipError_t launch_recov(const Geo& G, const StageArgs& A, hipStream_t st) {
   const long long n = box(G.nm1 + 1, G.ni - G.nm1) * jbox(G, G.nm1 + 1, G.nj - G.nm1) * box(G.nm1 + 1, G.nk - G.nm1);
   const int bt = slot_threads(G, TS_RECOV);
-  if (g_rec) { AnyArgs a{}; a.st = A; record(LK_RECOV, 0, bt, nblk(n, bt), G, a); return hipSuccess; }
+  if (g_rec) { AnyArgs a{}; a.st = A; record(LK_RECOV, G.iwvec, bt, nblk(n, bt), G, a); return hipSuccess; }
   if (n == 0) return hipSuccess;
-  hipLaunchKernelGGL(k_recov, dim3(nblk(n, bt)), dim3(bt), 0, st, G, A);
+  if (G.iwvec == 2) hipLaunchKernelGGL(k_recov<2>, dim3(nblk(n, bt)), dim3(bt), 0, st, G, A);
+  else hipLaunchKernelGGL(k_recov<1>, dim3(nblk(n, bt)), dim3(bt), 0, st, G, A);
   return hipGetLastError();
 }
 
 hipError_t launch_recov_carry(const Geo& G, const FixArgs& A, hipStream_t st) {
-  if (g_rec) { AnyArgs a{}; a.fx = A; record(LK_CARRY, 0, 1024, 1, G, a); return hipSuccess; }
-  hipLaunchKernelGGL(k_recov_carry, dim3(1), dim3(1024), 0, st, G, A);
+  if (g_rec) { AnyArgs a{}; a.fx = A; record(LK_CARRY, G.iwvec, 1024, 1, G, a); return hipSuccess; }
+  if (G.iwvec == 2) hipLaunchKernelGGL(k_recov_carry<2>, dim3(1), dim3(1024), 0, st, G, A);
+  else hipLaunchKernelGGL(k_recov_carry<1>, dim3(1), dim3(1024), 0, st, G, A);
   return hipGetLastError();
 }
 
@@ -1736,14 +1929,20 @@ hipError_t launch_batched(int kind, int variant, int threads, const BatchItem* i
 #undef L
       break;
     }
-    case LK_RECOV: hipLaunchKernelGGL(kb_recov, g, b, 0, st, it, n); break;
+    case LK_RECOV:
+      if (variant == 2) hipLaunchKernelGGL(kb_recov<2>, g, b, 0, st, it, n);
+      else hipLaunchKernelGGL(kb_recov<1>, g, b, 0, st, it, n);
+      break;
     case LK_BND: hipLaunchKernelGGL(kb_bnd, g, b, 0, st, it, n); break;
     case LK_POST: hipLaunchKernelGGL(kb_post, g, b, 0, st, it, n); break;
     case LK_CFL:
       with_icha(variant, [&](auto C) { hipLaunchKernelGGL((kb_cfl<C.value>), g, b, 0, st, it, n); });
       break;
     case LK_COPY: hipLaunchKernelGGL(kb_copy, g, b, 0, st, it, n); break;
-    case LK_CARRY: hipLaunchKernelGGL(kb_recov_carry, dim3(n), dim3(1024), 0, st, it, n); break;
+    case LK_CARRY:
+      if (variant == 2) hipLaunchKernelGGL(kb_recov_carry<2>, dim3(n), dim3(1024), 0, st, it, n);
+      else hipLaunchKernelGGL(kb_recov_carry<1>, dim3(n), dim3(1024), 0, st, it, n);
+      break;
     case LK_CFL_FINISH: hipLaunchKernelGGL(kb_cfl_finish, dim3(n), dim3(1024), 0, st, it, n); break;
     case LK_ADVANCE: hipLaunchKernelGGL(kb_advance_time, dim3(n), dim3(64), 0, st, it, n); break;
     default: return hipErrorInvalidValue;

@@ -51,7 +51,7 @@ bool check_pram(const raishin_pram& p) {
     set_error("icha=%d not built (0 Gammie, 1 Leismann, 2 Del Zanna speeds)", p.icha);
     return false;
   }
-  if (p.iwvec != 1) { set_error("iwvec=%d not built (1 = recov1a)", p.iwvec); return false; }
+  if (p.iwvec != 1 && p.iwvec != 2) { set_error("iwvec=%d not built (1 recov1a, 2 recov2)", p.iwvec); return false; }
   if (p.ieos != 0) { set_error("ieos=%d not built (0 = gamma law)", p.ieos); return false; }
   const int nm1 = ghost_nm0(p.irec) + 1;
   const int n[3] = {p.imax, p.jmax, p.kmax};
@@ -231,6 +231,7 @@ static bool build_model_(raishin_model* m, const raishin_pram* p, const double* 
   G.ni = p->imax; G.nj = p->jmax; G.nk = p->kmax;
   G.nm0 = ghost_nm0(p->irec); G.nm1 = G.nm0 + 1;
   G.icha = p->icha;
+  G.iwvec = p->iwvec;
   G.N = (long long)G.ni * G.nj * G.nk;
   G.P2 = (long long)G.ni * G.nk;
   G.njs = G.nj;

@@ -120,7 +120,8 @@ def test_batch_mp5_rk3():
 def test_batch_mc():
     members = [(configs.U_MC, False, 0, 100.0), (configs.T2D_MC, True, 1, 10.0), (configs.U_MC, True, 2, 100.0),
                (replace(configs.T2D_MC, name="t2dmc_leis", icha=1), False, 3, 10.0),
-               (replace(configs.U_MC, name="umc_gam", icha=0), True, 4, 100.0)]
+               (replace(configs.U_MC, name="umc_gam", icha=0), True, 4, 100.0),
+               (replace(configs.T2D_MC, name="t2dmc_rc2", iwvec=2), False, 5, 10.0)]
     alone, batched = pairs(members)
     b = native.Batch(batched)
     for mm in alone:

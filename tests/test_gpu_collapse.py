@@ -42,7 +42,7 @@ def same(a, b, tag):
     assert a.counters() == b.counters(), tag
 
 
-@pytest.mark.parametrize("name,mixed", [("t2d", False), ("c1", False), ("t2d", True), ("t2dmc", False)])
+@pytest.mark.parametrize("name,mixed", [("t2d", False), ("c1", False), ("t2d", True), ("t2dmc", False), ("t2drc2", False)])
 def test_collapsed_rows_match_full(name, mixed):
     (col, full), _ = pair(configs.ALL[name], mixed)
     assert col.stored_rows() == 1 and full.stored_rows() == col.p.jmax

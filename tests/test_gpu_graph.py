@@ -27,7 +27,7 @@ def build(name, mixed=False):
     return mod
 
 
-@pytest.mark.parametrize("name,mixed", [("t2d", False), ("ump5", False), ("uppm", True), ("t2dmc", True)])
+@pytest.mark.parametrize("name,mixed", [("t2d", False), ("ump5", False), ("uppm", True), ("t2dmc", True), ("urc2", False)])
 def test_graph_replays_match_eager(name, mixed):
     a, b = build(name, mixed), build(name, mixed)
     for rep in range(3):

@@ -59,6 +59,9 @@ SMALL = {
     "gppm": replace(configs.G_PPM, kmax=16),
     "gmp5": replace(configs.G_MP5, kmax=13),
     "gweno": replace(configs.G_WENO, kmax=13),
+    # recov2 on the MP5 torus: many non-converged cells and negative
+    # pressures, so the failure path and the skipped Newton cycles run
+    "t3drc2": replace(configs.T3D, name="t3drc2", iwvec=2),
 }
 
 
@@ -255,6 +258,9 @@ ORACLE_CASES = [
     ("t2dmc", "torus", 6),
     ("uleis", "torus", 6),
     ("t2dgam", "torus", 6),
+    ("t2drc2", "torus", 8),
+    ("urc2", "torus", 6),
+    ("t3drc2", "torus", 4),
     ("gppm", "synthetic", 4),
     ("gmp5", "synthetic", 4),
     ("gweno", "synthetic", 4),

@@ -112,10 +112,14 @@ def corner_divergence(ww):
                    - (bz[:-1, :-1, :-1] + bz[1:, :-1, :-1] + bz[:-1, 1:, :-1] + bz[1:, 1:, :-1]))
 
 
-def test_recovery_round_trip(oracle_lib):
-    """uri -> caluu1 -> recov1a returns the primitives (Newton-Raphson
-    tolerance 1e-7, recov.f90:645)."""
-    p = configs.T2D.resolved()
+@pytest.mark.parametrize("iwvec,tol", [(1, 1e-6), (2, 1e-5)])
+def test_recovery_round_trip(oracle_lib, iwvec, tol):
+    """uri -> caluu1 -> recov returns the primitives: recov1a's 2D
+    Newton-Raphson (tolerance 1e-7, recov.f90:645) and recov2's 1D one on
+    W - D (tolerance 1e-8, recov.f90:1986), all four velocity and pressure
+    fields, from a poor initial guess."""
+    from dataclasses import replace as _r
+    p = _r(configs.T2D, iwvec=iwvec).resolved()
     g = native.setup_grid(p)
     m = native.setup_metric(p, g)
     uri = native.setup_torus(p, g, m)
@@ -124,12 +128,18 @@ def test_recovery_round_trip(oracle_lib):
     o.call("caluu1", uri, uu, m["gcov"], m["gcon"], m["detg"])
     w = uri.copy(order="F")
     w[[0, 4]] *= 1.3   # a poor initial guess
-    o.call("recov1a", uu, w, m["gcov"], m["gcon"], m["detg"], g["x1"], p.nm1)
+    o.reset_counters()
+    o.call("recov", uu, w, m["gcov"], m["gcon"], m["detg"], g["x1"], p.nm1)
+    assert o.counters()["recov_fail"] == 0
     n1 = p.nm1
     s = (slice(None), slice(n1, p.imax - n1), slice(n1, p.jmax - n1), slice(n1, p.kmax - n1))
     for v in (0, 4):
         rel = np.abs(w[v][s[1:]] - uri[v][s[1:]]) / np.abs(uri[v][s[1:]])
-        assert rel.max() < 1e-6, (v, rel.max())
+        assert rel.max() < tol, (v, rel.max())
+    vmax = np.abs(uri[1:4][s]).max()
+    for v in (1, 2, 3):
+        err = np.abs(w[v][s[1:]] - uri[v][s[1:]]).max() / vmax
+        assert err < tol, (v, err)
 
 
 def test_cfl_static_gas_flat_metric(oracle_lib):

@@ -32,7 +32,7 @@ def test_setup_matches_reference(name):
         assert bits_equal(m[k], m2[k]), k
 
 
-@pytest.mark.parametrize("name", ["uppm", "ump5", "uweno"])
+@pytest.mark.parametrize("name", ["uppm", "ump5", "uweno", "urc2"])
 def test_routines_match_reference(name):
     need(name)
     r, g, m = grid_and_metric(name)
@@ -50,12 +50,12 @@ def test_routines_match_reference(name):
     uh[:, n1:ni - n1, n1:nj - n1, n1:nk - n1] *= 1.0 + 2e-3
     wr, wo, ur, uo = uri.copy(order="F"), uri.copy(order="F"), uh.copy(order="F"), uh.copy(order="F")
     r.call("recov", ur, wr, m["gcov"], m["gcon"], m["detg"], g["x1"], g["x2"], g["x3"], n1, *B)
-    o.call("recov1a", uo, wo, m["gcov"], m["gcon"], m["detg"], g["x1"], n1)
+    o.call("recov", uo, wo, m["gcov"], m["gcon"], m["detg"], g["x1"], n1)
     assert bits_equal(wr, wo) and bits_equal(ur, uo)
 
 
 @pytest.mark.parametrize("name,steps", [("t2d", 3), ("t3d", 2), ("utube", 6), ("umc", 4), ("t2dmc", 3), ("uleis", 3),
-                                        ("t2dgam", 3)])
+                                        ("t2dgam", 3), ("t2drc2", 3), ("urc2", 3)])
 def test_step_matches_reference(name, steps):
     """Composed step: reference routines in main.f90's order (bnd4/infchk from the
     oracle, bnd.f90 is not buildable here) against orc_step; NaN-aware.  The
