@@ -1148,6 +1148,15 @@ RH_DEV void b_bnd(const Geo& G, const PostArgs& A, long long blk) {
 }
 
 // infchk + atmosphere1a + pminmax2b + caluu1 on every cell, after k_bnd.
+// The metric of cell (i,k) through a pointer the compiler cannot see
+// through, so its values are loaded again (from cache) in the next phase
+// instead of being held in registers across phases.  k_post's three phases
+// each read the 32 metric values; holding them costs two waves per SIMD.
+RH_DEV Met reload_met(const double* metC, const Geo& G, int i, int k) {
+  asm volatile("" : "+s"(metC));
+  return Met{metC, G.P2, q2(i, k, G)};
+}
+
 RH_DEV void b_post(const Geo& G, const PostArgs& A, long long blk) {
   const long long t = blk * blockDim.x + threadIdx.x;
   int i, j, k;
@@ -1208,6 +1217,7 @@ RH_DEV void b_post(const Geo& G, const PostArgs& A, long long blk) {
   // pminmax2b: correction.f90:808-961
   bool scaled = false, hit = false;
   {
+    const Met M = reload_met(A.metC, G, i, k);
     const double gflmax = 20.0;
     double g3[9], gi3[9];
     cal3met(M, g3, gi3);
@@ -1258,6 +1268,7 @@ RH_DEV void b_post(const Geo& G, const PostArgs& A, long long blk) {
   for (int n = 0; n < 9; n++) A.Wout[(long long)n * N + c] = w[n];
   // caluu1 (caluu.f90:2-126): conserved variables of the final primitives
   {
+    const Met M = reload_met(A.metC, G, i, k);
     Prel s2;
     prelude(w, gam, M, s2);
     double u[9];
