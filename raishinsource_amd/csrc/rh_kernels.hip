@@ -348,8 +348,13 @@ RH_DEV void b_sweep(const Geo& G, const SweepArgs& A, long long blk) {
   int i, j, k;
   const long long t = blk * blockDim.x + threadIdx.x;
   const int lj = jlo(G, G.nm0), hj = jhi(G, G.nj - G.nm0);
-  const bool in = (D == 3) ? box_index_ikj(t, G.nm0, G.ni - G.nm0, lj, hj, G.nm0, G.nk - G.nm0, i, j, k)
-                           : box_index(t, G.nm0, G.ni - G.nm0, lj, hj, G.nm0, G.nk - G.nm0, i, j, k);
+  // k sweeps that read a stencil along k (MP5, WENO5) walk k before j so
+  // neighbouring workgroups share the stencil planes in L2; with the face
+  // values precomputed (PPM, MC) the i,j,k walk keeps each (i,k) metric
+  // entry in L2 across the j rows instead.
+  constexpr bool kfirst = D == 3 && (IREC == 6 || IREC == 7);
+  const bool in = kfirst ? box_index_ikj(t, G.nm0, G.ni - G.nm0, lj, hj, G.nm0, G.nk - G.nm0, i, j, k)
+                         : box_index(t, G.nm0, G.ni - G.nm0, lj, hj, G.nm0, G.nk - G.nm0, i, j, k);
   if (!in) return;
   const long long N = G.N;
   const long long s = (D == 1) ? 1LL : (D == 2) ? jstride(G) : kstride(G);
