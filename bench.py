@@ -127,7 +127,7 @@ def main():
     p = configs.ALL[args.config].resolved()
     t_setup = time.perf_counter()
     g = native.setup_grid(p)
-    m = native.setup_metric(p, g)
+    m = native.setup_metric(p, g, row=True)  # the axisymmetric metric's one phi row
     uri = native.setup_torus(p, g, m, seed=0, noise=0.02)
     model = native.Model(p, g, m)
     model.set_state(uri)
@@ -259,7 +259,7 @@ def ensemble(args):
         p = replace(configs.C1, name=f"ens{q}", imax=ni + 2 * configs.C1.nm1, kmax=nk + 2 * configs.C1.nm1,
                     akm=a, xmin=None).resolved()
         g = native.setup_grid(p)
-        m = native.setup_metric(p, g)
+        m = native.setup_metric(p, g, row=True)
         uri = native.setup_torus(p, g, m, rin=rin, rmax=rin + 6.0, beta=beta, noise=0.02, seed=q)
         mod = native.Model(p, g, m)
         models.append(mod)

@@ -62,6 +62,19 @@ raishin_model *raishin_model_create(const raishin_pram *p, const double *x1, con
                                     const double *gconj, const double *gcovk, const double *gconk,
                                     const double *detg, const double *detgi, const double *detgj,
                                     const double *detgk, const double *christ);
+/* The same model from the one phi row of an axisymmetric metric: gcov..gconk
+ * (4,4,imax,kmax), detg..detgk (imax,kmax), christ (4,4,4,imax,kmax), as
+ * raishin_coord1 / raishin_calchrist write them for a pram with jmax = 1.
+ * Bit-identical tables to raishin_model_create on the full arrays, without
+ * building (and first-touching) 200 doubles per cell on the host; no
+ * counterpart in the reference, whose metric arrays always hold every cell. */
+raishin_model *raishin_model_create_ik(const raishin_pram *p, const double *x1, const double *x2,
+                                       const double *x3, const double *dx1b, const double *dx2b,
+                                       const double *dx3b, const double *gcov, const double *gcon,
+                                       const double *gcovi, const double *gconi, const double *gcovj,
+                                       const double *gconj, const double *gcovk, const double *gconk,
+                                       const double *detg, const double *detgi, const double *detgj,
+                                       const double *detgk, const double *christ);
 void raishin_model_destroy(raishin_model *m);
 
 /* Primitive (uri) and conserved (uu) state, reference layout (9,imax,jmax,kmax).

@@ -45,6 +45,13 @@ void raishin_calchrist(const raishin_pram *p, double *christ, const double *x1, 
 int raishin_mdtorus1(const raishin_pram *p, const raishin_torus *t, double *uri, const double *gcov,
                      const double *gcon, const double *detg, const double *x1, const double *x2,
                      const double *x3);
+/* The same torus (bit for bit) from the one phi row of the axisymmetric
+ * metric: gcov/gcon(4,4,imax,kmax), detg(imax,kmax), as raishin_coord1 writes
+ * them for a pram with jmax = 1.  The full metric of a 512x256 torus is
+ * 2.3 GB of host arrays whose first touch costs seconds; the row is 16 MB. */
+int raishin_mdtorus1_ik(const raishin_pram *p, const raishin_torus *t, double *uri, const double *gcov,
+                        const double *gcon, const double *detg, const double *x1, const double *x2,
+                        const double *x3);
 
 /* Riemann-tube states (left [0], right [1]): density, pressure, 3-velocity,
  * magnetic field. */

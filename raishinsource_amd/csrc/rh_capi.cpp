@@ -42,24 +42,44 @@ void raishin_clear_error(void) {
   g_err.clear();
 }
 
-raishin_model* raishin_model_create(const raishin_pram* p, const double* x1, const double* x2, const double* x3,
-                                    const double* dx1b, const double* dx2b, const double* dx3b, const double* gcov,
-                                    const double* gcon, const double* gcovi, const double* gconi, const double* gcovj,
-                                    const double* gconj, const double* gcovk, const double* gconk, const double* detg,
-                                    const double* detgi, const double* detgj, const double* detgk,
-                                    const double* christ) {
+static raishin_model* create_model(const raishin_pram* p, const double* x1, const double* x2, const double* x3,
+                                   const double* dx1b, const double* dx2b, const double* dx3b, const double* gcov,
+                                   const double* gcon, const double* gcovi, const double* gconi, const double* gcovj,
+                                   const double* gconj, const double* gcovk, const double* gconk, const double* detg,
+                                   const double* detgi, const double* detgj, const double* detgk, const double* christ,
+                                   int metric_rows) {
   if (!p || !x1 || !x2 || !x3 || !gcov || !gcon || !christ) { set_error("raishin_model_create: null argument"); return nullptr; }
   raishin_model* m = new raishin_model();
   const double* gcv[4] = {gcov, gcovi, gcovj, gcovk};
   const double* gcn[4] = {gcon, gconi, gconj, gconk};
   const double* dtg[4] = {detg, detgi, detgj, detgk};
-  if (!build_model(m, p, x1, x2, x3, dx1b, dx2b, dx3b, gcv, gcn, dtg, christ)) {
+  if (!build_model(m, p, x1, x2, x3, dx1b, dx2b, dx3b, gcv, gcn, dtg, christ, metric_rows)) {
     dfree_all(m);
     if (m->st) (void)hipStreamDestroy(m->st);
     delete m;
     return nullptr;
   }
   return m;
+}
+
+raishin_model* raishin_model_create(const raishin_pram* p, const double* x1, const double* x2, const double* x3,
+                                    const double* dx1b, const double* dx2b, const double* dx3b, const double* gcov,
+                                    const double* gcon, const double* gcovi, const double* gconi, const double* gcovj,
+                                    const double* gconj, const double* gcovk, const double* gconk, const double* detg,
+                                    const double* detgi, const double* detgj, const double* detgk,
+                                    const double* christ) {
+  return create_model(p, x1, x2, x3, dx1b, dx2b, dx3b, gcov, gcon, gcovi, gconi, gcovj, gconj, gcovk, gconk, detg,
+                      detgi, detgj, detgk, christ, p ? p->jmax : 0);
+}
+
+raishin_model* raishin_model_create_ik(const raishin_pram* p, const double* x1, const double* x2, const double* x3,
+                                       const double* dx1b, const double* dx2b, const double* dx3b, const double* gcov,
+                                       const double* gcon, const double* gcovi, const double* gconi,
+                                       const double* gcovj, const double* gconj, const double* gcovk,
+                                       const double* gconk, const double* detg, const double* detgi,
+                                       const double* detgj, const double* detgk, const double* christ) {
+  return create_model(p, x1, x2, x3, dx1b, dx2b, dx3b, gcov, gcon, gcovi, gconi, gcovj, gconj, gcovk, gconk, detg,
+                      detgi, detgj, detgk, christ, 1);
 }
 
 void raishin_model_destroy(raishin_model* m) {

@@ -104,10 +104,11 @@ void dfree_all(raishin_model* m) {
 }
 
 // (i,k) tables of one metric location: planes 0..15 gcov, 16..31 gcon, 32 detg.
-// The reference arrays are checked to be independent of j, bit for bit.
-bool metric_tables(const raishin_pram& p, const double* gcov, const double* gcon, const double* detg,
+// The reference arrays are checked to be independent of j, bit for bit; nj
+// is the number of phi rows the arrays hold (jmax, or 1 for one row).
+bool metric_tables(const raishin_pram& p, long long nj, const double* gcov, const double* gcon, const double* detg,
                    std::vector<double>& out, const char* what) {
-  const long long ni = p.imax, nj = p.jmax, nk = p.kmax, P2 = ni * nk;
+  const long long ni = p.imax, nk = p.kmax, P2 = ni * nk;
   out.assign(33 * P2, 0.0);
   for (long long k = 0; k < nk; k++)
     for (long long j = 0; j < nj; j++)
@@ -210,13 +211,13 @@ std::vector<double> ppm_coefficients(const std::vector<double>& ds) {
 static bool build_model_(raishin_model* m, const raishin_pram* p, const double* x1, const double* x2,
                          const double* x3, const double* dx1b, const double* dx2b, const double* dx3b,
                          const double* const gcv[4], const double* const gcn[4], const double* const dtg[4],
-                         const double* christ);
+                         const double* christ, int metric_rows);
 
 bool build_model(raishin_model* m, const raishin_pram* p, const double* x1, const double* x2, const double* x3,
                  const double* dx1b, const double* dx2b, const double* dx3b, const double* const gcv[4],
-                 const double* const gcn[4], const double* const dtg[4], const double* christ) {
+                 const double* const gcn[4], const double* const dtg[4], const double* christ, int metric_rows) {
   const size_t bytes0 = g_dalloc_bytes;
-  const bool ok = build_model_(m, p, x1, x2, x3, dx1b, dx2b, dx3b, gcv, gcn, dtg, christ);
+  const bool ok = build_model_(m, p, x1, x2, x3, dx1b, dx2b, dx3b, gcv, gcn, dtg, christ, metric_rows);
   m->dev_bytes = g_dalloc_bytes - bytes0;
   return ok;
 }
@@ -224,7 +225,7 @@ bool build_model(raishin_model* m, const raishin_pram* p, const double* x1, cons
 static bool build_model_(raishin_model* m, const raishin_pram* p, const double* x1, const double* x2,
                          const double* x3, const double* dx1b, const double* dx2b, const double* dx3b,
                          const double* const gcv[4], const double* const gcn[4], const double* const dtg[4],
-                         const double* christ) {
+                         const double* christ, int metric_rows) {
   m->p = *p;
   if (!check_pram(*p)) return false;
   Geo& G = m->G;
@@ -273,16 +274,17 @@ static bool build_model_(raishin_model* m, const raishin_pram* p, const double* 
   const char* names[4] = {"centre", "i-face", "j-face", "k-face"};
   std::vector<double> tab;
   for (int a = 0; a < 4; a++) {
-    if (!metric_tables(*p, gcv[a], gcn[a], dtg[a], tab, names[a])) return false;
+    if (!metric_tables(*p, metric_rows, gcv[a], gcn[a], dtg[a], tab, names[a])) return false;
     if (!dalloc(&m->met[a], tab.size())) return false;
     if (!h2d_sync(m->met[a], tab.data(), tab.size() * 8, m->st)) return false;
   }
   {
     std::vector<double> ct(64 * G.P2);
+    const long long mr = metric_rows;
     for (long long k = 0; k < G.nk; k++)
-      for (long long j = 0; j < G.nj; j++)
+      for (long long j = 0; j < mr; j++)
         for (long long i = 0; i < G.ni; i++) {
-          const long long c = i + G.ni * (j + G.nj * k), q = i + G.ni * k;
+          const long long c = i + G.ni * (j + mr * k), q = i + G.ni * k;
           for (int e = 0; e < 64; e++) {
             const double v = christ[64 * c + e];
             if (j == 0) ct[e * G.P2 + q] = v;

@@ -177,6 +177,169 @@ inline double splitmix_uniform(uint64_t& s) {
 
 }  // namespace
 
+static int mdtorus1_impl(const raishin_pram* pp, const raishin_torus* tp, double* uri, const double* gcov,
+                         const double* gcon, const double* detg, const double* x1, const double* x2,
+                         const double* x3, bool slice) {
+  // the metric arrays hold every cell, or (slice) the one phi row of an
+  // axisymmetric metric: (4,4,imax,kmax) and (imax,kmax)
+  auto mq = [&](int i, int j, int k) -> size_t {
+    return slice ? (size_t)(i - 1) + (size_t)pp->imax * (size_t)(k - 1) : cell(*pp, i, j, k);
+  };
+  // Fishbone-Moncrief torus with a single poloidal loop, after mdtorus1
+  // (mdgrmhd.f90:879-1654): atmosphere, torus, density normalisation, seeded
+  // pressure noise, A_phi = max(rho/rho_max - 0.2, 0), field scaled to beta_min.
+  const raishin_pram& p = *pp;
+  const raishin_torus& t = *tp;
+  (void)x2;
+  const double gam = p.gam, akm = p.akm, kpol1 = 1.0e-3;
+  const double rin = t.rin, rmax = t.rmax;
+  const double thedge = (double)0.02f * PI, thedge1 = (double)0.98f * PI, small = 1.0e-12;
+  const int ni = p.imax, nj = p.jmax, nk = p.kmax;
+  const size_t N = (size_t)ni * nj * nk;
+  std::vector<double> alnh(N, 0.0);
+#pragma omp parallel for schedule(static)
+  for (int k = 1; k <= nk; k++)
+    for (int j = 1; j <= nj; j++)
+      for (int i = 1; i <= ni; i++) {
+        double* w = &uri[9 * cell(p, i, j, k)];
+        const double rr = x1[i - 1];
+        const double de = p.dmin * std::pow(rr, -1.5), pr = p.pmin * (gam - 1.0) * std::pow(rr, -2.5);
+        w[0] = de; w[1] = w[2] = w[3] = 0.0; w[4] = pr; w[5] = pr * (1.0 / std::pow(de, gam - 1.0));
+        w[6] = w[7] = w[8] = 0.0;
+      }
+  if (p.metric != 303) return 0;
+  const double clm = callfish(akm, rmax);
+  const double delin = rin * rin - 2.0 * rin + akm * akm;
+  const double aain = (rin * rin + akm * akm) * (rin * rin + akm * akm) - delin * akm * akm;
+  const double sigin = rin * rin;
+  const double t2a = 4.0 * (clm * clm * sigin * sigin) * delin * (1.0 / (aain * aain));
+  const double t2c = sigin * delin * (1.0 / aain);
+  const double t2d = 0.5 * std::log((1.0 + std::sqrt(1.0 + t2a)) * (1.0 / t2c));
+  const double t2e = 0.5 * std::sqrt(1.0 + t2a);
+  const double t2f = 2.0 * akm * rin * clm * (1.0 / aain);
+  double rhomax = 0.0;
+#pragma omp parallel for schedule(dynamic) reduction(max : rhomax)
+  for (int k = 1; k <= nk; k++)
+    for (int j = 1; j <= nj; j++)
+      for (int i = 1; i <= ni; i++) {
+        const double rr = x1[i - 1], th = x3[k - 1];
+        double sth = std::sin(th);
+        if (std::fabs(sth) < small) sth = (sth >= 0.0) ? small : -small;
+        const double cth = std::cos(th);
+        const double del = (rr * rr) - (2.0 * rr) + (akm * akm);
+        const double aa = (rr * rr + akm * akm) * (rr * rr + akm * akm) - del * akm * akm * sth * sth;
+        const double sig = rr * rr + akm * akm * cth * cth;
+        double al = 0.0;
+        if (rr >= rin && th >= thedge && th <= thedge1) {
+          const double t1a = 4.0 * (clm * clm * sig * sig) * del * (1.0 / (aa * sth * aa * sth));
+          const double t1c = sig * del * (1.0 / aa);
+          const double t1d = 0.5 * std::log((1.0 + std::sqrt(1.0 + t1a)) * (1.0 / t1c));
+          const double t1e = 0.5 * std::sqrt(1.0 + t1a);
+          const double t1f = 2.0 * akm * rr * clm * (1.0 / aa);
+          al = t1d - t1e - t1f - (t2d - t2e - t2f);
+          if (al <= 0.0) al = 0.0;
+        }
+        alnh[cell(p, i, j, k)] = al;
+        const double hm1 = std::exp(al) - 1.0;
+        if (hm1 > 0.0 && rr >= rin && th >= thedge && th <= thedge1) {
+          const double de = std::pow(hm1 * (gam - 1.0) / (kpol1 * gam), 1.0 / (gam - 1.0));
+          const double pr = kpol1 * std::pow(de, gam);
+          const double expm2chi = sig * sig * del * (1.0 / (aa * aa * sth * sth));
+          const double up1 = std::sqrt((-1.0 + std::sqrt(1.0 + 4.0 * clm * clm * expm2chi)) / 2.0);
+          const double t3a = 2.0 * akm * rr * std::sqrt(1.0 + up1 * up1);
+          const double t3b = std::sqrt(aa * sig * del);
+          const double t3c = std::sqrt(sig * (1.0 / aa)) * up1 * (1.0 / sth);
+          double ut[4] = {0.0, 0.0, t3a * (1.0 / t3b) + t3c, 0.0}, un[4];
+          transbl2ks(akm, ut, un, rr, th);
+          double* w = &uri[9 * cell(p, i, j, k)];
+          w[0] = de; w[1] = un[1]; w[2] = un[2]; w[3] = un[3]; w[4] = pr;
+          w[5] = pr * (1.0 / std::pow(de, gam - 1.0));
+          if (de > rhomax) rhomax = de;
+        }
+      }
+  if (rhomax <= 0.0) return -1;
+  // normalise density to 1 and perturb the pressure.  The noise is drawn in the
+  // reference's (k,j,i) order from a seeded splitmix64 stream; with one interior
+  // phi cell the model is axisymmetric and one value is drawn per (i,k).
+  const int nm0 = (p.irec == 5) ? 4 : (p.irec == 1 || p.irec == 2 || p.irec == 3) ? 2 : 3;  // main.f90:179-211
+  const bool axisym = (nj - 2 * (nm0 + 1)) <= 1;
+  uint64_t seed = (uint64_t)t.seed;
+  std::vector<double> noise((size_t)ni * nk, 0.0);
+  for (int k = 1; k <= nk; k++)
+    for (int j = 1; j <= nj; j++)
+      for (int i = 1; i <= ni; i++) {
+        const double rr = x1[i - 1], th = x3[k - 1];
+        const double rhomin1 = p.dmin * std::pow(rr, -1.5), prmin1 = p.pmin * (gam - 1.0) * std::pow(rr, -2.5);
+        const double hm1 = std::exp(alnh[cell(p, i, j, k)]) - 1.0;
+        if (!(hm1 > 0.0 && rr >= rin && th >= thedge && th <= thedge1)) continue;
+        double* w = &uri[9 * cell(p, i, j, k)];
+        double de = w[0] * (1.0 / rhomax), pr = w[4] * (1.0 / rhomax);
+        double u;
+        if (axisym) {
+          double& nz = noise[(size_t)(i - 1) + (size_t)ni * (k - 1)];
+          if (j == 1) nz = splitmix_uniform(seed);
+          u = nz;
+        } else {
+          u = splitmix_uniform(seed);
+        }
+        pr = pr * (1.0 + t.noise * (u - 0.5));
+        if (de <= rhomin1 || pr <= prmin1) { de = rhomin1; w[1] = w[2] = w[3] = 0.0; pr = prmin1; }
+        w[0] = de; w[4] = pr; w[5] = pr * (1.0 / std::pow(de, gam - 1.0));
+      }
+  // vector potential at cell corners and B = curl A / sqrt(-g): mdgrmhd.f90:1350-1460
+  std::vector<double> pom(N, 0.0);
+  for (int k = 2; k <= nk; k++)
+    for (int j = 1; j <= nj; j++)
+      for (int i = 2; i <= ni; i++) {
+        const double rhoav = 0.25 * (uri[9 * cell(p, i, j, k)] + uri[9 * cell(p, i - 1, j, k)] +
+                                     uri[9 * cell(p, i, j, k - 1)] + uri[9 * cell(p, i - 1, j, k - 1)]);
+        const double pm = (rhoav != 0.0) ? rhoav - t.acut : 0.0;
+        if (pm > 0.0) pom[cell(p, i, j, k)] = pm;
+      }
+  for (int j = 1; j <= nj; j++)
+    for (int i = 1; i <= ni; i++) pom[cell(p, i, j, 1)] = pom[cell(p, i, j, 2)];
+  for (int k = 1; k <= nk; k++)
+    for (int j = 1; j <= nj; j++) pom[cell(p, 1, j, k)] = pom[cell(p, 2, j, k)];
+  for (int k = 2; k <= nk - 1; k++)
+    for (int j = 1; j <= nj; j++)
+      for (int i = 2; i <= ni - 1; i++) {
+        const size_t c = cell(p, i, j, k);
+        const double ta = 0.5 * (pom[cell(p, i + 1, j, k + 1)] + pom[cell(p, i, j, k + 1)] - pom[cell(p, i + 1, j, k)] - pom[c]);
+        const double dx3a = 0.5 * (x3[k] + x3[k - 1]) - 0.5 * (x3[k - 1] + x3[k - 2]);
+        const double tb = 0.5 * (pom[cell(p, i + 1, j, k + 1)] + pom[cell(p, i + 1, j, k)] - pom[cell(p, i, j, k + 1)] - pom[c]);
+        const double dx1a = 0.5 * (x1[i] + x1[i - 1]) - 0.5 * (x1[i - 1] + x1[i - 2]);
+        uri[9 * c + 6] = (ta != 0.0) ? -ta * (1.0 / (dx3a * detg[mq(i, j, k)])) : 0.0;
+        uri[9 * c + 8] = (tb != 0.0) ? tb * (1.0 / (dx1a * detg[mq(i, j, k)])) : 0.0;
+      }
+  for (int j = 1; j <= nj; j++)
+    for (int i = 1; i <= ni; i++)
+      for (int q : {6, 8}) {
+        uri[9 * cell(p, i, j, 1) + q] = uri[9 * cell(p, i, j, 2) + q];
+        uri[9 * cell(p, i, j, nk) + q] = uri[9 * cell(p, i, j, nk - 1) + q];
+      }
+  for (int k = 1; k <= nk; k++)
+    for (int j = 1; j <= nj; j++)
+      for (int q : {6, 8}) {
+        uri[9 * cell(p, 1, j, k) + q] = uri[9 * cell(p, 2, j, k) + q];
+        uri[9 * cell(p, ni, j, k) + q] = uri[9 * cell(p, ni - 1, j, k) + q];
+      }
+  // beta normalisation: B *= sqrt(beta_act / beta), beta_act = min 2p/b^2
+  double bmin = 1.0e5;
+#pragma omp parallel for schedule(static) reduction(min : bmin)
+  for (int k = 1; k <= nk; k++)
+    for (int j = 1; j <= nj; j++)
+      for (int i = 1; i <= ni; i++) {
+        const size_t c = cell(p, i, j, k), q = mq(i, j, k);
+        const double b2 = bsq_of(&uri[9 * c], &gcov[16 * q], &gcon[16 * q]);
+        const double pb = (b2 != 0.0) ? 2.0 * uri[9 * c + 4] / b2 : 1.0e5;
+        if (pb < bmin) bmin = pb;
+      }
+  const double bnorm = (t.beta == 0.0) ? 0.0 : std::sqrt(bmin / t.beta);
+  for (size_t c = 0; c < N; c++)
+    for (int q = 6; q <= 8; q++) uri[9 * c + q] = uri[9 * c + q] * bnorm;
+  return 0;
+}
+
 extern "C" {
 
 void raishin_grid(const raishin_pram* pp, const raishin_grid_ext* ext, double* x1, double* x2, double* x3,
@@ -314,153 +477,12 @@ void raishin_calchrist(const raishin_pram* pp, double* christ, const double* x1,
 
 int raishin_mdtorus1(const raishin_pram* pp, const raishin_torus* tp, double* uri, const double* gcov,
                      const double* gcon, const double* detg, const double* x1, const double* x2, const double* x3) {
-  // Fishbone-Moncrief torus with a single poloidal loop, after mdtorus1
-  // (mdgrmhd.f90:879-1654): atmosphere, torus, density normalisation, seeded
-  // pressure noise, A_phi = max(rho/rho_max - 0.2, 0), field scaled to beta_min.
-  const raishin_pram& p = *pp;
-  const raishin_torus& t = *tp;
-  (void)x2;
-  const double gam = p.gam, akm = p.akm, kpol1 = 1.0e-3;
-  const double rin = t.rin, rmax = t.rmax;
-  const double thedge = (double)0.02f * PI, thedge1 = (double)0.98f * PI, small = 1.0e-12;
-  const int ni = p.imax, nj = p.jmax, nk = p.kmax;
-  const size_t N = (size_t)ni * nj * nk;
-  std::vector<double> alnh(N, 0.0);
-  for (int k = 1; k <= nk; k++)
-    for (int j = 1; j <= nj; j++)
-      for (int i = 1; i <= ni; i++) {
-        double* w = &uri[9 * cell(p, i, j, k)];
-        const double rr = x1[i - 1];
-        const double de = p.dmin * std::pow(rr, -1.5), pr = p.pmin * (gam - 1.0) * std::pow(rr, -2.5);
-        w[0] = de; w[1] = w[2] = w[3] = 0.0; w[4] = pr; w[5] = pr * (1.0 / std::pow(de, gam - 1.0));
-        w[6] = w[7] = w[8] = 0.0;
-      }
-  if (p.metric != 303) return 0;
-  const double clm = callfish(akm, rmax);
-  const double delin = rin * rin - 2.0 * rin + akm * akm;
-  const double aain = (rin * rin + akm * akm) * (rin * rin + akm * akm) - delin * akm * akm;
-  const double sigin = rin * rin;
-  const double t2a = 4.0 * (clm * clm * sigin * sigin) * delin * (1.0 / (aain * aain));
-  const double t2c = sigin * delin * (1.0 / aain);
-  const double t2d = 0.5 * std::log((1.0 + std::sqrt(1.0 + t2a)) * (1.0 / t2c));
-  const double t2e = 0.5 * std::sqrt(1.0 + t2a);
-  const double t2f = 2.0 * akm * rin * clm * (1.0 / aain);
-  double rhomax = 0.0;
-  for (int k = 1; k <= nk; k++)
-    for (int j = 1; j <= nj; j++)
-      for (int i = 1; i <= ni; i++) {
-        const double rr = x1[i - 1], th = x3[k - 1];
-        double sth = std::sin(th);
-        if (std::fabs(sth) < small) sth = (sth >= 0.0) ? small : -small;
-        const double cth = std::cos(th);
-        const double del = (rr * rr) - (2.0 * rr) + (akm * akm);
-        const double aa = (rr * rr + akm * akm) * (rr * rr + akm * akm) - del * akm * akm * sth * sth;
-        const double sig = rr * rr + akm * akm * cth * cth;
-        double al = 0.0;
-        if (rr >= rin && th >= thedge && th <= thedge1) {
-          const double t1a = 4.0 * (clm * clm * sig * sig) * del * (1.0 / (aa * sth * aa * sth));
-          const double t1c = sig * del * (1.0 / aa);
-          const double t1d = 0.5 * std::log((1.0 + std::sqrt(1.0 + t1a)) * (1.0 / t1c));
-          const double t1e = 0.5 * std::sqrt(1.0 + t1a);
-          const double t1f = 2.0 * akm * rr * clm * (1.0 / aa);
-          al = t1d - t1e - t1f - (t2d - t2e - t2f);
-          if (al <= 0.0) al = 0.0;
-        }
-        alnh[cell(p, i, j, k)] = al;
-        const double hm1 = std::exp(al) - 1.0;
-        if (hm1 > 0.0 && rr >= rin && th >= thedge && th <= thedge1) {
-          const double de = std::pow(hm1 * (gam - 1.0) / (kpol1 * gam), 1.0 / (gam - 1.0));
-          const double pr = kpol1 * std::pow(de, gam);
-          const double expm2chi = sig * sig * del * (1.0 / (aa * aa * sth * sth));
-          const double up1 = std::sqrt((-1.0 + std::sqrt(1.0 + 4.0 * clm * clm * expm2chi)) / 2.0);
-          const double t3a = 2.0 * akm * rr * std::sqrt(1.0 + up1 * up1);
-          const double t3b = std::sqrt(aa * sig * del);
-          const double t3c = std::sqrt(sig * (1.0 / aa)) * up1 * (1.0 / sth);
-          double ut[4] = {0.0, 0.0, t3a * (1.0 / t3b) + t3c, 0.0}, un[4];
-          transbl2ks(akm, ut, un, rr, th);
-          double* w = &uri[9 * cell(p, i, j, k)];
-          w[0] = de; w[1] = un[1]; w[2] = un[2]; w[3] = un[3]; w[4] = pr;
-          w[5] = pr * (1.0 / std::pow(de, gam - 1.0));
-          if (de > rhomax) rhomax = de;
-        }
-      }
-  if (rhomax <= 0.0) return -1;
-  // normalise density to 1 and perturb the pressure.  The noise is drawn in the
-  // reference's (k,j,i) order from a seeded splitmix64 stream; with one interior
-  // phi cell the model is axisymmetric and one value is drawn per (i,k).
-  const int nm0 = (p.irec == 5) ? 4 : (p.irec == 1 || p.irec == 2 || p.irec == 3) ? 2 : 3;  // main.f90:179-211
-  const bool axisym = (nj - 2 * (nm0 + 1)) <= 1;
-  uint64_t seed = (uint64_t)t.seed;
-  std::vector<double> noise((size_t)ni * nk, 0.0);
-  for (int k = 1; k <= nk; k++)
-    for (int j = 1; j <= nj; j++)
-      for (int i = 1; i <= ni; i++) {
-        const double rr = x1[i - 1], th = x3[k - 1];
-        const double rhomin1 = p.dmin * std::pow(rr, -1.5), prmin1 = p.pmin * (gam - 1.0) * std::pow(rr, -2.5);
-        const double hm1 = std::exp(alnh[cell(p, i, j, k)]) - 1.0;
-        if (!(hm1 > 0.0 && rr >= rin && th >= thedge && th <= thedge1)) continue;
-        double* w = &uri[9 * cell(p, i, j, k)];
-        double de = w[0] * (1.0 / rhomax), pr = w[4] * (1.0 / rhomax);
-        double u;
-        if (axisym) {
-          double& nz = noise[(size_t)(i - 1) + (size_t)ni * (k - 1)];
-          if (j == 1) nz = splitmix_uniform(seed);
-          u = nz;
-        } else {
-          u = splitmix_uniform(seed);
-        }
-        pr = pr * (1.0 + t.noise * (u - 0.5));
-        if (de <= rhomin1 || pr <= prmin1) { de = rhomin1; w[1] = w[2] = w[3] = 0.0; pr = prmin1; }
-        w[0] = de; w[4] = pr; w[5] = pr * (1.0 / std::pow(de, gam - 1.0));
-      }
-  // vector potential at cell corners and B = curl A / sqrt(-g): mdgrmhd.f90:1350-1460
-  std::vector<double> pom(N, 0.0);
-  for (int k = 2; k <= nk; k++)
-    for (int j = 1; j <= nj; j++)
-      for (int i = 2; i <= ni; i++) {
-        const double rhoav = 0.25 * (uri[9 * cell(p, i, j, k)] + uri[9 * cell(p, i - 1, j, k)] +
-                                     uri[9 * cell(p, i, j, k - 1)] + uri[9 * cell(p, i - 1, j, k - 1)]);
-        const double pm = (rhoav != 0.0) ? rhoav - t.acut : 0.0;
-        if (pm > 0.0) pom[cell(p, i, j, k)] = pm;
-      }
-  for (int j = 1; j <= nj; j++)
-    for (int i = 1; i <= ni; i++) pom[cell(p, i, j, 1)] = pom[cell(p, i, j, 2)];
-  for (int k = 1; k <= nk; k++)
-    for (int j = 1; j <= nj; j++) pom[cell(p, 1, j, k)] = pom[cell(p, 2, j, k)];
-  for (int k = 2; k <= nk - 1; k++)
-    for (int j = 1; j <= nj; j++)
-      for (int i = 2; i <= ni - 1; i++) {
-        const size_t c = cell(p, i, j, k);
-        const double ta = 0.5 * (pom[cell(p, i + 1, j, k + 1)] + pom[cell(p, i, j, k + 1)] - pom[cell(p, i + 1, j, k)] - pom[c]);
-        const double dx3a = 0.5 * (x3[k] + x3[k - 1]) - 0.5 * (x3[k - 1] + x3[k - 2]);
-        const double tb = 0.5 * (pom[cell(p, i + 1, j, k + 1)] + pom[cell(p, i + 1, j, k)] - pom[cell(p, i, j, k + 1)] - pom[c]);
-        const double dx1a = 0.5 * (x1[i] + x1[i - 1]) - 0.5 * (x1[i - 1] + x1[i - 2]);
-        uri[9 * c + 6] = (ta != 0.0) ? -ta * (1.0 / (dx3a * detg[c])) : 0.0;
-        uri[9 * c + 8] = (tb != 0.0) ? tb * (1.0 / (dx1a * detg[c])) : 0.0;
-      }
-  for (int j = 1; j <= nj; j++)
-    for (int i = 1; i <= ni; i++)
-      for (int q : {6, 8}) {
-        uri[9 * cell(p, i, j, 1) + q] = uri[9 * cell(p, i, j, 2) + q];
-        uri[9 * cell(p, i, j, nk) + q] = uri[9 * cell(p, i, j, nk - 1) + q];
-      }
-  for (int k = 1; k <= nk; k++)
-    for (int j = 1; j <= nj; j++)
-      for (int q : {6, 8}) {
-        uri[9 * cell(p, 1, j, k) + q] = uri[9 * cell(p, 2, j, k) + q];
-        uri[9 * cell(p, ni, j, k) + q] = uri[9 * cell(p, ni - 1, j, k) + q];
-      }
-  // beta normalisation: B *= sqrt(beta_act / beta), beta_act = min 2p/b^2
-  double bmin = 1.0e5;
-  for (size_t c = 0; c < N; c++) {
-    const double b2 = bsq_of(&uri[9 * c], &gcov[16 * c], &gcon[16 * c]);
-    const double pb = (b2 != 0.0) ? 2.0 * uri[9 * c + 4] / b2 : 1.0e5;
-    if (pb < bmin) bmin = pb;
-  }
-  const double bnorm = (t.beta == 0.0) ? 0.0 : std::sqrt(bmin / t.beta);
-  for (size_t c = 0; c < N; c++)
-    for (int q = 6; q <= 8; q++) uri[9 * c + q] = uri[9 * c + q] * bnorm;
-  return 0;
+  return mdtorus1_impl(pp, tp, uri, gcov, gcon, detg, x1, x2, x3, false);
+}
+
+int raishin_mdtorus1_ik(const raishin_pram* pp, const raishin_torus* tp, double* uri, const double* gcov,
+                        const double* gcon, const double* detg, const double* x1, const double* x2, const double* x3) {
+  return mdtorus1_impl(pp, tp, uri, gcov, gcon, detg, x1, x2, x3, true);
 }
 
 }  // extern "C"

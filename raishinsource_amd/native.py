@@ -9,6 +9,7 @@ The library must be built (raishinsource_amd/_build.py); there is no CPU
 fallback: every entry point raises if the library or a GPU is missing.
 """
 import ctypes as C
+from dataclasses import replace
 import os
 
 import numpy as np
@@ -60,7 +61,7 @@ EXPORTS = (
     "raishin_batch_sync", "raishin_mdshoctub", "raishin_model_set_block_sizes", "raishin_model_get_block_sizes",
     "raishin_model_autotune", "raishin_model_allow_collapse", "raishin_model_stored_rows",
     "raishin_store1", "raishin_restar2", "raishin_output1", "raishin_model_store1", "raishin_model_restar2",
-    "raishin_model_output1", "raishin_model_get_pram",
+    "raishin_model_output1", "raishin_model_get_pram", "raishin_model_create_ik", "raishin_mdtorus1_ik",
 )
 
 TUNE_SLOTS = ("ppm_edges", "sweep_i", "sweep_j", "sweep_k", "fluxct", "update", "recov", "post")
@@ -94,6 +95,8 @@ def lib():
         vp = C.c_void_p
         L.raishin_model_create.restype = vp
         L.raishin_model_create.argtypes = [C.POINTER(PramS)] + [vp] * 19
+        L.raishin_model_create_ik.restype = vp
+        L.raishin_model_create_ik.argtypes = [C.POINTER(PramS)] + [vp] * 19
         L.raishin_model_destroy.argtypes = [vp]
         L.raishin_model_set_state.argtypes = [vp, vp, vp, C.c_double]
         L.raishin_model_get_state.argtypes = [vp, vp, vp, C.POINTER(C.c_double)]
@@ -111,6 +114,8 @@ def lib():
         L.raishin_calchrist.argtypes = [C.POINTER(PramS)] + [vp] * 4
         L.raishin_mdtorus1.argtypes = [C.POINTER(PramS), C.POINTER(TorusS)] + [vp] * 7
         L.raishin_mdtorus1.restype = C.c_int
+        L.raishin_mdtorus1_ik.argtypes = [C.POINTER(PramS), C.POINTER(TorusS)] + [vp] * 7
+        L.raishin_mdtorus1_ik.restype = C.c_int
         L.raishin_mdshoctub.argtypes = [C.POINTER(PramS), C.POINTER(TubeS), vp]
         L.raishin_mdshoctub.restype = C.c_int
         L.raishin_model_set_timing.argtypes = [vp, C.c_int]
@@ -242,9 +247,20 @@ def setup_grid(p: Pram):
     return g
 
 
-def setup_metric(p: Pram, g):
-    """coord1 + caldetg (coord.f90:5,1016) and calchrist (coord.f90:1089)."""
+def one_row(p: Pram, m):
+    """True when the metric arrays m hold the one phi row of an axisymmetric
+    metric (setup_metric(..., row=True)) rather than every cell."""
+    return m["gcov"].shape[3] == 1 and p.resolved().jmax > 1
+
+
+def setup_metric(p: Pram, g, row=False):
+    """coord1 + caldetg (coord.f90:5,1016) and calchrist (coord.f90:1089).
+    row=True: only the phi row j=1 (the metric built here has no phi
+    dependence), shaped (4,4,imax,1,kmax) etc.; Model and setup_torus take it
+    in place of the full arrays and give bit-identical results."""
     p = p.resolved()
+    if row:
+        p = replace(p, jmax=1)
     ni, nj, nk = p.imax, p.jmax, p.kmax
     m = {nm: fzeros(4, 4, ni, nj, nk) for nm in ("gcov", "gcovi", "gcovj", "gcovk", "gcon", "gconi", "gconj", "gconk")}
     for nm in ("detg", "detgi", "detgj", "detgk"):
@@ -264,8 +280,9 @@ def setup_torus(p: Pram, g, m, rin=6.0, rmax=12.0, beta=100.0, acut=0.2, noise=0
     uri = fzeros(9, p.imax, p.jmax, p.kmax)
     t = TorusS(rin, rmax, beta, acut, noise, seed)
     ps = pram_struct(p)
-    rc = lib().raishin_mdtorus1(C.byref(ps), C.byref(t), _p(uri), _p(m["gcov"]), _p(m["gcon"]), _p(m["detg"]),
-                                _p(g["x1"]), _p(g["x2"]), _p(g["x3"]))
+    f = lib().raishin_mdtorus1_ik if one_row(p, m) else lib().raishin_mdtorus1
+    rc = f(C.byref(ps), C.byref(t), _p(uri), _p(m["gcov"]), _p(m["gcon"]), _p(m["detg"]), _p(g["x1"]), _p(g["x2"]),
+           _p(g["x3"]))
     if rc != 0:
         raise RaishinError("torus set-up found no torus cells")
     return uri
@@ -369,7 +386,8 @@ class Model:
         self.p = p.resolved()
         self._ps = pram_struct(self.p)
         L = lib()
-        self.h = L.raishin_model_create(C.byref(self._ps), _p(g["x1"]), _p(g["x2"]), _p(g["x3"]), _p(g["dx1b"]),
+        create = L.raishin_model_create_ik if one_row(self.p, m) else L.raishin_model_create
+        self.h = create(C.byref(self._ps), _p(g["x1"]), _p(g["x2"]), _p(g["x3"]), _p(g["dx1b"]),
                                         _p(g["dx2b"]), _p(g["dx3b"]), _p(m["gcov"]), _p(m["gcon"]),
                                         _p(m["gcovi"]), _p(m["gconi"]), _p(m["gcovj"]), _p(m["gconj"]),
                                         _p(m["gcovk"]), _p(m["gconk"]), _p(m["detg"]), _p(m["detgi"]),

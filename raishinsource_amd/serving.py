@@ -130,7 +130,7 @@ def build(req):
     """Host set-up, device model and initial state of one request."""
     p = req.p.resolved()
     g = native.setup_grid(p)
-    m = native.setup_metric(p, g)
+    m = native.setup_metric(p, g, row=True)  # one phi row: the metrics here are axisymmetric
     if req.kind == "tube":
         uri = native.setup_tube(p, req.init["left"], req.init["right"])
     else:
