@@ -75,7 +75,7 @@ def test_fortran_loop_matches_reference_trajectory(tmp_path):
         assert err <= 1e-11, (v, err)
 
 
-@pytest.mark.parametrize("name,nsteps", [("t2d", 5), ("uppm", 6), ("ump5", 4), ("uweno", 3)])
+@pytest.mark.parametrize("name,nsteps", [("t2d", 5), ("uppm", 6), ("ump5", 4), ("uweno", 3), ("umc", 4)])
 def test_fortran_loop_equals_model_api(name, nsteps, tmp_path):
     """Same kernels, two boundaries: the routine-by-routine calls (dt and
     dt/dx computed by the Fortran host) give the resident-model result bit
