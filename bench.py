@@ -119,7 +119,7 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {args.gpus}): every box here has one MI355X and "
-              "this path has no multi-GPU form; the shard design is described in DESIGN.md.", file=sys.stderr)
+              "this path has no multi-GPU form; sharding the subdomain over several GPUs is designed, not built.", file=sys.stderr)
         return 2
 
     from raishinsource_amd import configs, native
