@@ -259,9 +259,8 @@ def ensemble(args):
         p = replace(configs.C1, name=f"ens{q}", imax=ni + 2 * configs.C1.nm1, kmax=nk + 2 * configs.C1.nm1,
                     akm=a, xmin=None).resolved()
         g = native.setup_grid(p)
-        m = native.setup_metric(p, g, row=True)
-        uri = native.setup_torus(p, g, m, rin=rin, rmax=rin + 6.0, beta=beta, noise=0.02, seed=q)
-        mod = native.Model(p, g, m)
+        mod = native.Model.from_grid(p, g)  # metric tables built on the device
+        uri = native.setup_torus(p, g, mod.metric_row(), rin=rin, rmax=rin + 6.0, beta=beta, noise=0.02, seed=q)
         models.append(mod)
         states.append(uri)
         zones += ni * nk

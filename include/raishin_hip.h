@@ -75,6 +75,22 @@ raishin_model *raishin_model_create_ik(const raishin_pram *p, const double *x1, 
                                        const double *gconj, const double *gcovk, const double *gconk,
                                        const double *detg, const double *detgi, const double *detgj,
                                        const double *detgk, const double *christ);
+/* The same model with no metric arrays at all: the metric at centres and
+ * faces, sqrt(-g) and the Christoffel symbols (metric 303 or 1, the set-up's
+ * metrics) are computed on the device from the grid (grid.f90: x1, x2, x3,
+ * dx*b, and the face positions x1a, x3a).  Only sin and cos run on the host
+ * (per theta), so the tables equal those of raishin_coord1 and
+ * raishin_calchrist bit for bit.  No counterpart in the reference; for
+ * drivers that create many models (ensembles, serving). */
+raishin_model *raishin_model_create_grid(const raishin_pram *p, const double *x1, const double *x2,
+                                         const double *x3, const double *dx1b, const double *dx2b,
+                                         const double *dx3b, const double *x1a, const double *x3a);
+/* Copy one of the model's (i,k) tables to the host (planes of imax*kmax,
+ * i fastest): which 0..3 the metric at centres, i-, j- and k-faces (33
+ * planes: gcov(m,n) at m+4n, gcon(m,n) at 16+m+4n, sqrt(-g) at 32), 4 the
+ * Christoffel symbols (64 planes: Gamma^l_{mn} at l+4m+16n).  0, or -1 with
+ * the error set. */
+int raishin_model_tables(raishin_model *m, int which, double *out);
 void raishin_model_destroy(raishin_model *m);
 
 /* Primitive (uri) and conserved (uu) state, reference layout (9,imax,jmax,kmax).

@@ -82,6 +82,33 @@ raishin_model* raishin_model_create_ik(const raishin_pram* p, const double* x1, 
                       detgi, detgj, detgk, christ, 1);
 }
 
+raishin_model* raishin_model_create_grid(const raishin_pram* p, const double* x1, const double* x2,
+                                         const double* x3, const double* dx1b, const double* dx2b,
+                                         const double* dx3b, const double* x1a, const double* x3a) {
+  if (!p || !x1 || !x2 || !x3 || !x1a || !x3a) { set_error("raishin_model_create_grid: null argument"); return nullptr; }
+  raishin_model* m = new raishin_model();
+  const double* none[4] = {nullptr, nullptr, nullptr, nullptr};
+  if (!build_model(m, p, x1, x2, x3, dx1b, dx2b, dx3b, nullptr, none, none, nullptr, 1, x1a, x3a)) {
+    dfree_all(m);
+    if (m->st) (void)hipStreamDestroy(m->st);
+    delete m;
+    return nullptr;
+  }
+  return m;
+}
+
+int raishin_model_tables(raishin_model* m, int which, double* out) {
+  if (!m || !out || which < 0 || which > 4) { set_error("raishin_model_tables: bad argument"); return -1; }
+  const double* src = which < 4 ? m->met[which] : m->chr;
+  const size_t n = (size_t)(which < 4 ? 33 : 64) * (size_t)m->G.P2;
+  if (hipMemcpyAsync(out, src, n * 8, hipMemcpyDeviceToHost, m->st) != hipSuccess ||
+      hipStreamSynchronize(m->st) != hipSuccess) {
+    set_error("raishin_model_tables: copy failed");
+    return -1;
+  }
+  return 0;
+}
+
 void raishin_model_destroy(raishin_model* m) {
   if (!m) return;
   (void)hipStreamSynchronize(m->st);

@@ -9,6 +9,7 @@
 // Each kernel cites the reference routines it restates.
 #include <type_traits>
 #include "rh_device.hpp"
+#include "rh_metric.hpp"
 #include "rh_kernels.hpp"
 
 namespace rh {
@@ -1514,6 +1515,61 @@ RH_DEV void b_recov_carry(const Geo& G, const FixArgs& A) {
   }
   __syncthreads();
   if (threadIdx.x == 0) A.carry[0] = 0;
+}
+
+// Set-up tables of an axisymmetric metric (metric 303 or 1) on the device,
+// one thread per (i,k): the metric at centres, i-, j- and k-faces (j-face =
+// centre, coord.f90:116,121) with sqrt(-g), and the Christoffel symbols.
+// tr holds per k the host's sin/cos of the folded theta (clamped): at
+// x3(k), x3(k)+1e-5, x3(k)-1e-5 and x3a(k); everything else is arithmetic,
+// so the tables equal the host set-up's bit for bit.  *sym is cleared where
+// Gamma^l_{ab} != Gamma^l_{ba} (the caller sets it to 1 first).
+__global__ void __launch_bounds__(256) k_metric_tables(int ni, int nk, int metric, double akm,
+                                                       const double* __restrict__ x1, const double* __restrict__ x1a,
+                                                       const double* __restrict__ x3, double ph,
+                                                       const double* __restrict__ tr, double* met0, double* met1,
+                                                       double* met2, double* met3, double* chr, int* sym) {
+  const long long P2 = (long long)ni * nk;
+  const long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= P2) return;
+  const int i = (int)(q % ni), k = (int)(q / ni);
+  const double* T = tr + 8LL * k;
+  const double r = x1[i];
+  double g[16], h[16];
+  auto put = [&](double* met) {
+    for (int e = 0; e < 16; e++) { met[e * P2 + q] = g[e]; met[(16 + e) * P2 + q] = h[e]; }
+    met[32 * P2 + q] = rhmet::detg(g);
+  };
+  rhmet::cov(metric, akm, g, r, T[0], T[1]);
+  rhmet::con(metric, akm, h, r, T[0], T[1]);
+  put(met0);
+  put(met2);
+  rhmet::cov(metric, akm, g, x1a[i], T[0], T[1]);
+  rhmet::con(metric, akm, h, x1a[i], T[0], T[1]);
+  put(met1);
+  rhmet::cov(metric, akm, g, r, T[6], T[7]);
+  rhmet::con(metric, akm, h, r, T[6], T[7]);
+  put(met3);
+  const rhmet::ThetaTrig tt{T[0], T[1], T[2], T[3], T[4], T[5]};
+  double ch[64];
+  rhmet::christoffel(metric, akm, r, x3[k], ph, tt, ch);
+  bool s = true;
+  for (int e = 0; e < 64; e++) chr[e * P2 + q] = ch[e];
+  for (int l = 0; l < 4; l++)
+    for (int a = 0; a < 4; a++)
+      for (int b = a + 1; b < 4; b++)
+        s = s && __double_as_longlong(ch[l + 4 * a + 16 * b]) == __double_as_longlong(ch[l + 4 * b + 16 * a]);
+  if (!s) atomicAnd(sym, 0);
+}
+
+hipError_t launch_metric_tables(int ni, int nk, int metric, double akm, const double* x1, const double* x1a,
+                                const double* x3, double ph, const double* tr, double* const met[4], double* chr,
+                                int* sym, hipStream_t st) {
+  const long long P2 = (long long)ni * nk;
+  if (P2 <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_metric_tables, dim3((unsigned)((P2 + 255) / 256)), dim3(256), 0, st, ni, nk, metric, akm, x1,
+                     x1a, x3, ph, tr, met[0], met[1], met[2], met[3], chr, sym);
+  return hipGetLastError();
 }
 
 // caluu1 (caluu.f90:2-126) on every cell: conserved variables from primitives.

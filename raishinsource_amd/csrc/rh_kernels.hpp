@@ -167,6 +167,12 @@ hipError_t launch_sum_counters(unsigned long long* cpart, long long rows, unsign
 hipError_t launch_advance_time(const Geo& G, const unsigned long long* dtbits, double cfl, double* tdt,
                                hipStream_t st);
 // dst <- src (n doubles), stream-ordered
+// The metric and Christoffel (i,k) tables of an axisymmetric metric from the
+// grid, on the device (k_metric_tables); tr: per k the host's sin/cos at
+// x3, x3+1e-5, x3-1e-5 and x3a.
+hipError_t launch_metric_tables(int ni, int nk, int metric, double akm, const double* x1, const double* x1a,
+                                const double* x3, double ph, const double* tr, double* const met[4], double* chr,
+                                int* sym, hipStream_t st);
 hipError_t launch_copy(const Geo& G, double* dst, const double* src, long long n, hipStream_t st);
 hipError_t launch_caluu1(const Geo& G, const double* W, const double* metC, double gam, double* U, hipStream_t st);
 hipError_t launch_aos_to_soa(const double* aos, double* soa, long long N, hipStream_t st);

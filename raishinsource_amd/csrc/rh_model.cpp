@@ -211,13 +211,15 @@ std::vector<double> ppm_coefficients(const std::vector<double>& ds) {
 static bool build_model_(raishin_model* m, const raishin_pram* p, const double* x1, const double* x2,
                          const double* x3, const double* dx1b, const double* dx2b, const double* dx3b,
                          const double* const gcv[4], const double* const gcn[4], const double* const dtg[4],
-                         const double* christ, int metric_rows);
+                         const double* christ, int metric_rows, const double* x1a, const double* x3a);
 
 bool build_model(raishin_model* m, const raishin_pram* p, const double* x1, const double* x2, const double* x3,
                  const double* dx1b, const double* dx2b, const double* dx3b, const double* const gcv[4],
-                 const double* const gcn[4], const double* const dtg[4], const double* christ, int metric_rows) {
+                 const double* const gcn[4], const double* const dtg[4], const double* christ, int metric_rows,
+                 const double* x1a, const double* x3a) {
   const size_t bytes0 = g_dalloc_bytes;
-  const bool ok = build_model_(m, p, x1, x2, x3, dx1b, dx2b, dx3b, gcv, gcn, dtg, christ, metric_rows);
+  const bool ok =
+      build_model_(m, p, x1, x2, x3, dx1b, dx2b, dx3b, gcv, gcn, dtg, christ, metric_rows, x1a, x3a);
   m->dev_bytes = g_dalloc_bytes - bytes0;
   return ok;
 }
@@ -225,7 +227,8 @@ bool build_model(raishin_model* m, const raishin_pram* p, const double* x1, cons
 static bool build_model_(raishin_model* m, const raishin_pram* p, const double* x1, const double* x2,
                          const double* x3, const double* dx1b, const double* dx2b, const double* dx3b,
                          const double* const gcv[4], const double* const gcn[4], const double* const dtg[4],
-                         const double* christ, int metric_rows) {
+                         const double* christ, int metric_rows, const double* x1a,
+                         const double* x3a) {
   m->p = *p;
   if (!check_pram(*p)) return false;
   Geo& G = m->G;
@@ -271,6 +274,34 @@ static bool build_model_(raishin_model* m, const raishin_pram* p, const double* 
   HIPCHK(hipMemsetAsync(m->Wpre, 0, 9 * N * 8, m->st));
   HIPCHK(hipMemsetAsync(m->cflag, 0, N, m->st));
   // metric tables
+  if (!gcv) {
+    // from the grid on the device (k_metric_tables); its inputs are staged in
+    // wgout, which the recovery does not use before the first step
+    if (p->metric != 303 && p->metric != 1) {
+      set_error("metric=%d: the device tables are built for metric 303 and 1", p->metric);
+      return false;
+    }
+    if (!x1a || !x3a) { set_error("raishin_model_create_grid: face positions x1a, x3a missing"); return false; }
+    for (int a = 0; a < 4; a++)
+      if (!dalloc(&m->met[a], 33 * G.P2)) return false;
+    const long long need = 2LL * G.ni + 9LL * G.nk + 1;
+    if (need > N) { set_error("grid too small to stage the metric inputs"); return false; }
+    std::vector<double> tr(8 * (size_t)G.nk);
+    theta_trig_table(*p, x3, x3a, tr.data());
+    double* sc = m->wgout;
+    int* sym = reinterpret_cast<int*>(sc + 2LL * G.ni + 9LL * G.nk);
+    const int one = 1;
+    if (!h2d_sync(sc, x1, G.ni * 8, m->st) || !h2d_sync(sc + G.ni, x1a, G.ni * 8, m->st) ||
+        !h2d_sync(sc + 2LL * G.ni, x3, G.nk * 8, m->st) || !h2d_sync(sc + 2LL * G.ni + G.nk, tr.data(), tr.size() * 8, m->st) ||
+        !h2d_sync(sym, &one, 4, m->st))
+      return false;
+    HIPCHK(launch_metric_tables(G.ni, G.nk, p->metric, p->akm, sc, sc + G.ni, sc + 2LL * G.ni, x2[0],
+                                sc + 2LL * G.ni + G.nk, m->met, m->chr, sym, m->st));
+    int symh = 0;
+    HIPCHK(hipMemcpyAsync(&symh, sym, 4, hipMemcpyDeviceToHost, m->st));
+    HIPCHK(hipStreamSynchronize(m->st));
+    m->chr_sym = symh != 0;
+  } else {
   const char* names[4] = {"centre", "i-face", "j-face", "k-face"};
   std::vector<double> tab;
   for (int a = 0; a < 4; a++) {
@@ -299,6 +330,7 @@ static bool build_model_(raishin_model* m, const raishin_pram* p, const double* 
         for (int b = a + 1; b < 4 && m->chr_sym; b++)
           m->chr_sym = std::memcmp(&ct[(size_t)(l + 4 * a + 16 * b) * G.P2], &ct[(size_t)(l + 4 * b + 16 * a) * G.P2],
                                    (size_t)G.P2 * 8) == 0;
+  }
   }
   // 1D grids; dxs[d] = rec2's dummy dx*b(d) = actual dx*b(d-1) (rec.f90:58), clamped at the ends
   const double* xsrc[3] = {x1, x2, x3};

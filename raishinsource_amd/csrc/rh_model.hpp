@@ -99,10 +99,14 @@ void unpack_row(const rh::Geo& G, const std::vector<double>& in, double* full);
 void resolve_timing(raishin_model* m);
 // metric_rows: the phi rows the metric arrays hold, jmax (the reference's
 // arrays, checked to be independent of phi) or 1 (the row of an
-// axisymmetric metric).
+// axisymmetric metric).  gcv == nullptr: no metric arrays; the tables are
+// computed on the device from the grid (x1a, x3a: face positions).
 bool build_model(raishin_model* m, const raishin_pram* p, const double* x1, const double* x2, const double* x3,
                  const double* dx1b, const double* dx2b, const double* dx3b, const double* const gcv[4],
-                 const double* const gcn[4], const double* const dtg[4], const double* christ, int metric_rows);
+                 const double* const gcn[4], const double* const dtg[4], const double* christ, int metric_rows,
+                 const double* x1a = nullptr, const double* x3a = nullptr);
+// rh_setup.cpp: per k the sin/cos inputs of the metric (8 values per k)
+void theta_trig_table(const raishin_pram& p, const double* x3, const double* x3a, double* out);
 
 // Pieces of one RK stage on explicit buffers (SoA planes on the device).
 // Fluxes: the three direction sweeps and flux-CT from primitives W into F (27 planes).

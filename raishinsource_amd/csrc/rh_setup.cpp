@@ -8,6 +8,7 @@
 
 #include "../../include/raishin_hip.h"
 #include "../../include/raishin_setup.h"
+#include "rh_metric.hpp"
 
 namespace {
 
@@ -18,64 +19,38 @@ inline size_t cell(const raishin_pram& p, int i, int j, int k) {  // 1-based
 }
 #define GM(g, m, n) (g)[(m) + 4 * (n)]
 
-// kshcov1 / kshcon1: coord.f90:535-690
-void kshcov1(double akm, double* g, double rr, double x3) {
+// sin and cos of the folded theta, clamped away from zero, as kshcov1 /
+// kshcon1 take them (coord.f90:535-690)
+void ks_trig(double x3, double& sth, double& cth) {
   const double small = 1.0e-12;
   const double theta = (x3 > PI) ? 2.0 * PI - x3 : x3;
-  double sth = std::sin(theta);
+  sth = std::sin(theta);
   if (std::fabs(sth) < small) sth = (sth > 0.0) ? small : -small;
-  const double s2 = sth * sth;
-  double cth = std::cos(theta);
+  cth = std::cos(theta);
   if (std::fabs(cth) < small) cth = (cth > 0.0) ? small : -small;
-  const double a2 = akm * akm, r2 = rr * rr;
-  const double del = r2 - (2.0 * rr) + a2;
-  const double sig = r2 + a2 * cth * cth;
-  const double aa = ((r2 + a2) * (r2 + a2)) - a2 * del * s2;
-  for (int e = 0; e < 16; e++) g[e] = 0.0;
-  GM(g, 0, 0) = -1.0 + 2.0 * rr / sig;
-  GM(g, 0, 1) = 2.0 * rr / sig;
-  GM(g, 0, 2) = -2.0 * akm * rr * s2 * (1.0 / sig);
-  GM(g, 1, 0) = GM(g, 0, 1);
-  GM(g, 1, 1) = 1.0 + 2.0 * rr / sig;
-  GM(g, 1, 2) = -1.0 * akm * s2 * (1.0 + 2.0 * rr / sig);
-  GM(g, 2, 0) = GM(g, 0, 2);
-  GM(g, 2, 1) = GM(g, 1, 2);
-  GM(g, 2, 2) = aa * s2 * (1.0 / sig);
-  GM(g, 3, 3) = sig;
+}
+rhmet::ThetaTrig theta_trig(double th) {
+  const double delta = 1.0e-5;
+  rhmet::ThetaTrig t;
+  ks_trig(th, t.s, t.c);
+  ks_trig(th + delta, t.sa, t.ca);
+  ks_trig(th - delta, t.sb, t.cb);
+  return t;
 }
 void kshcon1(double akm, double* g, double rr, double x3) {
-  const double small = 1.0e-12;
-  const double theta = (x3 > PI) ? 2.0 * PI - x3 : x3;
-  double sth = std::sin(theta);
-  if (std::fabs(sth) < small) sth = (sth > 0.0) ? small : -small;
-  const double s2 = sth * sth;
-  double cth = std::cos(theta);
-  if (std::fabs(cth) < small) cth = (cth > 0.0) ? small : -small;
-  const double a2 = akm * akm, r2 = rr * rr;
-  const double del = r2 - (2.0 * rr) + a2;
-  const double sig = r2 + a2 * cth * cth;
-  for (int e = 0; e < 16; e++) g[e] = 0.0;
-  GM(g, 0, 0) = -1.0 - 2.0 * rr / sig;
-  GM(g, 0, 1) = 2.0 * rr / sig;
-  GM(g, 1, 0) = GM(g, 0, 1);
-  GM(g, 1, 1) = del / sig;
-  GM(g, 1, 2) = akm * (1.0 / sig);
-  GM(g, 2, 1) = GM(g, 1, 2);
-  GM(g, 2, 2) = 1.0 / (sig * s2);
-  GM(g, 3, 3) = 1.0 / sig;
-}
-// mincov1 / mincon1: coord.f90:149-198
-void minkowski(double* g, double sgn00) {
-  for (int e = 0; e < 16; e++) g[e] = 0.0;
-  GM(g, 0, 0) = sgn00; GM(g, 1, 1) = 1.0; GM(g, 2, 2) = 1.0; GM(g, 3, 3) = 1.0;
+  double s, c;
+  ks_trig(x3, s, c);
+  rhmet::ks_con(akm, g, rr, s, c);
 }
 void metric_cov(const raishin_pram& p, double* g, double r, double th) {
-  if (p.metric == 303) kshcov1(p.akm, g, r, th);
-  else minkowski(g, -1.0);
+  double s = 0.0, c = 0.0;
+  if (p.metric == 303) ks_trig(th, s, c);
+  rhmet::cov(p.metric, p.akm, g, r, s, c);
 }
 void metric_con(const raishin_pram& p, double* g, double r, double th) {
-  if (p.metric == 303) kshcon1(p.akm, g, r, th);
-  else minkowski(g, -1.0);
+  double s = 0.0, c = 0.0;
+  if (p.metric == 303) ks_trig(th, s, c);
+  rhmet::con(p.metric, p.akm, g, r, s, c);
 }
 
 // kercov1: coord.f90:369-450 (Boyer-Lindquist, used by the torus set-up)
@@ -340,6 +315,21 @@ static int mdtorus1_impl(const raishin_pram* pp, const raishin_torus* tp, double
   return 0;
 }
 
+namespace rhm {
+// Per k the sin/cos pairs the set-up's metric takes (folded, clamped): at
+// x3(k), x3(k)+1e-5, x3(k)-1e-5 and x3a(k), 8 values per k.  The device table
+// kernel starts from these, so its tables equal raishin_coord1's and
+// raishin_calchrist's.
+void theta_trig_table(const raishin_pram& p, const double* x3, const double* x3a, double* out) {
+  for (int k = 0; k < p.kmax; k++) {
+    const rhmet::ThetaTrig t = theta_trig(x3[k]);
+    double* o = out + 8 * k;
+    o[0] = t.s; o[1] = t.c; o[2] = t.sa; o[3] = t.ca; o[4] = t.sb; o[5] = t.cb;
+    ks_trig(x3a[k], o[6], o[7]);
+  }
+}
+}  // namespace rhm
+
 extern "C" {
 
 void raishin_grid(const raishin_pram* pp, const raishin_grid_ext* ext, double* x1, double* x2, double* x3,
@@ -417,21 +407,7 @@ void raishin_coord1(const raishin_pram* pp, double* gcov, double* gcovi, double*
   const size_t N = (size_t)p.imax * p.jmax * p.kmax;
   for (int s = 0; s < 4; s++)
 #pragma omp parallel for schedule(static)
-    for (long long cc = 0; cc < (long long)N; cc++) {
-      const size_t c = (size_t)cc;
-      const double* g = &Gs[s][16 * c];
-      double aa, bb, cc, pp_, qq, rr, xx, yy, zz;
-      aa = GM(g, 1, 1); bb = GM(g, 1, 2); cc = GM(g, 1, 3); pp_ = GM(g, 2, 1); qq = GM(g, 2, 2); rr = GM(g, 2, 3);
-      xx = GM(g, 3, 1); yy = GM(g, 3, 2); zz = GM(g, 3, 3);
-      const double t1 = aa * qq * zz + bb * rr * xx + cc * pp_ * yy - cc * qq * xx - bb * pp_ * zz - aa * rr * yy;
-      aa = GM(g, 0, 1); bb = GM(g, 0, 2); cc = GM(g, 0, 3);
-      const double t2 = aa * qq * zz + bb * rr * xx + cc * pp_ * yy - cc * qq * xx - bb * pp_ * zz - aa * rr * yy;
-      pp_ = GM(g, 1, 1); qq = GM(g, 1, 2); rr = GM(g, 1, 3);
-      const double t3 = aa * qq * zz + bb * rr * xx + cc * pp_ * yy - cc * qq * xx - bb * pp_ * zz - aa * rr * yy;
-      xx = GM(g, 2, 1); yy = GM(g, 2, 2); zz = GM(g, 2, 3);
-      const double t4 = aa * qq * zz + bb * rr * xx + cc * pp_ * yy - cc * qq * xx - bb * pp_ * zz - aa * rr * yy;
-      D[s][c] = std::sqrt(-1.0 * (GM(g, 0, 0) * t1 - GM(g, 1, 0) * t2 + GM(g, 2, 0) * t3 - GM(g, 3, 0) * t4));
-    }
+    for (long long cc = 0; cc < (long long)N; cc++) D[s][cc] = rhmet::detg(&Gs[s][16 * cc]);
 }
 
 void raishin_calchrist(const raishin_pram* pp, double* christ, const double* x1, const double* x2, const double* x3) {
@@ -440,39 +416,15 @@ void raishin_calchrist(const raishin_pram* pp, double* christ, const double* x1,
   // phi dependence, and the phi derivative's two metric evaluations are equal
   // (its difference is zero for every phi).
   const raishin_pram& p = *pp;
-  const double delta = 1.0e-5, small = 1.0e-12;
 #pragma omp parallel for schedule(dynamic)
-  for (int k = 1; k <= p.kmax; k++)
-    for (int j = 1; j <= 1; j++)
-      for (int i = 1; i <= p.imax; i++) {
-        const double r = x1[i - 1], ph = x2[j - 1], th = x3[k - 1];
-        const double ria = r + delta, rib = r - delta, pja = ph + delta, pjb = ph - delta, tka = th + delta,
-                     tkb = th - delta;
-        double gia[16], gib[16], gja[16], gjb[16], gka[16], gkb[16], gi[16];
-        metric_cov(p, gia, ria, th); metric_cov(p, gib, rib, th);
-        metric_cov(p, gja, r, th); metric_cov(p, gjb, r, th);  // no phi dependence
-        metric_cov(p, gka, r, tka); metric_cov(p, gkb, r, tkb);
-        metric_con(p, gi, r, th);
-        double dg[4][4][4];  // dg[l][m][n] = d g_lm / dx^n
-        for (int m = 0; m < 4; m++)
-          for (int l = 0; l < 4; l++) {
-            dg[l][m][0] = 0.0;
-            dg[l][m][1] = (GM(gia, l, m) - GM(gib, l, m)) / (ria - rib + small);
-            dg[l][m][2] = (GM(gja, l, m) - GM(gjb, l, m)) / (pja - pjb + small);
-            dg[l][m][3] = (GM(gka, l, m) - GM(gkb, l, m)) / (tka - tkb + small);
-          }
-        double t2[4][4][4];
-        for (int n = 0; n < 4; n++)
-          for (int m = 0; m < 4; m++)
-            for (int l = 0; l < 4; l++) t2[l][m][n] = 0.5 * (dg[m][l][n] + dg[n][l][m] - dg[n][m][l]);
-        double* ch = &christ[64 * cell(p, i, j, k)];
-        for (int n = 0; n < 4; n++)
-          for (int m = 0; m < 4; m++)
-            for (int l = 0; l < 4; l++)
-              ch[l + 4 * m + 16 * n] = GM(gi, l, 0) * t2[0][m][n] + GM(gi, l, 1) * t2[1][m][n] +
-                                       GM(gi, l, 2) * t2[2][m][n] + GM(gi, l, 3) * t2[3][m][n];
-        for (int jj = 2; jj <= p.jmax; jj++) std::memcpy(&christ[64 * cell(p, i, jj, k)], ch, 64 * sizeof(double));
-      }
+  for (int k = 1; k <= p.kmax; k++) {
+    const rhmet::ThetaTrig tt = theta_trig(x3[k - 1]);
+    for (int i = 1; i <= p.imax; i++) {
+      double* ch = &christ[64 * cell(p, i, 1, k)];
+      rhmet::christoffel(p.metric, p.akm, x1[i - 1], x3[k - 1], x2[0], tt, ch);
+      for (int jj = 2; jj <= p.jmax; jj++) std::memcpy(&christ[64 * cell(p, i, jj, k)], ch, 64 * sizeof(double));
+    }
+  }
 }
 
 int raishin_mdtorus1(const raishin_pram* pp, const raishin_torus* tp, double* uri, const double* gcov,

@@ -62,6 +62,7 @@ EXPORTS = (
     "raishin_model_autotune", "raishin_model_allow_collapse", "raishin_model_stored_rows",
     "raishin_store1", "raishin_restar2", "raishin_output1", "raishin_model_store1", "raishin_model_restar2",
     "raishin_model_output1", "raishin_model_get_pram", "raishin_model_create_ik", "raishin_mdtorus1_ik",
+    "raishin_model_create_grid", "raishin_model_tables",
 )
 
 TUNE_SLOTS = ("ppm_edges", "sweep_i", "sweep_j", "sweep_k", "fluxct", "update", "recov", "post")
@@ -97,6 +98,10 @@ def lib():
         L.raishin_model_create.argtypes = [C.POINTER(PramS)] + [vp] * 19
         L.raishin_model_create_ik.restype = vp
         L.raishin_model_create_ik.argtypes = [C.POINTER(PramS)] + [vp] * 19
+        L.raishin_model_create_grid.restype = vp
+        L.raishin_model_create_grid.argtypes = [C.POINTER(PramS)] + [vp] * 8
+        L.raishin_model_tables.argtypes = [vp, C.c_int, vp]
+        L.raishin_model_tables.restype = C.c_int
         L.raishin_model_destroy.argtypes = [vp]
         L.raishin_model_set_state.argtypes = [vp, vp, vp, C.c_double]
         L.raishin_model_get_state.argtypes = [vp, vp, vp, C.POINTER(C.c_double)]
@@ -395,6 +400,40 @@ class Model:
         if not self.h:
             check()
             raise RaishinError("raishin_model_create failed")
+
+    @classmethod
+    def from_grid(cls, p: Pram, g):
+        """A model whose metric and Christoffel tables are computed on the
+        device from the grid (raishin_model_create_grid): no host metric
+        arrays; tables identical to setup_metric's."""
+        self = cls.__new__(cls)
+        self.p = p.resolved()
+        self._ps = pram_struct(self.p)
+        self.h = lib().raishin_model_create_grid(C.byref(self._ps), *[_p(g[k]) for k in (
+            "x1", "x2", "x3", "dx1b", "dx2b", "dx3b", "x1a", "x3a")])
+        if not self.h:
+            check()
+            raise RaishinError("raishin_model_create_grid failed")
+        return self
+
+    def tables(self, which):
+        """(i,k) table `which` (0..3 metric at centres, i-, j-, k-faces: 33
+        planes; 4 Christoffel symbols: 64 planes) as planes x (imax, kmax)."""
+        p = self.p
+        out = np.zeros((33 if which < 4 else 64, p.kmax, p.imax))
+        if lib().raishin_model_tables(self.h, which, out.ctypes.data_as(C.c_void_p)):
+            check()
+        return out.transpose(0, 2, 1)  # (plane, i, k)
+
+    def metric_row(self):
+        """The centre metric as the one phi row setup_torus takes: gcov, gcon
+        (4,4,imax,1,kmax), detg (imax,1,kmax)."""
+        t = self.tables(0)
+        ni, nk = self.p.imax, self.p.kmax
+        gcov = np.asfortranarray(t[0:16].reshape(4, 4, ni, nk, order="F")[:, :, :, None, :])
+        gcon = np.asfortranarray(t[16:32].reshape(4, 4, ni, nk, order="F")[:, :, :, None, :])
+        detg = np.asfortranarray(t[32][:, None, :])
+        return {"gcov": gcov, "gcon": gcon, "detg": detg}
 
     def set_state(self, uri, uu=None, time=0.0):
         lib().raishin_model_set_state(self.h, _p(np.asfortranarray(uri)), _p(None if uu is None else np.asfortranarray(uu)), time)

@@ -130,13 +130,12 @@ def build(req):
     """Host set-up, device model and initial state of one request."""
     p = req.p.resolved()
     g = native.setup_grid(p)
-    m = native.setup_metric(p, g, row=True)  # one phi row: the metrics here are axisymmetric
+    mod = native.Model.from_grid(p, g)  # metric tables built on the device
     if req.kind == "tube":
         uri = native.setup_tube(p, req.init["left"], req.init["right"])
     else:
-        uri = native.setup_torus(p, g, m, rin=req.init["rin"], rmax=req.init["rmax"], beta=req.init["beta"],
-                                 noise=0.02, seed=req.init["seed"])
-    mod = native.Model(p, g, m)
+        uri = native.setup_torus(p, g, mod.metric_row(), rin=req.init["rin"], rmax=req.init["rmax"],
+                                 beta=req.init["beta"], noise=0.02, seed=req.init["seed"])
     mod.set_state(uri)
     mod.set_precision(req.mixed)
     mod.sync()

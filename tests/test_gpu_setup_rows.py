@@ -51,3 +51,42 @@ def test_row_tube_model():
         mod.close()
     assert res[0][2] == res[1][2]
     assert bitdiff(res[0][0], res[1][0]) == 0 and bitdiff(res[0][1], res[1][1]) == 0
+
+
+@pytest.mark.parametrize("name", ["t2d", "uppm", "utube", "c1", "t3d"])
+def test_device_tables_equal_host_set_up(name):
+    """raishin_model_create_grid computes the metric, sqrt(-g) and Christoffel
+    tables on the device; they equal the host set-up's (setup_metric) bit for
+    bit, and the model steps like the one built from the host arrays."""
+    from dataclasses import replace
+    p = configs.ALL[name].resolved()
+    if name == "t2d":
+        p = replace(p, name="t2d_a3", akm=0.3, xmin=None).resolved()
+    g = native.setup_grid(p)
+    m = native.setup_metric(p, g, row=True)
+    dev = native.Model.from_grid(p, g)
+    host = native.Model(p, g, m)
+    ni, nk = p.imax, p.kmax
+    locs = [("gcov", "gcon", "detg"), ("gcovi", "gconi", "detgi"), ("gcovj", "gconj", "detgj"),
+            ("gcovk", "gconk", "detgk")]
+    for a, (gc, gn, dg) in enumerate(locs):
+        td = dev.tables(a)
+        ref = np.concatenate([m[gc].reshape(16, ni, nk, order="F"), m[gn].reshape(16, ni, nk, order="F"),
+                              m[dg].reshape(1, ni, nk, order="F")])
+        assert bitdiff(td, ref) == 0, (name, a)
+        assert bitdiff(host.tables(a), ref) == 0, (name, a)
+    ch = m["christ"].reshape(64, ni, nk, order="F")
+    assert bitdiff(dev.tables(4), ch) == 0 and bitdiff(host.tables(4), ch) == 0
+    if p.metric == 303:
+        uri = native.setup_torus(p, g, dev.metric_row(), seed=4, noise=0.02)
+        assert bitdiff(uri, native.setup_torus(p, g, m, seed=4, noise=0.02)) == 0
+    else:
+        uri = native.setup_tube(p, *native.random_tube_states(np.random.default_rng(4)))
+    for mod in (dev, host):
+        mod.set_state(uri)
+        mod.step(3)
+    (wd, ud, td_), (wh, uh, th) = dev.get_state(), host.get_state()
+    assert td_ == th and bitdiff(wd, wh) == 0 and bitdiff(ud, uh) == 0
+    assert dev.counters() == host.counters() and dev.device_bytes() == host.device_bytes()
+    dev.close()
+    host.close()
