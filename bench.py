@@ -98,6 +98,34 @@ def cpu_baseline(p, phi_cells):
     return nint / el, el, q
 
 
+def cpu_baseline_reference(name):
+    """The reference's own routines (oracle/_ref/<name>: the Fortran sources
+    built with flang -O2, one core) stepped in main.f90:743-1149's order, with
+    the grid, metric and Christoffel symbols from its own grid.f90 / coord1 /
+    calchrist; bnd4 and infchk come from the oracle (bnd.f90 needs mpif.h).
+    The same torus; one warm-up step, then two timed steps.  Returns
+    (zone-updates/s, seconds timed, pram), or None where the build is absent."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    from harness import Oracle, RefStepper, fzeros, grid_and_metric, have_ref  # noqa: E402
+    from raishinsource_amd import native
+
+    if not have_ref(name):
+        return None
+    r, g, m = grid_and_metric(name)
+    q = r.p
+    uri = native.setup_torus(q, g, m, seed=0, noise=0.02)
+    uu = fzeros(9, q.imax, q.jmax, q.kmax)
+    Oracle(q).call("caluu1", uri, uu, m["gcov"], m["gcon"], m["detg"])
+    rs = RefStepper(name, g, m)
+    _, t = rs.step(uu, uri, 0.0)
+    t0 = time.perf_counter()
+    for _ in range(2):
+        _, t = rs.step(uu, uri, t)
+    el = time.perf_counter() - t0
+    nint = (q.imax - 2 * q.nm1) * (q.jmax - 2 * q.nm1) * (q.kmax - 2 * q.nm1)
+    return 2 * nint / el, el, q
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -123,6 +151,14 @@ def main():
         return 2
 
     from raishinsource_amd import configs, native
+
+    # stdout carries the one result line.  Native code that prints (the
+    # reference's Fortran routines of the CPU baseline write to unit 6, and
+    # their runtime may flush at exit) goes to stderr: fd 1 points there for
+    # the whole run, and the result line is written to the saved stdout.
+    sys.stdout.flush()
+    result_fd = os.dup(1)
+    os.dup2(2, 1)
 
     p = configs.ALL[args.config].resolved()
     t_setup = time.perf_counter()
@@ -180,6 +216,17 @@ def main():
                "sample": f"one step (after one warm-up step) of the same torus on {q.imax}x{q.jmax}x{q.kmax} "
                          f"({args.cpu_phi} of the {p.jmax - 2 * p.nm1} interior phi cells), {el_cpu:.1f}s, "
                          "oracle/raishin_oracle.c single-threaded"}
+        ref = cpu_baseline_reference("c2cpu") if p.name == "c2ppm" else None
+        if ref is not None:
+            rv, rel, rq = ref
+            print(f"[bench] cpu baseline: reference build {rv:.3g} zone-updates/s ({rel:.1f}s)", file=sys.stderr,
+                  flush=True)
+            cpu = {"value": rv, "unit": "zone-updates/s", "cores": 1, "kind": "reference",
+                   "sample": f"two steps (after one warm-up step) of the same torus on {rq.imax}x{rq.jmax}x{rq.kmax} "
+                             f"({rq.jmax - 2 * rq.nm1} of the {p.jmax - 2 * p.nm1} interior phi cells), "
+                             f"{rel:.1f}s: the reference's Fortran routines (flang -O2, oracle/_ref/c2cpu) in "
+                             "main.f90's step order, bnd4/infchk from the oracle (bnd.f90 needs mpif.h)",
+                   "port": cpu}
 
     line = {
         "metric": "zone-updates/s on 1 GPU, mixed vs FP64 at L1 err<=1e-5; %HBM peak; p99 latency",
@@ -227,7 +274,8 @@ def main():
         line["ensemble"] = ensemble(args)
     if args.serve_rates:
         line["serving"] = serve(args)
-    print(json.dumps(line))
+    sys.stdout.flush()
+    os.write(result_fd, (json.dumps(line) + "\n").encode())
     return 0
 
 

@@ -90,6 +90,9 @@ C1 = Pram("c1", 128, 11, 128, irec=5, irkt=2, cfl=0.4)
 # one GPU's 3D subdomain, MP5 + RK3
 C2 = Pram("c2", 256, 128, 128, irec=6, irkt=3, cfl=0.5)
 C2_PPM = replace(C2, name="c2ppm", irec=5, jmax=128)
+# the CPU baseline's sample of c2ppm: the same r-theta grid and torus with 8
+# interior phi cells (a bounded run of the reference build on one core)
+C2_CPU = replace(C2_PPM, name="c2cpu", jmax=8 + 2 * 5)
 C2_WENO = replace(C2, name="c2weno", irec=7)
 
 # small grids used for unit-level golden vectors and short trajectories
@@ -135,5 +138,5 @@ U_RC2 = Pram("urc2", 16, 10, 18, irec=1, irkt=3, cfl=0.3, icha=1, iwvec=2)
 TUBE = tube("tube", 1024)
 U_TUBE = tube("utube", 64)
 
-ALL = {p.name: p for p in (C1, C2, C2_PPM, C2_WENO, U_PPM, U_MP5, U_WENO, T2D, T3D, G_PPM, G_MP5, G_WENO, TUBE,
+ALL = {p.name: p for p in (C1, C2, C2_PPM, C2_CPU, C2_WENO, U_PPM, U_MP5, U_WENO, T2D, T3D, G_PPM, G_MP5, G_WENO, TUBE,
                            U_TUBE, U_MC, T2D_MC, U_LEIS, T2D_GAM, T2D_RC2, U_RC2)}
