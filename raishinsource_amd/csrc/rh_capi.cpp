@@ -122,11 +122,25 @@ void raishin_model_destroy(raishin_model* m) {
 }
 
 void raishin_model_set_state(raishin_model* m, const double* uri, const double* uu, double time) {
+  (void)rhm::model_set_state(m, uri, uu, time);
+}
+
+void raishin_model_get_state(raishin_model* m, double* uri, double* uu, double* time) {
+  (void)rhm::model_get_state(m, uri, uu, time);
+}
+
+}  // extern "C"
+
+namespace rhm {
+
+// raishin_model_set_state / get_state with their outcome: false when this
+// call failed (the error is set), whatever error an earlier call left.
+bool model_set_state(raishin_model* m, const double* uri, const double* uu, double time) {
   // one stored phi row when every row of the state is the same (and stays
   // so: periodic phi with one interior cell, r and theta not periodic)
   const int j0 = m->G.nm1 + 1;
   const bool collapse = collapsible(m) && rows_equal(m->G, uri, j0) && (!uu || rows_equal(m->G, uu, j0));
-  HIPCHK_VOID(hipStreamSynchronize(m->st));
+  HIPCHK(hipStreamSynchronize(m->st));
   set_rows(m, collapse);
   const long long N = m->G.N;
   std::vector<double> packed;
@@ -154,10 +168,10 @@ void raishin_model_set_state(raishin_model* m, const double* uri, const double* 
     HIPCHK(hipStreamSynchronize(m->st));
     return true;
   };
-  (void)go();
+  return go();
 }
 
-void raishin_model_get_state(raishin_model* m, double* uri, double* uu, double* time) {
+bool model_get_state(raishin_model* m, double* uri, double* uu, double* time) {
   const long long N = m->G.N;
   std::vector<double> packed;
   // a collapsed model's row goes to every phi row of the host array
@@ -186,8 +200,12 @@ void raishin_model_get_state(raishin_model* m, double* uri, double* uu, double* 
     }
     return true;
   };
-  (void)go();
+  return go();
 }
+
+}  // namespace rhm
+
+extern "C" {
 
 void raishin_model_step(raishin_model* m, int nsteps, double* time, double* dt_last) {
   for (int s = 0; s < nsteps; s++)

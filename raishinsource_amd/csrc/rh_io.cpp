@@ -17,6 +17,9 @@
 
 namespace rhm {
 void set_error(const char* fmt, ...);  // rh_model.cpp: the sticky error
+// rh_capi.cpp: raishin_model_get_state / set_state reporting this call's outcome
+bool model_get_state(raishin_model* m, double* uri, double* uu, double* time);
+bool model_set_state(raishin_model* m, const double* uri, const double* uu, double time);
 }
 using rhm::set_error;
 
@@ -289,9 +292,7 @@ int raishin_model_store1(raishin_model* m, const char* path, const double* u0, c
   const long long n = (long long)p.imax * p.jmax * p.kmax;
   std::vector<double> uri(n * NV), uu(n * NV);
   double time = 0.0;
-  char e[8];
-  raishin_model_get_state(m, uri.data(), uu.data(), &time);
-  if (raishin_last_error(e, sizeof e)) return -1;
+  if (!rhm::model_get_state(m, uri.data(), uu.data(), &time)) return -1;
   return write_restart(restart_name(path, myrank), uu.data(), u0 ? u0 : uu.data(), uri.data(),
                        uri0 ? uri0 : uri.data(), it0, ih0, time, nd, myrank, n)
              ? 0
@@ -307,9 +308,7 @@ int raishin_model_restar2(raishin_model* m, const char* path, double* u0, double
   double time = 0.0;
   if (!read_restart(restart_name(path, myrank), uu.data(), u0, uri.data(), uri0, it0, ih0, &time, nd, myrank, n))
     return -1;
-  char e[8];
-  raishin_model_set_state(m, uri.data(), uu.data(), time);
-  return raishin_last_error(e, sizeof e) ? -1 : 0;
+  return rhm::model_set_state(m, uri.data(), uu.data(), time) ? 0 : -1;
 }
 
 int raishin_model_output1(raishin_model* m, const char* path, const double* detg, const double* x1,
@@ -319,9 +318,7 @@ int raishin_model_output1(raishin_model* m, const char* path, const double* detg
   const long long n = (long long)p.imax * p.jmax * p.kmax;
   std::vector<double> uri(n * NV), uu(n * NV);
   double time = 0.0;
-  char e[8];
-  raishin_model_get_state(m, uri.data(), uu.data(), &time);
-  if (raishin_last_error(e, sizeof e)) return -1;
+  if (!rhm::model_get_state(m, uri.data(), uu.data(), &time)) return -1;
   const int lo[3] = {1, 1, 1}, hi[3] = {p.imax, p.jmax, p.kmax};
   return write_output(output_name(path, myrank, nd), uri.data(), detg, x1, x2, x3, time, myrank, npe, p.imax,
                       p.jmax, p.kmax, lo, hi)

@@ -97,6 +97,10 @@ void set_rows(raishin_model* m, bool collapse);
 void pack_row(const rh::Geo& G, const double* full, std::vector<double>& out);
 void unpack_row(const rh::Geo& G, const std::vector<double>& in, double* full);
 void resolve_timing(raishin_model* m);
+// raishin_model_set_state / get_state reporting this call's outcome (false:
+// failed, error set), for callers inside the library
+bool model_set_state(raishin_model* m, const double* uri, const double* uu, double time);
+bool model_get_state(raishin_model* m, double* uri, double* uu, double* time);
 // metric_rows: the phi rows the metric arrays hold, jmax (the reference's
 // arrays, checked to be independent of phi) or 1 (the row of an
 // axisymmetric metric).  gcv == nullptr: no metric arrays; the tables are

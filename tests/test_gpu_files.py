@@ -110,3 +110,26 @@ def test_fortran_loop_restarts_from_its_own_files(tmp_path):
     assert (split / "structr0000-0001.outdat").read_bytes() == (split / "host.outdat").read_bytes()
     t, w, u = run_fortran(p, 4, split, 3)          # restar2, 4 more steps
     assert t == t7 and bitdiff(w, w7) == 0 and bitdiff(u, u7) == 0
+
+
+def test_checkpoint_succeeds_with_an_earlier_error_pending(tmp_path):
+    """The model entry points report their own outcome: an unrelated sticky
+    error left by an earlier call does not make a good checkpoint fail."""
+    import ctypes as C
+    p = configs.U_PPM.resolved()
+    a, g, m = fresh(p)
+    a.set_state(native.setup_torus(p, g, m, seed=1, noise=0.02))
+    b, _, _ = fresh(p)
+    lib = native.lib()
+    lib.raishin_clear_error()
+    assert lib.raishin_model_tables(a.h, 99, None) == -1  # leaves "bad argument" pending
+    assert lib.raishin_model_store1(a.h, str(tmp_path / "r.outdat").encode(), None, None, 1, 0, 0, 0) == 0
+    assert lib.raishin_model_restar2(b.h, str(tmp_path / "r.outdat").encode(), None, None, None, None, None, 0) == 0
+    buf = C.create_string_buffer(256)
+    assert lib.raishin_last_error(buf, 256) == 1 and b"bad argument" in buf.value
+    lib.raishin_clear_error()
+    wa, ua, ta = a.get_state()
+    wb, ub, tb = b.get_state()
+    assert ta == tb and bitdiff(wa, wb) == 0 and bitdiff(ua, ub) == 0
+    a.close()
+    b.close()
