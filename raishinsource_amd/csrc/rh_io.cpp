@@ -254,6 +254,11 @@ void raishin_store1(const char* path, const double* uu, const double* u0, const 
                     const int* myrank, const int* is1, const int* ie1, const int* js1, const int* je1,
                     const int* ks1, const int* ke1) {
   (void)npe;
+  if (!uu || !u0 || !uri || !uri0 || !it0 || !ih0 || !time || !nd || !myrank || !is1 || !ie1 || !js1 || !je1 ||
+      !ks1 || !ke1) {
+    set_error("raishin_store1: null argument");
+    return;
+  }
   long long ni, nj, nk;
   if (!extents(is1, ie1, js1, je1, ks1, ke1, ni, nj, nk)) return;
   write_restart(restart_name(path, *myrank), uu, u0, uri, uri0, *it0, *ih0, *time, *nd, *myrank, ni * nj * nk);
@@ -263,6 +268,10 @@ void raishin_restar2(const char* path, double* uu, double* u0, double* uri, doub
                      double* time, int* nd, const int* npe, const int* myrank, const int* is1, const int* ie1,
                      const int* js1, const int* je1, const int* ks1, const int* ke1) {
   (void)npe;
+  if (!uu || !uri || !myrank || !is1 || !ie1 || !js1 || !je1 || !ks1 || !ke1) {
+    set_error("raishin_restar2: null argument");
+    return;
+  }
   long long ni, nj, nk;
   if (!extents(is1, ie1, js1, je1, ks1, ke1, ni, nj, nk)) return;
   read_restart(restart_name(path, *myrank), uu, u0, uri, uri0, it0, ih0, time, nd, *myrank, ni * nj * nk);
@@ -274,6 +283,11 @@ void raishin_output1(const char* path, const double* uri, const double* detg, co
                      const int* is1, const int* ie1, const int* js1, const int* je1, const int* ks1,
                      const int* ke1) {
   (void)nm1;
+  if (!uri || !detg || !x1 || !x2 || !x3 || !time || !nd || !npe || !myrank || !myranki || !myrankj || !myrankk ||
+      !is1 || !ie1 || !js1 || !je1 || !ks1 || !ke1) {
+    set_error("raishin_output1: null argument");
+    return;
+  }
   long long ni, nj, nk;
   if (!extents(is1, ie1, js1, je1, ks1, ke1, ni, nj, nk)) return;
   if (*myranki != 0 || *myrankj != 0 || *myrankk != 0) {
@@ -287,6 +301,7 @@ void raishin_output1(const char* path, const double* uri, const double* detg, co
 
 int raishin_model_store1(raishin_model* m, const char* path, const double* u0, const double* uri0, int it0,
                          int ih0, int nd, int myrank) {
+  if (!m) { set_error("raishin_model_store1: null model"); return -1; }
   raishin_pram p;
   raishin_model_get_pram(m, &p);
   const long long n = (long long)p.imax * p.jmax * p.kmax;
@@ -301,6 +316,7 @@ int raishin_model_store1(raishin_model* m, const char* path, const double* u0, c
 
 int raishin_model_restar2(raishin_model* m, const char* path, double* u0, double* uri0, int* it0, int* ih0,
                           int* nd, int myrank) {
+  if (!m) { set_error("raishin_model_restar2: null model"); return -1; }
   raishin_pram p;
   raishin_model_get_pram(m, &p);
   const long long n = (long long)p.imax * p.jmax * p.kmax;
@@ -313,6 +329,7 @@ int raishin_model_restar2(raishin_model* m, const char* path, double* u0, double
 
 int raishin_model_output1(raishin_model* m, const char* path, const double* detg, const double* x1,
                           const double* x2, const double* x3, int nd, int myrank, int npe) {
+  if (!m || !detg || !x1 || !x2 || !x3) { set_error("raishin_model_output1: null argument"); return -1; }
   raishin_pram p;
   raishin_model_get_pram(m, &p);
   const long long n = (long long)p.imax * p.jmax * p.kmax;

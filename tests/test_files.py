@@ -126,3 +126,17 @@ def test_bad_restart_files_are_rejected(tmp_path):
         native.restar2(tmp_path / "bad.outdat", SHAPE)
     with pytest.raises(native.RaishinError, match="open"):
         native.restar2(tmp_path / "missing.outdat", SHAPE)
+
+
+def test_null_arguments_are_errors():
+    import ctypes as C
+    lib = native.lib()
+    lib.raishin_clear_error()
+    one = C.byref(C.c_int(1))
+    lib.raishin_store1(None, None, None, None, None, one, one, C.byref(C.c_double(0)), one, one, one, one, one, one,
+                       one, one, one)
+    buf = C.create_string_buffer(256)
+    assert lib.raishin_last_error(buf, 256) == 1 and b"null argument" in buf.value
+    lib.raishin_clear_error()
+    assert lib.raishin_model_store1(None, None, None, None, 0, 0, 0, 0) == -1
+    lib.raishin_clear_error()
