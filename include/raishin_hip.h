@@ -140,6 +140,10 @@ void raishin_model_set_precision(raishin_model *m, int mode);
  * set_timing(1) clears the totals and starts recording; get_timing returns
  * the number of kinds and fills total milliseconds and launch counts. */
 void raishin_model_set_timing(raishin_model *m, int on);
+/* As set_timing, for the kinds whose bit (1 << kind) is set in mask: the
+ * events between launches cost a little, so a measurement that needs one
+ * kind's launch times times that kind alone. */
+void raishin_model_set_timing_mask(raishin_model *m, unsigned mask);
 
 /* One RK stage of a step (stage 1 includes the CFL reduction), exposing its
  * intermediates in reference layouts when the pointers are non-NULL:

@@ -95,8 +95,8 @@ bool make_plan(raishin_batch* b) {
   b->at_gen.resize(n);
   for (size_t q = 0; q < n; q++) {
     raishin_model* m = b->ms[q];
-    const bool timing = m->timing;
-    m->timing = false;
+    const unsigned timing = m->timing;
+    m->timing = 0;
     bool ok = true;
     for (int h = 0; h < 2 && ok; h++) {
       b->at[h][q] = ptrs_of(m);

@@ -234,8 +234,8 @@ namespace {
 bool capture_pair(raishin_model* m) {
   if (m->gexec) { (void)hipGraphExecDestroy(m->gexec); m->gexec = nullptr; }
   if (m->graph) { (void)hipGraphDestroy(m->graph); m->graph = nullptr; }
-  const bool timing = m->timing;
-  m->timing = false;
+  const unsigned timing = m->timing;
+  m->timing = 0;
   HIPCHK(hipStreamBeginCapture(m->st, hipStreamCaptureModeThreadLocal));
   const double* W0 = m->W;
   const double* U0 = m->Ucur;
@@ -369,9 +369,13 @@ void raishin_model_set_precision(raishin_model* m, int mode) {
 }
 
 void raishin_model_set_timing(raishin_model* m, int on) {
+  raishin_model_set_timing_mask(m, on ? (1u << K_NKINDS) - 1u : 0u);
+}
+
+void raishin_model_set_timing_mask(raishin_model* m, unsigned mask) {
   resolve_timing(m);
   for (int k = 0; k < K_NKINDS; k++) { m->ev_ms[k] = 0.0; m->ev_n[k] = 0; }
-  m->timing = on != 0;
+  m->timing = mask & ((1u << K_NKINDS) - 1u);
 }
 
 int raishin_model_get_timing(raishin_model* m, double* ms, long long* launches, int nkinds) {
@@ -412,7 +416,7 @@ int raishin_model_autotune(raishin_model* m, const double* uri, const double* uu
   double best_ms[TS_N];
   int best[TS_N];
   for (int q = 0; q < TS_N; q++) { best_ms[q] = 1e300; best[q] = 256; }
-  const bool timing = m->timing;
+  const unsigned timing = m->timing;
   for (int cand : TUNE_CANDIDATES) {
     int bs[TS_N];
     for (int q = 0; q < TS_N; q++) bs[q] = cand;
@@ -436,7 +440,7 @@ int raishin_model_autotune(raishin_model* m, const double* uri, const double* uu
   // start again from the given state, as if no step had been taken
   raishin_model_set_state(m, uri, uu, time);
   raishin_model_reset_counters(m);
-  raishin_model_set_timing(m, timing ? 1 : 0);
+  raishin_model_set_timing_mask(m, timing);
   return 0;
 }
 

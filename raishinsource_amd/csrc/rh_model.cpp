@@ -135,7 +135,7 @@ struct TimedLaunch {
   int kind;
   hipEvent_t b = nullptr, e = nullptr;
   TimedLaunch(raishin_model* m_, int k) : m(m_), kind(k) {
-    if (!m->timing) return;
+    if (!((m->timing >> kind) & 1u)) return;
     auto& v = m->ev[kind];
     if (m->ev_used[kind] == v.size()) {
       hipEvent_t x, y;

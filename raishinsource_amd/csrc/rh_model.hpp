@@ -23,8 +23,9 @@ enum { K_SWEEP1 = 0, K_SWEEP2, K_SWEEP3, K_FLUXCT, K_UPDATE, K_CARRY, K_POST, K_
 
 struct raishin_model {
   raishin_pram p;
-  // per-kind HIP event pairs recorded around launches; resolved on read
-  bool timing = false;
+  // per-kind HIP event pairs recorded around launches of the kinds whose bit
+  // is set in timing (0: none); resolved on read
+  unsigned timing = 0;
   std::vector<std::pair<hipEvent_t, hipEvent_t>> ev[K_NKINDS];
   size_t ev_used[K_NKINDS] = {0};
   double ev_ms[K_NKINDS] = {0};

@@ -62,7 +62,7 @@ EXPORTS = (
     "raishin_model_autotune", "raishin_model_allow_collapse", "raishin_model_stored_rows",
     "raishin_store1", "raishin_restar2", "raishin_output1", "raishin_model_store1", "raishin_model_restar2",
     "raishin_model_output1", "raishin_model_get_pram", "raishin_model_create_ik", "raishin_mdtorus1_ik",
-    "raishin_model_create_grid", "raishin_model_tables",
+    "raishin_model_create_grid", "raishin_model_tables", "raishin_model_set_timing_mask",
 )
 
 TUNE_SLOTS = ("ppm_edges", "sweep_i", "sweep_j", "sweep_k", "fluxct", "update", "recov", "post")
@@ -124,6 +124,7 @@ def lib():
         L.raishin_mdshoctub.argtypes = [C.POINTER(PramS), C.POINTER(TubeS), vp]
         L.raishin_mdshoctub.restype = C.c_int
         L.raishin_model_set_timing.argtypes = [vp, C.c_int]
+        L.raishin_model_set_timing_mask.argtypes = [vp, C.c_uint]
         L.raishin_model_get_timing.argtypes = [vp, C.POINTER(C.c_double), C.POINTER(C.c_longlong), C.c_int]
         L.raishin_model_get_timing.restype = C.c_int
         L.raishin_model_debug_stage.argtypes = [vp, C.c_int, vp, vp]
@@ -503,8 +504,13 @@ class Model:
         lib().raishin_model_set_precision(self.h, 1 if mixed else 0)
         check()
 
-    def set_timing(self, on=True):
-        lib().raishin_model_set_timing(self.h, 1 if on else 0)
+    def set_timing(self, on=True, kinds=None):
+        """Time every kernel kind (on), none, or only the named `kinds`."""
+        if kinds is not None:
+            mask = sum(1 << KERNEL_KINDS.index(k) for k in kinds)
+            lib().raishin_model_set_timing_mask(self.h, mask)
+        else:
+            lib().raishin_model_set_timing(self.h, 1 if on else 0)
         check()
 
     def get_timing(self):
