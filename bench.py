@@ -511,15 +511,19 @@ def mixed_precision(p, g, m, uri, args, s_per_step_fp64):
     model.set_state(uri)
     model.set_precision(True)
     sizes = model.block_sizes() if args.no_autotune else model.autotune(uri, steps=2)
+    # as the FP64 run: the warm-up steps time every kernel kind, the timed
+    # steps carry no events
+    model.set_timing(True)
     for _ in range(args.warmup):
         model.step(1, sync=False)
-    model.set_timing(True)
+    model.sync()
+    kt = model.get_timing()
+    model.set_timing(False)
     model.sync()
     t0 = time.perf_counter()
     model.step(args.steps, sync=False)
     model.sync()
     el = time.perf_counter() - t0
-    kt = model.get_timing()
     model.close()
     nint = (p.imax - 2 * p.nm1) * (p.jmax - 2 * p.nm1) * (p.kmax - 2 * p.nm1)
     print(f"[bench] mixed: timed {args.steps} steps: {el:.3f}s", file=sys.stderr, flush=True)
