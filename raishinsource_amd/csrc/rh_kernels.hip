@@ -58,6 +58,37 @@ RH_DEV bool box_index_ikj(long long t, int ilo, int ihi, int jlo, int jhi, int k
   return true;
 }
 
+// Waves along the D axis (D = 2 or 3): a lane's i within a 64-wide chunk
+// of the row, consecutive waves on consecutive j (D=2) or k (D=3) rows, so
+// the cell's index along D is the same in every lane of a wave.  Covers
+// 64*ceil(ci/64)*cj*ck indices; lanes past the row's end return false.
+template <int D>
+RH_DEV bool box_index_wave(long long t, int ilo, int ihi, int jlo, int jhi, int klo, int khi, int& i, int& j,
+                           int& k) {
+  const long long ci = ihi - ilo + 1, cj = jhi - jlo + 1, ck = khi - klo + 1;
+  if (ci <= 0 || cj <= 0 || ck <= 0) return false;
+  const long long nic = (ci + 63) / 64;
+  const long long w = t >> 6;
+  long long ic;
+  if (D == 2) {
+    j = jlo + (int)(w % cj);
+    const long long r = w / cj;
+    ic = r % nic;
+    const long long kk = r / nic;
+    if (kk >= ck) return false;
+    k = klo + (int)kk;
+  } else {
+    k = klo + (int)(w % ck);
+    const long long r = w / ck;
+    ic = r % nic;
+    const long long jj = r / nic;
+    if (jj >= cj) return false;
+    j = jlo + (int)jj;
+  }
+  i = ilo + (int)(ic * 64 + (t & 63));
+  return i <= ihi;
+}
+
 // Counter slots (raishin_counters field order).
 enum { C_FALLBACK = 0, C_FAIL, C_NEGRHO, C_NEGP, C_VSQGT1, C_DET0, C_WGCARRY, C_ATM, C_PMMSCALE, C_PMMFLOOR,
        C_GUARD, C_NCOUNT };
@@ -257,13 +288,15 @@ RH_DEV void b_ppm_edges(const Geo& G, const EdgeArgs& A, long long blk) {
   const long long t = blk * blockDim.x + threadIdx.x;
   const int hi = G.ni - G.nm0 + (D == 1), hj = jhi(G, G.nj - G.nm0 + (D == 2)), hk = G.nk - G.nm0 + (D == 3);
   const int lj = jlo(G, G.nm0);
-  const bool in = (D == 3) ? box_index_ikj(t, G.nm0, hi, lj, hj, G.nm0, hk, i, j, k)
-                           : box_index(t, G.nm0, hi, lj, hj, G.nm0, hk, i, j, k);
+  const bool in = (D == 1) ? box_index(t, G.nm0, hi, lj, hj, G.nm0, hk, i, j, k)
+                           : box_index_wave<D>(t, G.nm0, hi, lj, hj, G.nm0, hk, i, j, k);
   if (!in) return;
   const long long N = G.N;
   const long long s = (D == 1) ? 1LL : (D == 2) ? jstride(G) : kstride(G);
   const long long c = cidx(i, j, k, G);
-  const int p = (D == 1) ? i : (D == 2) ? j : k;
+  // along j and k every lane of a wave has the same index along D: its
+  // coefficients and widths are loaded once per wave, into scalar registers
+  const int p = (D == 1) ? i : __builtin_amdgcn_readfirstlane((D == 2) ? j : k);
   const long long npc = ((D == 1) ? G.ni : (D == 2) ? G.nj : G.nk) + 2;
   PpmCoef<R> kc;
   ppm_coef<R>(pc, npc, p, kc);
@@ -1781,8 +1814,10 @@ bool one_group_per_model(int kind) { return kind == LK_CARRY || kind == LK_CFL_F
 
 hipError_t launch_edges(const Geo& G, int dir, int irec, const double* W, const double* dxs, const double* xs,
                         const double* tab, double gam, void* E, bool mixed, hipStream_t st) {
-  const long long n = box(G.nm0, G.ni - G.nm0 + (dir == 1)) * jbox(G, G.nm0, G.nj - G.nm0 + (dir == 2)) *
-                      box(G.nm0, G.nk - G.nm0 + (dir == 3));
+  const long long ci = box(G.nm0, G.ni - G.nm0 + (dir == 1));
+  // PPM along j and k: whole 64-lane rows (box_index_wave)
+  const long long ciw = (irec == 5 && dir != 1) ? 64 * ((ci + 63) / 64) : ci;
+  const long long n = ciw * jbox(G, G.nm0, G.nj - G.nm0 + (dir == 2)) * box(G.nm0, G.nk - G.nm0 + (dir == 3));
   EdgeArgs A{W, dxs, xs, tab, gam, E};
   const int bt = slot_threads(G, TS_EDGES);
   const unsigned nb = nblk(n, bt);
