@@ -7,6 +7,25 @@
  *   MAX(a,b) = a>b ? a : b ; MIN(a,b) = a<b ? a : b  (flang lowering)
  *   SIGN(1d0,x) = copysign(1,x) ; default-real literals are float.
  * Citations are reference file:line.
+ *
+ * Used only by the tests, the smoke check and the benchmark's CPU-baseline
+ * leg, as the checker; the library never links it.
+ *
+ * Pinning (what fixes each function's results):
+ *   - against the reference's own routines, compiled from their sources with
+ *     flang into oracle/_ref, bit for bit: rec2 with PPM, MC, MP5 and WENO5;
+ *     calcha with icha 0, 1 and 2; hll, ct, caltenr/calsf, the rk* and
+ *     rk*ads* routines, recov1a and recov2, atmosphere1a, pminmax2b, caluu1,
+ *     cdtcfl4, and the composed step in main.f90's order
+ *     (tests/test_reference_live.py), and the golden vectors and 10- and
+ *     100-step trajectories that build wrote (tests/test_oracle_golden.py);
+ *   - bnd4 and infchk (bnd.f90 needs mpif.h and is not built): by brute force
+ *     over every boundary code on tiny grids (tests/test_oracle_invariants.py);
+ *   - and by mathematics where no reference output reaches: exactness of the
+ *     reconstructions on linear data, HLLE consistency, the CT corner
+ *     divergence, the recovery round trips, the static-gas CFL, the
+ *     wave-speed identities, the Kerr-Schild identities.
+ * No function here is left unpinned.
  */
 #include "raishin_oracle.h"
 #include <math.h>
