@@ -1442,34 +1442,53 @@ RH_DEV void b_cfl(const Geo& G, const CflArgs& A, long long blk) {
 // lies in the last block that has one; that block's minimum after it and the
 // full minima of every later block make the result.
 RH_DEV void b_cfl_finish(const CflArgs& A) {
+  // the four accumulators together: two passes over the per-block results
+  // (last NaN, then the minimum after it); max and min do not depend on the
+  // order, so the result is the sequential fold's
   const int nblocks = A.nblocks;
-  __shared__ long long s_l[16];
-  __shared__ double s_m[16];
+  __shared__ long long s_l[16][4];
+  __shared__ double s_m[16][4];
+  __shared__ long long s_B[4];
   const int lane = threadIdx.x % 64, wid = threadIdx.x / 64, nw = blockDim.x / 64;
+  long long bg[4] = {-1, -1, -1, -1};
+  for (int b = threadIdx.x; b < nblocks; b += blockDim.x)
+#pragma unroll
+    for (int a = 0; a < 4; a++)
+      if (A.blast[4 * b + a] >= 0) bg[a] = b > bg[a] ? b : bg[a];
+#pragma unroll
   for (int a = 0; a < 4; a++) {
-    long long bg = -1;
-    for (int b = threadIdx.x; b < nblocks; b += blockDim.x)
-      if (A.blast[4 * b + a] >= 0) bg = b > bg ? b : bg;
-    bg = wave_max_ll(bg);
-    if (lane == 0) s_l[wid] = bg;
-    __syncthreads();
-    long long B = s_l[0];
-    for (int w = 1; w < nw; w++) B = s_l[w] > B ? s_l[w] : B;
-    double m = __builtin_inf();
-    for (int b = threadIdx.x; b < nblocks; b += blockDim.x)
-      if (b >= B) { const double v = A.bmin[4 * b + a]; m = (v < m) ? v : m; }
-    m = wave_min(m);
-    if (lane == 0) s_m[wid] = m;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-      double M = s_m[0];
-      for (int w = 1; w < nw; w++) M = (s_m[w] < M) ? s_m[w] : M;
-      double r;
-      if (B < 0) r = fminf_(1.0, M);                              // no NaN: MIN over 1.0 and all
-      else r = (M == __builtin_inf()) ? __builtin_nan("") : M;  // a trailing NaN stays
-      A.dtbits[a] = (unsigned long long)__double_as_longlong(r);
-    }
-    __syncthreads();
+    const long long v = wave_max_ll(bg[a]);
+    if (lane == 0) s_l[wid][a] = v;
+  }
+  __syncthreads();
+  if (threadIdx.x < 4) {
+    long long B = s_l[0][threadIdx.x];
+    for (int w = 1; w < nw; w++) B = s_l[w][threadIdx.x] > B ? s_l[w][threadIdx.x] : B;
+    s_B[threadIdx.x] = B;
+  }
+  __syncthreads();
+  long long B[4];
+  double m[4];
+#pragma unroll
+  for (int a = 0; a < 4; a++) { B[a] = s_B[a]; m[a] = __builtin_inf(); }
+  for (int b = threadIdx.x; b < nblocks; b += blockDim.x)
+#pragma unroll
+    for (int a = 0; a < 4; a++)
+      if (b >= B[a]) { const double v = A.bmin[4 * b + a]; m[a] = (v < m[a]) ? v : m[a]; }
+#pragma unroll
+  for (int a = 0; a < 4; a++) {
+    const double v = wave_min(m[a]);
+    if (lane == 0) s_m[wid][a] = v;
+  }
+  __syncthreads();
+  if (threadIdx.x < 4) {
+    const int a = threadIdx.x;
+    double M = s_m[0][a];
+    for (int w = 1; w < nw; w++) M = (s_m[w][a] < M) ? s_m[w][a] : M;
+    double r;
+    if (B[a] < 0) r = fminf_(1.0, M);                              // no NaN: MIN over 1.0 and all
+    else r = (M == __builtin_inf()) ? __builtin_nan("") : M;  // a trailing NaN stays
+    A.dtbits[a] = (unsigned long long)__double_as_longlong(r);
   }
 }
 
