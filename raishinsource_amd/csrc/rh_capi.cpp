@@ -274,6 +274,34 @@ bool read_time(raishin_model* m, double* time, double* dt_last, const char* who)
   return true;
 }
 
+// The flux buffers keep the last fluxes written (a sweep reads its slot back
+// where the wave speeds are unordered), and the mixed form stores them as
+// float: a change of form converts them.
+bool retype_fluxes(raishin_model* m, bool to_float) {
+  if (!m->F || !m->CT) return true;
+  const long long N = m->G.N;
+  double* tmp = nullptr;
+  HIPCHK(hipMalloc((void**)&tmp, (size_t)27 * N * sizeof(double)));
+  bool ok = true;
+  const struct { double* p; long long planes; } bufs[] = {{m->F, 27}, {m->CT, 6}};
+  for (const auto& b : bufs) {
+    const long long n = b.planes * N;
+    const size_t bytes = (size_t)n * (to_float ? sizeof(double) : sizeof(float));  // as stored now
+    if (hipMemcpyAsync(tmp, b.p, bytes, hipMemcpyDeviceToDevice, m->st) != hipSuccess ||
+        launch_retype(tmp, b.p, n, to_float, m->st) != hipSuccess) {
+      ok = false;
+      break;
+    }
+  }
+  const hipError_t e = hipStreamSynchronize(m->st);
+  (void)hipFree(tmp);
+  if (!ok || e != hipSuccess) {
+    set_error("raishin_model_set_precision: converting the stored fluxes failed");
+    return false;
+  }
+  return true;
+}
+
 bool graph_ready(raishin_model* m) {
   if (m->gexec && m->graph_W == m->W && m->graph_U == m->Ucur && m->graph_mixed == m->mixed) return true;
   return capture_pair(m);
@@ -365,7 +393,9 @@ void raishin_model_debug_stage(raishin_model* m, int stage, double* ww, double* 
 
 void raishin_model_set_precision(raishin_model* m, int mode) {
   if (mode != 0 && mode != 1) { set_error("raishin_model_set_precision: mode %d (0 FP64, 1 mixed)", mode); return; }
-  m->mixed = mode == 1;
+  const bool mixed = mode == 1;
+  if (mixed != m->mixed && !retype_fluxes(m, mixed)) return;
+  m->mixed = mixed;
 }
 
 void raishin_model_set_timing(raishin_model* m, int on) {

@@ -376,8 +376,8 @@ RH_DEV void b_sweep(const Geo& G, const SweepArgs& A, long long blk) {
   const double* __restrict__ W = A.W;
   const double* __restrict__ metF = A.metF;
   const R* __restrict__ E = (const R*)A.E;
-  double* __restrict__ F = A.F;
-  double* __restrict__ CTsrc = A.CTsrc;
+  R* __restrict__ F = (R*)A.F;
+  R* __restrict__ CTsrc = (R*)A.CTsrc;
   const R gam = R(A.gam);
   int i, j, k;
   const long long t = blk * blockDim.x + threadIdx.x;
@@ -438,7 +438,7 @@ RH_DEV void b_sweep(const Geo& G, const SweepArgs& A, long long blk) {
   const R cmax = fmaxf_(fmaxf_(R(0), vpr), vpl);
   const R cmin = (D == 3) ? fminf_(fminf_(R(0), vpr), vml)   // calcha.f90:107
                           : fminf_(fminf_(R(0), vmr), vml);
-  double* Fo = F + c;
+  R* Fo = F + c;
   // transverse B fluxes of this direction and their CTsrc planes
   // (0 ww(2,7) 1 ww(3,7) 2 ww(1,8) 3 ww(3,8) 4 ww(1,9) 5 ww(2,9))
   constexpr int nA = (D == 1) ? 7 : 6, nB = (D == 3) ? 7 : 8;
@@ -456,10 +456,10 @@ RH_DEV void b_sweep(const Geo& G, const SweepArgs& A, long long blk) {
     else o = Fo[(long long)n * N];  // unordered (NaN) speeds: hlle keeps ww, then flushes it
     if (absr(o) < 1.0e-40) o = 0.0;
     if (n == nA || n == nB) {
-      CTsrc[(long long)(n == nA ? pA : pB) * N + c] = o;
-      if (!in_ct) Fo[(long long)n * N] = o;  // inside the CT box fluxct writes this slot
+      CTsrc[(long long)(n == nA ? pA : pB) * N + c] = R(o);
+      if (!in_ct) Fo[(long long)n * N] = R(o);  // inside the CT box fluxct writes this slot
     } else {
-      Fo[(long long)n * N] = o;
+      Fo[(long long)n * N] = R(o);
     }
   }
 }
@@ -468,31 +468,34 @@ RH_DEV void b_sweep(const Geo& G, const SweepArgs& A, long long blk) {
 // Flux-CT (Toth 2000): fluxct, ct.f90:30-118.  Reads the six transverse B
 // fluxes from CTsrc (planes 0 ww(2,7) 1 ww(3,7) 2 ww(1,8) 3 ww(3,8) 4 ww(1,9)
 // 5 ww(2,9), as the sweeps left them) and writes the averaged values into
-// the flux array in place, like ct.f90:98-109.
+// the flux array in place, like ct.f90:98-109.  FT: the stored type (float
+// in the mixed form); the averages are formed in double.
 // ------------------------------------------------------------------------
+template <class FT>
 RH_DEV void b_fluxct(const Geo& G, const CtArgs& A, long long blk) {
-  const double* __restrict__ CTsrc = A.CTsrc;
-  double* __restrict__ F1 = A.F1;
-  double* __restrict__ F2 = A.F2;
-  double* __restrict__ F3 = A.F3;
+  const FT* __restrict__ CTsrc = (const FT*)A.CTsrc;
+  FT* __restrict__ F1 = (FT*)A.F1;
+  FT* __restrict__ F2 = (FT*)A.F2;
+  FT* __restrict__ F3 = (FT*)A.F3;
   int i, j, k;
   const long long t = blk * blockDim.x + threadIdx.x;
   if (!box_index(t, G.nm1, G.ni - G.nm1, jlo(G, G.nm1), jhi(G, G.nj - G.nm1), G.nm1, G.nk - G.nm1, i, j, k)) return;
   const long long N = G.N, c = cidx(i, j, k, G);
   const long long si = 1, sj = jstride(G), sk = kstride(G);
-  const double* fyx = CTsrc + 0 * N;  // ww(2,7)
-  const double* fzx = CTsrc + 1 * N;  // ww(3,7)
-  const double* fxy = CTsrc + 2 * N;  // ww(1,8)
-  const double* fzy = CTsrc + 3 * N;  // ww(3,8)
-  const double* fxz = CTsrc + 4 * N;  // ww(1,9)
-  const double* fyz = CTsrc + 5 * N;  // ww(2,9)
+  const FT* fyx = CTsrc + 0 * N;  // ww(2,7)
+  const FT* fzx = CTsrc + 1 * N;  // ww(3,7)
+  const FT* fxy = CTsrc + 2 * N;  // ww(1,8)
+  const FT* fzy = CTsrc + 3 * N;  // ww(3,8)
+  const FT* fxz = CTsrc + 4 * N;  // ww(1,9)
+  const FT* fyz = CTsrc + 5 * N;  // ww(2,9)
   const double c8 = 1.0 / 8.0;
-  F2[6 * N + c] = c8 * (2.0 * fyx[c] + fyx[c + si] + fyx[c - si] - fxy[c] - fxy[c + sj] - fxy[c - si] - fxy[c - si + sj]);
-  F3[6 * N + c] = c8 * (2.0 * fzx[c] + fzx[c + si] + fzx[c - si] - fxz[c] - fxz[c + sk] - fxz[c - si] - fxz[c - si + sk]);
-  F1[7 * N + c] = c8 * (2.0 * fxy[c] + fxy[c + sj] + fxy[c - sj] - fyx[c] - fyx[c + si] - fyx[c - sj] - fyx[c + si - sj]);
-  F3[7 * N + c] = c8 * (2.0 * fzy[c] + fzy[c + sj] + fzy[c - sj] - fyz[c] - fyz[c + sk] - fyz[c - sj] - fyz[c - sj + sk]);
-  F1[8 * N + c] = c8 * (2.0 * fxz[c] + fxz[c + sk] + fxz[c - sk] - fzx[c] - fzx[c + si] - fzx[c - sk] - fzx[c + si - sk]);
-  F2[8 * N + c] = c8 * (2.0 * fyz[c] + fyz[c + sk] + fyz[c - sk] - fzy[c] - fzy[c + sj] - fzy[c - sk] - fzy[c + sj - sk]);
+  // 2.0 * x leads each sum, so every term is added in double
+  F2[6 * N + c] = FT(c8 * (2.0 * fyx[c] + fyx[c + si] + fyx[c - si] - fxy[c] - fxy[c + sj] - fxy[c - si] - fxy[c - si + sj]));
+  F3[6 * N + c] = FT(c8 * (2.0 * fzx[c] + fzx[c + si] + fzx[c - si] - fxz[c] - fxz[c + sk] - fxz[c - si] - fxz[c - si + sk]));
+  F1[7 * N + c] = FT(c8 * (2.0 * fxy[c] + fxy[c + sj] + fxy[c - sj] - fyx[c] - fyx[c + si] - fyx[c - sj] - fyx[c + si - sj]));
+  F3[7 * N + c] = FT(c8 * (2.0 * fzy[c] + fzy[c + sj] + fzy[c - sj] - fyz[c] - fyz[c + sk] - fyz[c - sj] - fyz[c - sj + sk]));
+  F1[8 * N + c] = FT(c8 * (2.0 * fxz[c] + fxz[c + sk] + fxz[c - sk] - fzx[c] - fzx[c + si] - fzx[c - sk] - fzx[c + si - sk]));
+  F2[8 * N + c] = FT(c8 * (2.0 * fyz[c] + fyz[c + sk] + fyz[c - sk] - fzy[c] - fzy[c + sj] - fzy[c - sk] - fzy[c + sj - sk]));
 }
 
 // ------------------------------------------------------------------------
@@ -922,8 +925,11 @@ RH_DEV long long chr_plane(int l, int a, int b) {
   return (long long)(l + 4 * a + 16 * b);
 }
 
-template <int STAGE, bool SYM>
+template <int STAGE, bool SYM, class FT>
 RH_DEV void b_rk(const Geo& G, const StageArgs& A, long long blk) {
+  const FT* __restrict__ F1 = (const FT*)A.F1;
+  const FT* __restrict__ F2 = (const FT*)A.F2;
+  const FT* __restrict__ F3 = (const FT*)A.F3;
   int i, j, k;
   const long long t = blk * blockDim.x + threadIdx.x;
   if (!box_index(t, G.nm1 + 1, G.ni - G.nm1, jlo(G, G.nm1 + 1), jhi(G, G.nj - G.nm1), G.nm1 + 1, G.nk - G.nm1, i, j,
@@ -973,7 +979,8 @@ RH_DEV void b_rk(const Geo& G, const StageArgs& A, long long blk) {
 #pragma unroll
   for (int m = 0; m < 9; m++) {
     const long long o = (long long)m * N + c;
-    const double d1 = A.F1[o] - A.F1[o - 1], d2 = A.F2[o] - A.F2[o - sj], d3 = A.F3[o] - A.F3[o - sk];
+    const double d1 = double(F1[o]) - double(F1[o - 1]), d2 = double(F2[o]) - double(F2[o - sj]),
+                 d3 = double(F3[o]) - double(F3[o - sk]);
     double v;
     if (STAGE == 1) {
       v = A.Ua[o] - a1 * d1 - a2 * d2 - a3 * d3;
@@ -1667,6 +1674,12 @@ __global__ void k_ww_to_planes(const double* __restrict__ ww, double* __restrict
 #pragma unroll
     for (int n = 0; n < 9; n++) F[(long long)(d * 9 + n) * N + t] = ww[27 * t + d + 3 * n];
 }
+// the stored flux planes between the two precision forms
+template <class S, class T>
+__global__ void k_retype(const S* __restrict__ src, T* __restrict__ dst, long long n) {
+  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < n) dst[t] = T(src[t]);
+}
 __global__ void k_planes_to_ww(const double* __restrict__ F, double* __restrict__ ww, long long N) {
   const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= N) return;
@@ -1705,9 +1718,10 @@ template <int D, class R>
 __global__ void __launch_bounds__(256) k_mc_edges(Geo G, EdgeArgs A) { b_mc_edges<D, R>(G, A, xcd_block()); }
 template <int D, int IREC, class R, int ICHA>
 __global__ void __launch_bounds__(256) k_sweep(Geo G, SweepArgs A) { b_sweep<D, IREC, R, ICHA>(G, A, xcd_block()); }
-__global__ void __launch_bounds__(256) k_fluxct(Geo G, CtArgs A) { b_fluxct(G, A, xcd_block()); }
-template <int STAGE, bool SYM>
-__global__ void __launch_bounds__(256) k_rk(Geo G, StageArgs A) { b_rk<STAGE, SYM>(G, A, xcd_block()); }
+template <class FT>
+__global__ void __launch_bounds__(256) k_fluxct(Geo G, CtArgs A) { b_fluxct<FT>(G, A, xcd_block()); }
+template <int STAGE, bool SYM, class FT>
+__global__ void __launch_bounds__(256) k_rk(Geo G, StageArgs A) { b_rk<STAGE, SYM, FT>(G, A, xcd_block()); }
 template <int IW>
 __global__ void __launch_bounds__(256) k_recov(Geo G, StageArgs A) { b_recov<IW>(G, A, xcd_block()); }
 template <int IW>
@@ -1742,16 +1756,17 @@ __global__ void __launch_bounds__(256) kb_sweep(const BatchItem* __restrict__ it
   const SweepArgs A = it[q].A.s;
   b_sweep<D, IREC, R, ICHA>(G, A, blk);
 }
+template <class FT>
 __global__ void __launch_bounds__(256) kb_fluxct(const BatchItem* __restrict__ it, int n) {
   RH_BATCH_PROLOGUE
   const CtArgs A = it[q].A.c;
-  b_fluxct(G, A, blk);
+  b_fluxct<FT>(G, A, blk);
 }
-template <int STAGE, bool SYM>
+template <int STAGE, bool SYM, class FT>
 __global__ void __launch_bounds__(256) kb_rk(const BatchItem* __restrict__ it, int n) {
   RH_BATCH_PROLOGUE
   const StageArgs A = it[q].A.st;
-  b_rk<STAGE, SYM>(G, A, blk);
+  b_rk<STAGE, SYM, FT>(G, A, blk);
 }
 template <int IW>
 __global__ void __launch_bounds__(256) kb_recov(const BatchItem* __restrict__ it, int n) {
@@ -1895,7 +1910,7 @@ static void with_sweep(int variant, F&& f) {
 }
 
 hipError_t launch_sweep(const Geo& G, int dir, int irec, const double* W, const double* metF, const void* E,
-                        double gam, double* F, double* CTsrc, bool mixed, hipStream_t st) {
+                        double gam, void* F, void* CTsrc, bool mixed, hipStream_t st) {
   const long long n = box(G.nm0, G.ni - G.nm0) * jbox(G, G.nm0, G.nj - G.nm0) * box(G.nm0, G.nk - G.nm0);
   SweepArgs A{W, metF, E, gam, F, CTsrc};
   const int bt = slot_threads(G, TS_SWEEP1 + dir - 1);
@@ -1914,13 +1929,14 @@ hipError_t launch_sweep(const Geo& G, int dir, int irec, const double* W, const 
   return hipGetLastError();
 }
 
-hipError_t launch_fluxct(const Geo& G, const double* CTsrc, double* F1, double* F2, double* F3, hipStream_t st) {
+hipError_t launch_fluxct(const Geo& G, const void* CTsrc, void* F1, void* F2, void* F3, bool f32, hipStream_t st) {
   const long long n = box(G.nm1, G.ni - G.nm1) * jbox(G, G.nm1, G.nj - G.nm1) * box(G.nm1, G.nk - G.nm1);
   CtArgs A{CTsrc, F1, F2, F3};
   const int bt = slot_threads(G, TS_FLUXCT);
-  if (g_rec) { AnyArgs a{}; a.c = A; record(LK_FLUXCT, 0, bt, nblk(n, bt), G, a); return hipSuccess; }
+  if (g_rec) { AnyArgs a{}; a.c = A; record(LK_FLUXCT, (int)f32, bt, nblk(n, bt), G, a); return hipSuccess; }
   if (n == 0) return hipSuccess;
-  hipLaunchKernelGGL(k_fluxct, dim3(nblk(n, bt)), dim3(bt), 0, st, G, A);
+  if (f32) hipLaunchKernelGGL(k_fluxct<float>, dim3(nblk(n, bt)), dim3(bt), 0, st, G, A);
+  else hipLaunchKernelGGL(k_fluxct<double>, dim3(nblk(n, bt)), dim3(bt), 0, st, G, A);
   return hipGetLastError();
 }
 
@@ -1928,16 +1944,23 @@ hipError_t launch_update(const Geo& G, int stage, const StageArgs& A, hipStream_
   const long long n = box(G.nm1 + 1, G.ni - G.nm1) * jbox(G, G.nm1 + 1, G.nj - G.nm1) * box(G.nm1 + 1, G.nk - G.nm1);
   const int bt = slot_threads(G, TS_RK);
   const unsigned nb = nblk(n, bt);
-  if (g_rec) { AnyArgs a{}; a.st = A; record(LK_RK, stage + 8 * (int)A.chr_sym, bt, nb, G, a); return hipSuccess; }
+  if (g_rec) {
+    AnyArgs a{};
+    a.st = A;
+    record(LK_RK, stage + 8 * (int)A.chr_sym + 16 * (int)A.f32flux, bt, nb, G, a);
+    return hipSuccess;
+  }
   if (n == 0) return hipSuccess;
-#define L(S) \
-  do { if (A.chr_sym) hipLaunchKernelGGL((k_rk<S, true>), dim3(nb), dim3(bt), 0, st, G, A); \
-       else hipLaunchKernelGGL((k_rk<S, false>), dim3(nb), dim3(bt), 0, st, G, A); } while (0)
+#define LT(S, Y) \
+  do { if (A.f32flux) hipLaunchKernelGGL((k_rk<S, Y, float>), dim3(nb), dim3(bt), 0, st, G, A); \
+       else hipLaunchKernelGGL((k_rk<S, Y, double>), dim3(nb), dim3(bt), 0, st, G, A); } while (0)
+#define L(S) do { if (A.chr_sym) LT(S, true); else LT(S, false); } while (0)
   if (stage == 1) L(1);
   else if (stage == 2) L(2);
   else if (stage == 3) L(3);
   else L(4);
 #undef L
+#undef LT
   return hipGetLastError();
 }
 
@@ -2041,18 +2064,23 @@ hipError_t launch_batched(int kind, int variant, int threads, const BatchItem* i
         hipLaunchKernelGGL((kb_sweep<D.value, I.value, typename decltype(T)::type, C.value>), g, b, 0, st, it, n);
       });
       break;
-    case LK_FLUXCT: hipLaunchKernelGGL(kb_fluxct, g, b, 0, st, it, n); break;
+    case LK_FLUXCT:
+      if (variant == 1) hipLaunchKernelGGL(kb_fluxct<float>, g, b, 0, st, it, n);
+      else hipLaunchKernelGGL(kb_fluxct<double>, g, b, 0, st, it, n);
+      break;
     case LK_RK: {
       const int stg = variant & 7;
-      const bool sym = (variant & 8) != 0;
-#define L(S) \
-  do { if (sym) hipLaunchKernelGGL((kb_rk<S, true>), g, b, 0, st, it, n); \
-       else hipLaunchKernelGGL((kb_rk<S, false>), g, b, 0, st, it, n); } while (0)
+      const bool sym = (variant & 8) != 0, f32 = (variant & 16) != 0;
+#define LT(S, Y) \
+  do { if (f32) hipLaunchKernelGGL((kb_rk<S, Y, float>), g, b, 0, st, it, n); \
+       else hipLaunchKernelGGL((kb_rk<S, Y, double>), g, b, 0, st, it, n); } while (0)
+#define L(S) do { if (sym) LT(S, true); else LT(S, false); } while (0)
       if (stg == 1) L(1);
       else if (stg == 2) L(2);
       else if (stg == 3) L(3);
       else L(4);
 #undef L
+#undef LT
       break;
     }
     case LK_RECOV:
@@ -2109,6 +2137,15 @@ hipError_t launch_ww_to_planes(const double* ww, double* F, long long N, hipStre
 hipError_t launch_planes_to_ww(const double* F, double* ww, long long N, hipStream_t st) {
   if (N == 0) return hipSuccess;
   hipLaunchKernelGGL(k_planes_to_ww, dim3(nblk(N, 256)), dim3(256), 0, st, F, ww, N);
+  return hipGetLastError();
+}
+
+hipError_t launch_retype(const void* src, void* dst, long long n, bool to_float, hipStream_t st) {
+  if (n <= 0) return hipSuccess;
+  if (to_float)
+    hipLaunchKernelGGL((k_retype<double, float>), dim3(nblk(n, 256)), dim3(256), 0, st, (const double*)src, (float*)dst, n);
+  else
+    hipLaunchKernelGGL((k_retype<float, double>), dim3(nblk(n, 256)), dim3(256), 0, st, (const float*)src, (double*)dst, n);
   return hipGetLastError();
 }
 

@@ -9,7 +9,7 @@ namespace rh {
 struct StageArgs {
   const double *W;                 // primitives at stage start (9 planes)
   const double *Ua, *Ub;           // RK inputs: stage1 uu; stage2 uo,uh; stage3 uo,us
-  const double *F1, *F2, *F3;      // ww(1..3,:,...) after ct (9 planes each)
+  const void *F1, *F2, *F3;        // ww(1..3,:,...) after ct (9 planes each; float when f32flux)
   const double *metC, *chr;        // centre metric tables, Christoffel tables
   const double *dx1b, *dx2b, *dx3b;
   const double *r15, *r25;         // x1(i)**(-3/2), x1(i)**(-5/2) from the host libm
@@ -26,6 +26,7 @@ struct StageArgs {
   double gam, cfl, dmin, pmin;
   int iter;
   bool chr_sym;                    // Christoffel tables symmetric in the lower indices
+  bool f32flux;                    // F1..F3 hold float (the mixed form's sweeps)
 };
 
 struct PostArgs {
@@ -68,12 +69,12 @@ struct SweepArgs {
   const double *W, *metF;
   const void *E;
   double gam;
-  double *F, *CTsrc;
+  void *F, *CTsrc;  // element type: the sweep's arithmetic type (double, or float when mixed)
 };
 
 struct CtArgs {
-  const double *CTsrc;
-  double *F1, *F2, *F3;
+  const void *CTsrc;  // double, or float when mixed (as the sweeps wrote them)
+  void *F1, *F2, *F3;
 };
 
 struct AdvArgs {
@@ -152,9 +153,14 @@ bool one_group_per_model(int kind);
 hipError_t launch_edges(const Geo& G, int dir, int irec, const double* W, const double* dxs, const double* xs,
                         const double* tab, double gam, void* E, bool mixed, hipStream_t st);
 // Fluxes at the interfaces along dir; E = the PPM edges (irec 5), unused otherwise.
+// F and CTsrc take doubles, or floats when mixed (every value the float
+// arithmetic produced, so storing it in float loses nothing).
 hipError_t launch_sweep(const Geo& G, int dir, int irec, const double* W, const double* metF, const void* E,
-                        double gam, double* F, double* CTsrc, bool mixed, hipStream_t st);
-hipError_t launch_fluxct(const Geo& G, const double* CTsrc, double* F1, double* F2, double* F3, hipStream_t st);
+                        double gam, void* F, void* CTsrc, bool mixed, hipStream_t st);
+// f32: CTsrc and F hold floats (the averages are formed in double either way)
+hipError_t launch_fluxct(const Geo& G, const void* CTsrc, void* F1, void* F2, void* F3, bool f32, hipStream_t st);
+// n values of type double (to_float) or float (else) from src into dst, the other type
+hipError_t launch_retype(const void* src, void* dst, long long n, bool to_float, hipStream_t st);
 // RK update + sources into A.Ukeep (every interior cell), then recov1a into A.Wpre.
 hipError_t launch_update(const Geo& G, int stage, const StageArgs& A, hipStream_t st);
 hipError_t launch_recov(const Geo& G, const StageArgs& A, hipStream_t st);

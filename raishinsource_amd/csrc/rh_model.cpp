@@ -375,15 +375,23 @@ static bool build_model_(raishin_model* m, const raishin_pram* p, const double* 
   return true;
 }
 
-bool copy_planes_out(raishin_model* m, const double* planes, int nplanes, int stride_out, int offset_out,
-                     double* host) {
+bool copy_planes_out(raishin_model* m, const void* planes, int nplanes, int stride_out, int offset_out,
+                     double* host, bool f32 = false) {
   // planes[n*N + c] -> host[offset_out + stride_out*(n + nplanes*c)] via a
-  // staging buffer; a collapsed model's row goes to every phi row
+  // staging buffer; a collapsed model's row goes to every phi row.  f32: the
+  // planes hold floats (the mixed form's fluxes), widened exactly.
   const Geo& G = m->G;
   const long long N = G.N;
   std::vector<double> tmp((size_t)nplanes * N);
-  HIPCHK(hipMemcpyAsync(tmp.data(), planes, (size_t)nplanes * N * 8, hipMemcpyDeviceToHost, m->st));
-  HIPCHK(hipStreamSynchronize(m->st));
+  if (f32) {
+    std::vector<float> tf((size_t)nplanes * N);
+    HIPCHK(hipMemcpyAsync(tf.data(), planes, tf.size() * sizeof(float), hipMemcpyDeviceToHost, m->st));
+    HIPCHK(hipStreamSynchronize(m->st));
+    for (size_t e = 0; e < tf.size(); e++) tmp[e] = tf[e];
+  } else {
+    HIPCHK(hipMemcpyAsync(tmp.data(), planes, (size_t)nplanes * N * 8, hipMemcpyDeviceToHost, m->st));
+    HIPCHK(hipStreamSynchronize(m->st));
+  }
   const long long ni = G.ni, nj = G.nj, nk = G.nk;
   for (long long k = 0; k < nk; k++)
     for (long long j = 0; j < nj; j++)
@@ -445,9 +453,14 @@ void unpack_row(const Geo& G, const std::vector<double>& in, double* full) {
       std::memcpy(full + row * (j + (size_t)G.nj * k), &in[row * k], row * sizeof(double));
 }
 
+// Plane p of the flux buffer F: 8-byte elements, or 4-byte ones in the mixed
+// form (the buffer keeps its FP64 size; floats use its first half).
+static void* flux_plane(const raishin_model* m, double* F, long long p) {
+  return m->mixed ? (void*)((float*)F + p * m->G.N) : (void*)(F + p * m->G.N);
+}
+
 bool stage_sweeps(raishin_model* m, const double* W, double* F) {
   const Geo& G = m->G;
-  const long long N = G.N;
   const int irec = m->p.irec;
   for (int d = 1; d <= 3; d++) {
     if (irec == 5 || irec == 1) {
@@ -456,14 +469,15 @@ bool stage_sweeps(raishin_model* m, const double* W, double* F) {
       HIPCHK(launch_edges(G, d, irec, W, m->dxs[d - 1], m->xs[d - 1], tab, m->p.gam, m->E, m->mixed, m->st));
     }
     TimedLaunch t(m, K_SWEEP1 + d - 1);
-    HIPCHK(launch_sweep(G, d, irec, W, m->met[d], m->E, m->p.gam, F + (d - 1) * 9 * N, m->CT, m->mixed, m->st));
+    HIPCHK(launch_sweep(G, d, irec, W, m->met[d], m->E, m->p.gam, flux_plane(m, F, (d - 1) * 9), m->CT, m->mixed,
+                        m->st));
   }
   return true;
 }
 
 bool stage_ct(raishin_model* m, double* F) {
-  const long long N = m->G.N;
-  { TimedLaunch t(m, K_FLUXCT); HIPCHK(launch_fluxct(m->G, m->CT, F, F + 9 * N, F + 18 * N, m->st)); }
+  TimedLaunch t(m, K_FLUXCT);
+  HIPCHK(launch_fluxct(m->G, m->CT, flux_plane(m, F, 0), flux_plane(m, F, 9), flux_plane(m, F, 18), m->mixed, m->st));
   return true;
 }
 
@@ -472,9 +486,12 @@ bool stage_fluxes(raishin_model* m, const double* W, double* F) { return stage_s
 bool stage_update(raishin_model* m, int rkstage, const double* W, const double* Ua, const double* Ub, double* Uout,
                   const double* F, const HostDt* hd) {
   const Geo& G = m->G;
-  const long long N = G.N;
   StageArgs A{};
-  A.W = W; A.Ua = Ua; A.Ub = Ub; A.F1 = F; A.F2 = F + 9 * N; A.F3 = F + 18 * N;
+  A.W = W; A.Ua = Ua; A.Ub = Ub;
+  A.F1 = flux_plane(m, const_cast<double*>(F), 0);
+  A.F2 = flux_plane(m, const_cast<double*>(F), 9);
+  A.F3 = flux_plane(m, const_cast<double*>(F), 18);
+  A.f32flux = m->mixed;
   A.metC = m->met[0]; A.chr = m->chr; A.dx1b = m->dxb[0]; A.dx2b = m->dxb[1]; A.dx3b = m->dxb[2];
   A.r15 = m->r15; A.r25 = m->r25; A.dtbits = m->dtbits; A.Wpre = m->Wpre; A.wgout = m->wgout; A.Ukeep = Uout;
   A.cnt = m->cnt; A.cpart = m->cpart; A.carry = m->carry; A.cflag = m->cflag; A.carry_cap = m->carry_cap;
@@ -505,12 +522,11 @@ bool stage_post(raishin_model* m, const double* Wold, const double* Wo, double* 
 // fix-up -> ghosts/floors/conserved.
 bool run_stage(raishin_model* m, int rkstage, const double* Ua, const double* Ub, double* Uout,
                const StageTap* tap) {
-  const long long N = m->G.N;
   if (!stage_fluxes(m, m->W, m->F)) return false;
   if (tap && tap->ww) {
     // ww(d,n,i,j,k) as after ct()
     for (int d = 0; d < 3; d++)
-      if (!copy_planes_out(m, m->F + (long long)d * 9 * N, 9, 3, d, tap->ww)) return false;
+      if (!copy_planes_out(m, flux_plane(m, m->F, (long long)d * 9), 9, 3, d, tap->ww, m->mixed)) return false;
   }
   if (!stage_update(m, rkstage, m->W, Ua, Ub, Uout, m->F, nullptr)) return false;
   if (tap && tap->wpre && !copy_planes_out(m, m->Wpre, 9, 1, 0, tap->wpre)) return false;

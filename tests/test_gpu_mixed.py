@@ -38,3 +38,30 @@ def test_mixed_density_l1_within_1e5(name, nsteps):
           f"fallbacks {c32['recov_fallback']} vs {c64['recov_fallback']}")
     assert 0.0 < l1 <= 1e-5
     assert abs(t32 / t64 - 1) < 1e-4
+
+
+@pytest.mark.parametrize("name", ["uppm", "ump5"])
+def test_precision_round_trip_keeps_stored_fluxes(name):
+    """The mixed form stores the face fluxes as float; switching to FP64 and
+    back converts them there and back exactly, so a model that switched in
+    the middle continues bit for bit like one that never did (ump5 has cells
+    whose unordered wave speeds make a sweep keep the stored flux)."""
+    p = configs.ALL[name].resolved()
+    g = native.setup_grid(p)
+    m = native.setup_metric(p, g)
+    uri = native.setup_torus(p, g, m, seed=0, noise=0.02)
+    a, b = native.Model(p, g, m), native.Model(p, g, m)
+    for mod in (a, b):
+        mod.set_state(uri)
+        mod.set_precision(True)
+        mod.step(3)
+    a.set_precision(False)
+    a.set_precision(True)
+    ta, _ = a.step(4)
+    tb, _ = b.step(4)
+    wa, ua, _ = a.get_state()
+    wb, ub, _ = b.get_state()
+    a.close()
+    b.close()
+    assert ta == tb
+    assert np.array_equal(wa, wb, equal_nan=True) and np.array_equal(ua, ub, equal_nan=True)
