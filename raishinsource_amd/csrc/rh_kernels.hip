@@ -1716,8 +1716,13 @@ template <int D, class R>
 __global__ void __launch_bounds__(256) k_ppm_edges(Geo G, EdgeArgs A) { b_ppm_edges<D, R>(G, A, xcd_block()); }
 template <int D, class R>
 __global__ void __launch_bounds__(256) k_mc_edges(Geo G, EdgeArgs A) { b_mc_edges<D, R>(G, A, xcd_block()); }
+// the float form over precomputed face values (PPM, MC) fits 5 waves per
+// SIMD (96 VGPRs); the others are left to the compiler
+template <class R, int IREC>
+constexpr int sweep_waves = (sizeof(R) == 4 && (IREC == 5 || IREC == 1)) ? 5 : 1;
 template <int D, int IREC, class R, int ICHA>
-__global__ void __launch_bounds__(256) k_sweep(Geo G, SweepArgs A) { b_sweep<D, IREC, R, ICHA>(G, A, xcd_block()); }
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(sweep_waves<R, IREC>)))
+k_sweep(Geo G, SweepArgs A) { b_sweep<D, IREC, R, ICHA>(G, A, xcd_block()); }
 template <class FT>
 __global__ void __launch_bounds__(256) k_fluxct(Geo G, CtArgs A) { b_fluxct<FT>(G, A, xcd_block()); }
 template <int STAGE, bool SYM, class FT>
@@ -1751,7 +1756,8 @@ __global__ void __launch_bounds__(256) kb_mc_edges(const BatchItem* __restrict__
   b_mc_edges<D, R>(G, A, blk);
 }
 template <int D, int IREC, class R, int ICHA>
-__global__ void __launch_bounds__(256) kb_sweep(const BatchItem* __restrict__ it, int n) {
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(sweep_waves<R, IREC>)))
+kb_sweep(const BatchItem* __restrict__ it, int n) {
   RH_BATCH_PROLOGUE
   const SweepArgs A = it[q].A.s;
   b_sweep<D, IREC, R, ICHA>(G, A, blk);
