@@ -129,8 +129,11 @@ void *raishin_model_stream(raishin_model *m);
  * HLLE): 0 = FP64, the reference's arithmetic (default); 1 = mixed: those
  * two in FP32 on the vector ALUs, while the metric tables, flux-CT, the RK
  * accumulation, the sources, recov1a, the floors and the state stay FP64.
+ * The face fluxes are stored as float in mode 1 (flux-CT averages them in
+ * FP64); a change of mode converts the stored fluxes, synchronises the
+ * model's stream and allocates a temporary buffer for the conversion.
  * Not part of the reference; its error against FP64 is measured by bench.py
- * --precision mixed.  Takes effect from the next stage. */
+ * (its "mixed" object, --mixed-steps).  Takes effect from the next stage. */
 void raishin_model_set_precision(raishin_model *m, int mode);
 
 /* Per-kernel timing with HIP events on the model's stream.  Kinds, in order:
