@@ -1716,10 +1716,11 @@ template <int D, class R>
 __global__ void __launch_bounds__(256) k_ppm_edges(Geo G, EdgeArgs A) { b_ppm_edges<D, R>(G, A, xcd_block()); }
 template <int D, class R>
 __global__ void __launch_bounds__(256) k_mc_edges(Geo G, EdgeArgs A) { b_mc_edges<D, R>(G, A, xcd_block()); }
-// the float form over precomputed face values (PPM, MC) fits 5 waves per
-// SIMD (96 VGPRs); the others are left to the compiler
+// the sweeps over precomputed face values (PPM, MC) fit 3 waves per SIMD in
+// FP64 (168 VGPRs, against 170 and 2 waves unconstrained) and 5 in float (96);
+// MP5 and WENO5 are left to the compiler
 template <class R, int IREC>
-constexpr int sweep_waves = (sizeof(R) == 4 && (IREC == 5 || IREC == 1)) ? 5 : 1;
+constexpr int sweep_waves = (IREC == 5 || IREC == 1) ? (sizeof(R) == 4 ? 5 : 3) : 1;
 template <int D, int IREC, class R, int ICHA>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(sweep_waves<R, IREC>)))
 k_sweep(Geo G, SweepArgs A) { b_sweep<D, IREC, R, ICHA>(G, A, xcd_block()); }
