@@ -131,7 +131,7 @@ void *raishin_model_stream(raishin_model *m);
  * accumulation, the sources, recov1a, the floors and the state stay FP64.
  * The face fluxes are stored as float in mode 1 (flux-CT averages them in
  * FP64); a change of mode converts the stored fluxes, synchronises the
- * model's stream and allocates a temporary buffer for the conversion.
+ * device and allocates a temporary buffer for the conversion.
  * Not part of the reference; its error against FP64 is measured by bench.py
  * (its "mixed" object, --mixed-steps).  Takes effect from the next stage. */
 void raishin_model_set_precision(raishin_model *m, int mode);

@@ -280,6 +280,8 @@ bool read_time(raishin_model* m, double* time, double* dt_last, const char* who)
 bool retype_fluxes(raishin_model* m, bool to_float) {
   if (!m->F || !m->CT) return true;
   const long long N = m->G.N;
+  // a batch may still be stepping this model on its own stream
+  HIPCHK(hipDeviceSynchronize());
   double* tmp = nullptr;
   HIPCHK(hipMalloc((void**)&tmp, (size_t)27 * N * sizeof(double)));
   bool ok = true;
