@@ -105,6 +105,9 @@ G_PPM = Pram("gppm", 14, 11, 12, irec=5, irkt=2, cfl=0.4)
 G_MP5 = Pram("gmp5", 12, 9, 10, irec=6, irkt=3, cfl=0.5)
 G_WENO = Pram("gweno", 12, 9, 10, irec=7, irkt=3, cfl=0.5)
 T3D = Pram("t3d", 32, 16, 32, irec=6, irkt=3, cfl=0.5)
+# a small 3D torus with the scheme of the 256x128x128 PPM run (PPM, RK3, CFL
+# 0.5, 2% pressure noise): 30 x 14 x 30 interior cells
+T3D_PPM = Pram("t3dppm", 40, 24, 40, irec=5, irkt=3, cfl=0.5)
 
 
 
@@ -138,5 +141,5 @@ U_RC2 = Pram("urc2", 16, 10, 18, irec=1, irkt=3, cfl=0.3, icha=1, iwvec=2)
 TUBE = tube("tube", 1024)
 U_TUBE = tube("utube", 64)
 
-ALL = {p.name: p for p in (C1, C2, C2_PPM, C2_CPU, C2_WENO, U_PPM, U_MP5, U_WENO, T2D, T3D, G_PPM, G_MP5, G_WENO, TUBE,
+ALL = {p.name: p for p in (C1, C2, C2_PPM, C2_CPU, C2_WENO, U_PPM, U_MP5, U_WENO, T2D, T3D, T3D_PPM, G_PPM, G_MP5, G_WENO, TUBE,
                            U_TUBE, U_MC, T2D_MC, U_LEIS, T2D_GAM, T2D_RC2, U_RC2)}

@@ -76,7 +76,7 @@ def unit_vectors(name):
     return out
 
 
-def trajectory(name, nsteps, seed=0, snap=()):
+def trajectory(name, nsteps, seed=0, snap=(), keep_snap_digest=False):
     """Torus from the package set-up, stepped by the reference in main.f90's order."""
     r, g, m = grid_and_metric(name)
     p = r.p
@@ -90,9 +90,14 @@ def trajectory(name, nsteps, seed=0, snap=()):
     for s in range(nsteps):
         dt, t = rs.step(uu, uri, t)
         dts.append(dt)
+        if keep_snap_digest:
+            print(f"reference step {s + 1} dt {dt!r}", flush=True)
         if s + 1 in snap:
-            out[f"uri_{s + 1}"] = uri.copy(order="F")
-            out[f"uu_{s + 1}"] = uu.copy(order="F")
+            if keep_snap_digest:
+                out[f"uri_{s + 1}"], out[f"uu_{s + 1}"] = digest(uri), digest(uu)
+            else:
+                out[f"uri_{s + 1}"] = uri.copy(order="F")
+                out[f"uu_{s + 1}"] = uu.copy(order="F")
     out["uri"] = uri
     out["uu"] = uu
     out["dt"] = np.array(dts)
@@ -115,11 +120,94 @@ def c1_trajectory():
                 uri100=tr["uri"][:, :, j, :], dt=tr["dt"], time=np.array(tr["time"]))
 
 
+C2_SLICES = (5, 66)          # 0-based phi rows kept whole: the first interior row and one mid-way
+
+
+def summary(p, uri, uu):
+    """What the 3D fixture keeps of a state: digests of the whole arrays, per
+    field L1 / L-infinity / signed sums over the interior, per-phi-row L1 of
+    every field, and two whole phi rows of the primitives and one of the
+    conserved variables."""
+    n1 = p.nm1
+    inner = (slice(None), slice(n1, p.imax - n1), slice(n1, p.jmax - n1), slice(n1, p.kmax - n1))
+    w = uri[inner]
+    fin = np.isfinite(w)
+    out = dict(uri_digest=digest(uri), uu_digest=digest(uu),
+               finite_frac=np.array(fin.all(axis=0).mean()),
+               l1=np.array([np.abs(w[v]).sum() for v in range(9)]),
+               linf=np.array([np.abs(w[v]).max() for v in range(9)]),
+               sum=np.array([w[v].sum() for v in range(9)]),
+               l1_rows=np.abs(w).sum(axis=(1, 3)))
+    for j in C2_SLICES:
+        out[f"uri_row{j}"] = np.ascontiguousarray(uri[:, :, j, :])
+    out[f"uu_row{C2_SLICES[0]}"] = np.ascontiguousarray(uu[:, :, C2_SLICES[0], :])
+    return out
+
+
+def _c2ppm_oracle(nsteps):
+    """The oracle's C restatement over the same steps (run in a child process
+    beside the reference): the same summaries, plus the inversion-failure and
+    floor counts, which the reference prints but does not count."""
+    from harness import Oracle
+    p = configs.C2_PPM.resolved()
+    g = native.setup_grid(p)
+    m = native.setup_metric(p, g)
+    uri = native.setup_torus(p, g, m, seed=0)
+    o = Oracle(p)
+    o.reset_counters()
+    uu = fzeros(9, p.imax, p.jmax, p.kmax)
+    o.call("caluu1", uri, uu, m["gcov"], m["gcon"], m["detg"])
+    t, dts, d10 = 0.0, [], None
+    for s in range(nsteps):
+        dt, t = o.step(uu, uri, g, m, t)
+        dts.append(dt)
+        if s + 1 == 10:
+            d10 = (digest(uri), digest(uu))
+        print(f"oracle step {s + 1} dt {dt!r}", flush=True)
+    out = summary(p, uri, uu)
+    out.update(dt=np.array(dts), time=np.array(t), uri10_digest=d10[0], uu10_digest=d10[1])
+    out["counters"] = np.array([o.counters()[k] for k in COUNTER_NAMES], dtype=np.int64)
+    return out
+
+
+COUNTER_NAMES = ("recov_fallback", "recov_fail", "recov_neg_rho", "recov_neg_p", "recov_vsq_gt1", "recov_det0",
+                 "recov_wg_carry", "atm_floor", "pmm_scale", "pmm_floor", "guard")
+
+
+def c2ppm_trajectory(nsteps=100):
+    """The 256 x 128 x 128 torus (PPM, HLLE, flux-CT, RK3, CFL 0.5, 2% pressure
+    noise from seed 0) stepped 100 times by the reference routines in
+    main.f90's order, with the oracle run beside it; the two must agree bit
+    for bit before anything is written."""
+    from concurrent.futures import ProcessPoolExecutor
+    import multiprocessing as mp
+    with ProcessPoolExecutor(1, mp_context=mp.get_context("fork")) as ex:
+        fo = ex.submit(_c2ppm_oracle, nsteps)
+        tr = trajectory("c2ppm", nsteps, snap=(10,), keep_snap_digest=True)
+        orc = fo.result()
+    p = configs.C2_PPM.resolved()
+    ref = summary(p, tr["uri"], tr["uu"])
+    ref.update(dt=tr["dt"], time=np.array(tr["time"]), uri10_digest=tr["uri_10"], uu10_digest=tr["uu_10"],
+               uri0_digest=digest(tr["uri0"]))
+    for k, v in ref.items():
+        if k == "uri0_digest":
+            continue
+        a, b = np.asarray(v), np.asarray(orc[k])
+        same = a.tobytes() == b.tobytes() if a.dtype.kind == "f" else np.array_equal(a, b)
+        assert same, f"oracle and reference differ in {k}"
+    ref["counters"] = orc["counters"]
+    ref["counter_names"] = np.array(COUNTER_NAMES)
+    return ref
+
+
 def main(which):
-    for n in UNIT + ["t2d", "c1"]:
+    for n in UNIT + ["t2d", "c1"] + (["c2ppm"] if "c2ppm" in which else []):
         if not have_ref(n):
             subprocess.run([sys.executable, os.path.join(ROOT, "oracle", "build_ref.py"), n], check=True,
                            stdout=subprocess.DEVNULL)
+    if "c2ppm" in which:
+        np.savez_compressed(os.path.join(HERE, "traj_c2ppm.npz"), **c2ppm_trajectory())
+        print("wrote traj_c2ppm")
     if "c1" in which:
         np.savez_compressed(os.path.join(HERE, "traj_c1.npz"), **c1_trajectory())
         print("wrote traj_c1")

@@ -47,6 +47,22 @@ def field_errors(a, b):
     return out
 
 
+def finite_fraction(p, uri):
+    """Fraction of interior cells (nm1 ghosts per side excluded) whose nine
+    primitives are all finite."""
+    n1 = p.nm1
+    w = uri[:, n1:p.imax - n1, n1:p.jmax - n1, n1:p.kmax - n1]
+    return float(np.isfinite(w).all(axis=0).mean())
+
+
+def assert_finite(tag, p, uri, least):
+    """A comparison only pins what is finite: fail when too little of the
+    compared state is (a NaN compared with a NaN says nothing)."""
+    f = finite_fraction(p, uri)
+    print(f"{tag}: finite interior fraction {f:.4f} (required {least})")
+    assert f >= least, (tag, f, least)
+
+
 def assert_close(tag, a, b, tol=TOL):
     errs = field_errors(a, b)
     print(f"{tag}: bitdiff {bitdiff(a, b)} / {a.size}; per-var rel err " + " ".join(f"{e:.1e}" for e in errs))
@@ -62,6 +78,9 @@ SMALL = {
     # recov2 on the MP5 torus: many non-converged cells and negative
     # pressures, so the failure path and the skipped Newton cycles run
     "t3drc2": replace(configs.T3D, name="t3drc2", iwvec=2),
+    # MP5 and WENO5 over many steps: smooth 3D states with a floor-level band
+    "smp5": replace(configs.G_MP5, name="smp5", imax=24, jmax=14, kmax=20),
+    "sweno": replace(configs.G_WENO, name="sweno", imax=24, jmax=14, kmax=20),
 }
 
 
@@ -249,32 +268,45 @@ def test_golden_c1_100_steps_reference(oracle_lib):
     assert_close("c1 uri after 100 steps", w[:, :, j, :], gold["uri100"])
 
 
+# (name, initial state, steps, least finite fraction of the interior at the
+# compared step).  MP5 and WENO5 turn the small tori to NaN within two steps
+# in the reference too (MP5's absolute 1e-10 bound lets face densities go
+# negative and de**gam is then NaN), so those tori are compared after one step
+# (16%, 7.5% and 89% of the interior still finite) and after two for t3d
+# (21%); the multi-step MP5 and WENO5 evidence is the synthetic 3D states,
+# which stay finite.
 ORACLE_CASES = [
-    ("uppm", "torus", 12),
-    ("ump5", "torus", 8),
-    ("uweno", "torus", 8),
-    ("t3d", "torus", 4),
-    ("umc", "torus", 8),
-    ("t2dmc", "torus", 6),
-    ("uleis", "torus", 6),
-    ("t2dgam", "torus", 6),
-    ("t2drc2", "torus", 8),
-    ("urc2", "torus", 6),
-    ("t3drc2", "torus", 4),
-    ("gppm", "synthetic", 4),
-    ("gmp5", "synthetic", 4),
-    ("gweno", "synthetic", 4),
+    ("uppm", "torus", 12, 1.0),
+    ("t3dppm", "torus", 20, 1.0),
+    ("ump5", "torus", 1, 0.15),
+    ("uweno", "torus", 1, 0.07),
+    ("t3d", "torus", 1, 0.85),
+    ("t3d", "torus", 2, 0.2),
+    ("umc", "torus", 8, 1.0),
+    ("t2dmc", "torus", 6, 1.0),
+    ("uleis", "torus", 6, 1.0),
+    ("t2dgam", "torus", 6, 1.0),
+    ("t2drc2", "torus", 8, 1.0),
+    ("urc2", "torus", 6, 1.0),
+    ("t3drc2", "torus", 4, 1.0),
+    ("gppm", "synthetic", 4, 1.0),
+    ("gmp5", "synthetic", 4, 1.0),
+    ("gweno", "synthetic", 4, 1.0),
+    ("smp5", "synthetic", 12, 1.0),
+    ("sweno", "synthetic", 12, 1.0),
 ]
 
 
-@pytest.mark.parametrize("name,state,nsteps", ORACLE_CASES)
-def test_steps_match_oracle(name, state, nsteps, oracle_lib):
+@pytest.mark.parametrize("name,state,nsteps,least", ORACLE_CASES,
+                         ids=[f"{c[0]}-{c[2]}" for c in ORACLE_CASES])
+def test_steps_match_oracle(name, state, nsteps, least, oracle_lib):
     """Whole steps (main.f90:743-1149 order) against the oracle, with the
     inversion-failure and floor counts."""
     p, g, m, uri = make(name, state)
     wo, uo, to, dto, co = oracle_run(p, g, m, uri, nsteps)
     wg, ug, tg, dtg, cg = gpu_run(p, g, m, uri, nsteps)
     print(name, "oracle counters", co, "\n", name, "gpu counters", cg)
+    assert_finite(name, p, wo, least)
     assert np.array_equal(np.isnan(dtg), np.isnan(dto))
     assert_close(name + " dt", dtg[None, np.isfinite(dto)], dto[None, np.isfinite(dto)])
     assert_close(name + " uri", wg, wo)
@@ -403,16 +435,19 @@ def test_full_subdomain_axisymmetric(name, nsteps, oracle_lib):
 
 
 @pytest.mark.timeout(900)
-@pytest.mark.parametrize("name,nsteps", [("c2", 2), ("c2weno", 1)])
-def test_full_subdomain_matches_oracle(name, nsteps, oracle_lib):
+@pytest.mark.parametrize("name,nsteps,least", [("c2", 2, 0.98), ("c2weno", 1, 0.98)])
+def test_full_subdomain_matches_oracle(name, nsteps, least, oracle_lib):
     """The 256 x 128 x 128 torus with 2% pressure noise (MP5 / WENO5, RK3)
     against the oracle at full size.  The reference turns this torus to NaN
     within a few steps (MP5's absolute 1e-10 limiter threshold lets face
-    densities go negative); NaN positions and all counters must agree."""
+    densities go negative); at the compared step 98.8% (MP5, 2 steps) and
+    98.6% (WENO5, 1 step) of the interior is still finite.  NaN positions and
+    all counters must agree."""
     p, g, m, uri = make(name)
     wg, ug, tg, dtg, cg = gpu_run(p, g, m, uri, nsteps)
     wo, uo, to, dto, co = oracle_run(p, g, m, uri, nsteps)
     print(name, "NaN cells", int(np.isnan(wo).any(axis=0).sum()), "\n oracle", co, "\n gpu", cg)
+    assert_finite(name, p, wo, least)
     assert np.array_equal(np.isnan(dtg), np.isnan(dto))
     assert_close(name + " dt", dtg[None], dto[None])
     assert_close(name + " uri", wg, wo)

@@ -230,6 +230,189 @@ def test_ghost_fill_codes(oracle_lib):
             assert u[1, p.imax - 1 - n, j, k] == expect
 
 
+def ghost_fill_by_index(u, uo, p, order=(0, 1, 2)):
+    """bnd4 (bnd.f90:4-1059, one rank) as whole-face index maps rather than
+    loops: the periodic copies over the interior ranges of the other two axes
+    (bndpefb4, bnd.f90:25-381), then for each axis in turn (x1, x2, x3,
+    bnd.f90:419, 624, 817) and each variable the inner and outer ghost layers
+    over the full extent of the other axes (bndryfb4).  Each code, as the
+    reference writes it: 2 keep uo; 3 (4u(n+1)-u(n+2))/3 and 4 2u(n+1)-u(n+2),
+    outward one layer at a time; 5 copy the first interior layer; 7 uo plus
+    the change of the next layer inward; 12/14 mirror (negated / not) about
+    the boundary face; 13 zero; 18 (x1 outer) copy the last interior layer,
+    or zero where its v^1 (var 2) is negative."""
+    u = u.copy(order="F")
+    n1 = p.nm1
+    ext = (p.imax, p.jmax, p.kmax)
+    bcs = ((p.iboux1in, p.iboux1ot), (p.iboux2in, p.iboux2ot), (p.iboux3in, p.iboux3ot))
+
+    def sl(ax, idx, inner_only=False):
+        s = [slice(None)] * 3
+        if inner_only:
+            s = [slice(n1, e - n1) for e in ext]
+        s[ax] = idx
+        return (slice(None),) + tuple(s)
+
+    # periodic pass: ghost layer g (0-based) takes the interior layer one
+    # interior length away; only interior cells of the other two axes
+    for ax in range(3):
+        e = ext[ax]
+        nint = e - 2 * n1
+        per = [m for m in range(9) if bcs[ax][0][m] == 1 or bcs[ax][1][m] == 1]
+        if not per:
+            continue
+        lo = np.arange(n1)
+        hi = np.arange(e - n1, e)
+        for m in per:
+            v = u[m:m + 1]
+            v[sl(ax, lo, True)] = v[sl(ax, lo + nint, True)]
+            v[sl(ax, hi, True)] = v[sl(ax, hi - nint, True)]
+    for ax in order:
+        e = ext[ax]
+        for m in range(9):
+            for outer in (0, 1):
+                code = bcs[ax][outer][m]
+                v, vo = u[m:m + 1], uo[m:m + 1]
+                # ghost layers ordered outward from the boundary face, and
+                # the interior layers mirrored onto them
+                ghosts = list(range(n1 - 1, -1, -1)) if not outer else list(range(e - n1, e))
+                inside = list(range(n1, 2 * n1)) if not outer else list(range(e - n1 - 1, e - 2 * n1 - 1, -1))
+                step = -1 if not outer else 1
+                if code == 1:
+                    continue
+                for q, g in enumerate(ghosts):
+                    if code == 2:
+                        v[sl(ax, g)] = vo[sl(ax, g)]
+                    elif code == 3:
+                        v[sl(ax, g)] = (4.0 * v[sl(ax, g - step)] - v[sl(ax, g - 2 * step)]) / 3.0
+                    elif code == 4:
+                        v[sl(ax, g)] = 2.0 * v[sl(ax, g - step)] - v[sl(ax, g - 2 * step)]
+                    elif code == 5:
+                        v[sl(ax, g)] = v[sl(ax, inside[0])]
+                    elif code == 7:
+                        v[sl(ax, g)] = vo[sl(ax, g)] + v[sl(ax, g - step)] - vo[sl(ax, g - step)]
+                    elif code in (12, 14):
+                        v[sl(ax, g)] = (-1.0 if code == 12 else 1.0) * v[sl(ax, inside[q])]
+                    elif code == 13:
+                        v[sl(ax, g)] = 0.0
+                    elif code == 18:
+                        assert ax == 0 and outer
+                        edge = u[1:2][sl(ax, e - n1 - 1)]
+                        v[sl(ax, g)] = np.where(edge < 0.0, 0.0, v[sl(ax, e - n1 - 1)])
+                    else:
+                        raise ValueError(code)
+    return u
+
+
+GHOST_SETS = {
+    # the torus: copy / outflow-no-inflow in r, periodic phi, mirror theta
+    "torus": {},
+    # every other code on some variable of some face
+    "codes": dict(iboux1in=(2, 3, 4, 5, 7, 12, 13, 14, 5), iboux1ot=(18, 18, 18, 7, 5, 4, 3, 2, 13),
+                  iboux2in=(12, 14, 13, 2, 3, 4, 5, 7, 14), iboux2ot=(14, 12, 2, 3, 4, 5, 7, 13, 12),
+                  iboux3in=(3, 4, 5, 7, 12, 13, 14, 2, 12), iboux3ot=(5, 7, 12, 13, 14, 2, 3, 4, 14)),
+    # periodic in every axis (the Riemann tubes' x2, x3)
+    "periodic": dict(iboux1in=(1,) * 9, iboux1ot=(1,) * 9, iboux3in=(1,) * 9, iboux3ot=(1,) * 9),
+}
+
+
+@pytest.mark.parametrize("which", sorted(GHOST_SETS))
+def test_ghost_fill_every_cell(which, oracle_lib):
+    """bnd4 against the index-map ghost fill on every cell, edges and corners
+    included, for every variable and every boundary code the oracle builds.
+    With every code present, swapping the axis order changes the corners, so
+    the comparison pins the x1 -> x2 -> x3 order too (the torus's copy in r
+    and mirror in theta commute at the corners)."""
+    p = replace(configs.G_MP5, imax=13, jmax=12, kmax=14, **GHOST_SETS[which]).resolved()
+    o = Oracle(p)
+    rng = np.random.default_rng(21)
+    u = np.asfortranarray(rng.standard_normal((9, p.imax, p.jmax, p.kmax)))
+    uo = np.asfortranarray(rng.standard_normal((9, p.imax, p.jmax, p.kmax)))
+    want = ghost_fill_by_index(u, uo, p)
+    got = u.copy(order="F")
+    o.call("bnd4", got, uo, p.nm1)
+    assert np.array_equal(got.view(np.uint64), want.view(np.uint64))
+    if which == "codes":
+        swapped = ghost_fill_by_index(u, uo, p, order=(2, 1, 0))
+        assert not np.array_equal(swapped, want)
+
+
+def test_infchk_stops_inflow_at_the_outer_boundary(oracle_lib):
+    """infchk (bnd.f90:1636-1824) on a Kerr-Schild grid with inflow (v^r < 0)
+    and outflow in the outer r ghosts: where v^r < 0 the radial velocity is
+    replaced by beta^r / alpha, i.e. the contravariant u^r = u~^r - gamma
+    beta^r / alpha becomes zero; v^theta, v^phi are kept; the new u~ is
+    gamma v with gamma = 1/sqrt(1 - v^2), v^2 clamped at 1 - 1/10^2 (so
+    gamma = 10) where it reaches 1; every other cell, the inner r ghosts
+    (whose branch writes nothing) and the other variables are unchanged."""
+    p = replace(configs.T3D, imax=14, jmax=10, kmax=12).resolved()
+    g = native.setup_grid(p)
+    m = native.setup_metric(p, g)
+    o = Oracle(p)
+    rng = np.random.default_rng(8)
+    uri = np.asfortranarray(rng.standard_normal((9, p.imax, p.jmax, p.kmax)) * 0.3)
+    uri[0] = 1.0
+    uri[4] = 0.1
+    n1 = p.nm1
+    # a quarter of the outer ghosts get a large tangential speed, so that
+    # v^2 >= 1 once v^r is reset and the clamp acts
+    big = rng.random((n1, p.jmax, p.kmax)) < 0.25
+    uri[3, p.imax - n1:][big] = 40.0
+    w = uri.copy(order="F")
+    o.call("infchk", w, m["gcov"], m["gcon"], n1)
+
+    def metric(name, i, j, k):
+        return m[name][:, :, i, j, k]
+
+    def quad(gc, v):
+        # g_ij v^i v^j summed m-outer, n-inner as physfunc.f90:17-21 and
+        # bnd.f90:1795-1800 do (v^2 near 1 amplifies the summation order)
+        acc = 0.0
+        for a in range(3):
+            for b in range(3):
+                acc = acc + gc[1 + a, 1 + b] * v[a] * v[b]
+        return acc
+
+    def gamma_of(ut, gc):
+        # calgfl (physfunc.f90:2-41): g_ij u~^i u~^j clamped to [.., 500]
+        # (small negatives taken absolute, larger ones set to 1e-10)
+        utsq = quad(gc, ut)
+        if utsq < 0.0:
+            utsq = abs(utsq) if abs(utsq) <= 1e-10 else 1e-10
+        return np.sqrt(1.0 + min(utsq, 500.0))
+
+    changed = clamped = kept = 0
+    for i in range(p.imax):
+        for j in range(p.jmax):
+            for k in range(p.kmax):
+                before, after = uri[:, i, j, k], w[:, i, j, k]
+                gc, gi = metric("gcov", i, j, k), metric("gcon", i, j, k)
+                ut = before[1:4]
+                gam0 = gamma_of(ut, gc)
+                v0 = ut * (1.0 / gam0)          # bnd.f90 multiplies by reciprocals
+                if i < p.imax - n1 or v0[0] >= 0.0:
+                    assert np.array_equal(after, before), (i, j, k)
+                    kept += 1
+                    continue
+                assert np.array_equal(np.delete(after, [1, 2, 3]), np.delete(before, [1, 2, 3]))
+                alpha = 1.0 / np.sqrt(-gi[0, 0])
+                beta_r = alpha * alpha * gi[0, 1]
+                v = np.array([beta_r * (1.0 / alpha), v0[1], v0[2]])
+                vsq = quad(gc, v)
+                gam = 10.0 if vsq >= 1.0 else 1.0 / np.sqrt(1.0 - vsq)
+                clamped += vsq >= 1.0
+                np.testing.assert_allclose(after[1:4], gam * v, rtol=1e-13, atol=1e-14)
+                if vsq < 1.0 and gam * gam - 1.0 < 500.0:
+                    # the new u~ is self-consistent (below calgfl's clamp):
+                    # gamma from it equals gamma, and the radial 4-velocity
+                    # vanishes
+                    np.testing.assert_allclose(gamma_of(after[1:4], gc), gam, rtol=1e-11)
+                    ucon_r = after[1] - gam * beta_r / alpha
+                    assert abs(ucon_r) <= 1e-13 * max(1.0, abs(after[1])), (i, j, k, ucon_r)
+                changed += 1
+    assert changed > 10 and clamped > 3 and kept > changed
+
+
 def test_rk3_single_precision_constants(oracle_lib):
     """rk3trd uses the float32 literals 1./3. and 2./3. (rkt.f90:169-172)."""
     p = configs.G_MP5.resolved()

@@ -54,7 +54,7 @@ def test_routines_match_reference(name):
     assert bits_equal(wr, wo) and bits_equal(ur, uo)
 
 
-@pytest.mark.parametrize("name,steps", [("t2d", 3), ("t3d", 2), ("utube", 6), ("umc", 4), ("t2dmc", 3), ("uleis", 3),
+@pytest.mark.parametrize("name,steps", [("t2d", 3), ("t3d", 2), ("t3dppm", 6), ("utube", 6), ("umc", 4), ("t2dmc", 3), ("uleis", 3),
                                         ("t2dgam", 3), ("t2drc2", 3), ("urc2", 3)])
 def test_step_matches_reference(name, steps):
     """Composed step: reference routines in main.f90's order (bnd4/infchk from the
