@@ -10,12 +10,15 @@ the reference itself turns this torus to NaN within a few steps; --config c2
 runs MP5 anyway.)
 
 Prints ONE JSON line: zone-updates/s (interior zones x steps / wall time of
-the timed steps), the roofline of the dominant kernel from HIP-event timings
-taken on the model's stream during the timed steps (events around that
-kernel's launches only; the per-kernel breakdown comes from the warm-up
-steps, which time every kernel), and the CPU baseline (the reference's own
-Fortran routines on one core, on a phi-slab of the same workload, with the
-C restatement in oracle/ beside it).
+the timed steps); the roofline of the whole step (compulsory state bytes per
+zone-update x zone-updates/s against 8 TB/s), with the bytes the step really
+moves and its VALU issue time from the committed rocprofv3 counter runs, and
+the dominant kernel's own figure from HIP-event timings taken on the
+model's stream during the timed steps (events around that kernel's launches
+only; the per-kernel breakdown comes from the warm-up steps, which time
+every kernel); and the CPU baseline (the reference's own Fortran routines,
+one process per host core on phi slabs of the same workload, with the C
+restatement in oracle/ on one core beside it).
 """
 import argparse
 import json
@@ -74,6 +77,68 @@ def algorithmic_bytes(p, kind):
     return 0, 0.0, ""
 
 
+def step_bytes_per_zone_update(p):
+    """Compulsory HBM bytes of one zone-update (one interior zone advanced one
+    full step): every RK stage reads the stage's primitives W and conserved
+    state U (plus U^n from the second stage on) and writes U and W, 72 B each
+    (288 B at stage 1, 360 B after: RK2 648 B, RK3 1008 B), plus the
+    axisymmetric metric tables, 103 doubles per (r, theta) column per stage
+    (3 locations x (10 + 10 + 1) + 40 Christoffel symbols = 824 B), shared by
+    the column's interior phi cells."""
+    per_stage = [288.0] + [360.0] * (p.irkt - 1)
+    nphi = max(p.jmax - 2 * p.nm1, 1)
+    return sum(per_stage) + p.irkt * 824.0 / nphi
+
+
+PROFILE_NOTE = ("rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE passes over this bench's command "
+                "(profiles/hbm_traffic_<config>.md), not the timed run")
+
+
+def profile_figures(config):
+    """Per-kernel HBM traffic (bytes per launch) and VALU issue figures taken
+    from the committed rocprofv3 counter runs of this workload
+    (profiles/traffic.json), or {} when there are none for `config`."""
+    tfile = os.path.join(ROOT, "profiles", "traffic.json")
+    if not os.path.exists(tfile):
+        return {}
+    with open(tfile) as f:
+        tr = json.load(f)
+    return tr.get(config, {})
+
+
+def step_roofline(p, zups, s_per_step, prof, dominant):
+    """The whole step against HBM: compulsory bytes per zone-update times
+    zone-updates/s over the 8 TB/s peak.  Beside it: the HBM bytes the step
+    really moves (sum of the per-kernel counter figures of the profile run),
+    the step's VALU issue time at one wave instruction per clock per CU (from
+    the same profile), and the dominant kernel's own figure."""
+    A = step_bytes_per_zone_update(p)
+    achieved = zups * A / 1e9
+    nint = (p.imax - 2 * p.nm1) * (p.jmax - 2 * p.nm1) * (p.kmax - 2 * p.nm1)
+    out = {
+        "bound": "hbm",
+        "scope": "whole time step (CFL reduction + RK stages): compulsory state bytes per zone-update",
+        "achieved": achieved,
+        "peak": HBM_PEAK_GBS,
+        "unit": "GB/s",
+        "frac": achieved / HBM_PEAK_GBS,
+        "algorithmic_bytes_per_zone_update": A,
+        "zone_updates_per_step": nint,
+        "avg_step_ms": s_per_step * 1e3,
+        "traffic": prof.get("step_bytes"),
+        "traffic_over_algorithmic": (prof["step_bytes"] / (A * nint)) if prof.get("step_bytes") else None,
+        "traffic_source": PROFILE_NOTE if prof.get("step_bytes") else None,
+        "valu": None,
+        "dominant_kernel": dominant,
+    }
+    v = prof.get("valu")
+    if v:
+        # the fraction of the step its VALU instructions alone need
+        out["valu"] = dict(v, issue_frac=v["issue_floor_ms_per_step"] / (s_per_step * 1e3),
+                           fp64_issue_frac=v["fp64_issue_floor_ms_per_step"] / (s_per_step * 1e3))
+    return out
+
+
 def cpu_baseline(p, phi_cells):
     """The oracle (C restatement, one core) on a bounded sample of the same
     workload: the same r-theta grid and torus with `phi_cells` interior phi
@@ -101,32 +166,75 @@ def cpu_baseline(p, phi_cells):
     return nint / el, el, q
 
 
-def cpu_baseline_reference(name):
-    """The reference's own routines (oracle/_ref/<name>: the Fortran sources
-    built with flang -O2, one core) stepped in main.f90:743-1149's order, with
-    the grid, metric and Christoffel symbols from its own grid.f90 / coord1 /
-    calchrist; bnd4 and infchk come from the oracle (bnd.f90 needs mpif.h).
-    The same torus; one warm-up step, then two timed steps.  Returns
-    (zone-updates/s, seconds timed, pram), or None where the build is absent."""
+def _ref_sample(name):
+    """Set-up of one reference-build sample: the reference's own grid, metric
+    and Christoffel arrays, the seeded torus, its conserved variables and a
+    stepper.  Returns (stepper, uu, uri, pram)."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
-    from harness import Oracle, RefStepper, fzeros, grid_and_metric, have_ref  # noqa: E402
+    from harness import Oracle, RefStepper, fzeros, grid_and_metric  # noqa: E402
     from raishinsource_amd import native
 
-    if not have_ref(name):
-        return None
     r, g, m = grid_and_metric(name)
     q = r.p
     uri = native.setup_torus(q, g, m, seed=0, noise=0.02)
     uu = fzeros(9, q.imax, q.jmax, q.kmax)
     Oracle(q).call("caluu1", uri, uu, m["gcov"], m["gcon"], m["detg"])
-    rs = RefStepper(name, g, m)
+    return RefStepper(name, g, m), uu, uri, q
+
+
+def _ref_worker(name, nsteps, barrier, out):
+    """One rank of the all-core CPU baseline: set up, one warm-up step, wait
+    for every rank, then time `nsteps` steps of its own slab."""
+    rs, uu, uri, q = _ref_sample(name)
     _, t = rs.step(uu, uri, 0.0)
+    barrier.wait()
     t0 = time.perf_counter()
-    for _ in range(2):
+    for _ in range(nsteps):
         _, t = rs.step(uu, uri, t)
-    el = time.perf_counter() - t0
+    out.put(time.perf_counter() - t0)
+
+
+def cpu_baseline_reference(name, ranks):
+    """The reference's own routines (oracle/_ref/<name>: the Fortran sources
+    built with flang -O2) stepped in main.f90:743-1149's order, with the
+    grid, metric and Christoffel symbols from its own grid.f90 / coord1 /
+    calchrist; bnd4 and infchk come from the oracle (bnd.f90 needs mpif.h).
+    `ranks` processes, one per host core, each stepping its own phi slab of
+    the same torus as one MPI rank of the reference's decomposition would
+    (mpisub.f90:6-59; the halo exchange between slabs is left out): one
+    warm-up step, then two timed steps, started together.  Aggregate
+    zone-updates/s = ranks x slab zones x 2 / slowest rank's time.  Returns
+    (zone-updates/s, seconds, pram), or None where the build is absent."""
+    import multiprocessing as mp
+
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    from harness import have_ref  # noqa: E402
+    from raishinsource_amd import configs
+
+    if not have_ref(name):
+        return None
+    q = configs.ALL[name].resolved()
+    ctx = mp.get_context("spawn")  # fresh interpreters: this process holds the GPU
+    bar, out = ctx.Barrier(ranks), ctx.Queue()
+    procs = [ctx.Process(target=_ref_worker, args=(name, 2, bar, out)) for _ in range(ranks)]
+    for pr in procs:
+        pr.start()
+    els = [out.get(timeout=600) for _ in procs]
+    for pr in procs:
+        pr.join()
+    el = max(els)
     nint = (q.imax - 2 * q.nm1) * (q.jmax - 2 * q.nm1) * (q.kmax - 2 * q.nm1)
-    return 2 * nint / el, el, q
+    return ranks * 2 * nint / el, el, q
+
+
+def host_cores():
+    """Host cores this process may use, at most 16 (one GPU's share of the
+    box's CPUs)."""
+    try:
+        n = len(os.sched_getaffinity(0))
+    except AttributeError:
+        n = os.cpu_count() or 1
+    return max(1, min(n, 16))
 
 
 def main():
@@ -215,14 +323,8 @@ def main():
     avg_ms = ms_tot / max(launches, 1)
     units, abytes, unit_name = algorithmic_bytes(p, dom)
     achieved = abytes / (avg_ms * 1e-3) / 1e9
-    traffic = None
-    tfile = os.path.join(ROOT, "profiles", "traffic.json")
-    if os.path.exists(tfile):
-        with open(tfile) as f:
-            tr = json.load(f)
-        key = f"{args.config}:{dom}"
-        if key in tr:
-            traffic = tr[key]
+    prof = profile_figures(args.config)
+    traffic = prof.get("traffic", {}).get(dom)
     kernel_ms = {k: round(v[0] / max(v[1], 1), 4) for k, v in kt_all.items()}  # per launch, warm-up steps
 
     model.close()
@@ -234,16 +336,19 @@ def main():
                "sample": f"one step (after one warm-up step) of the same torus on {q.imax}x{q.jmax}x{q.kmax} "
                          f"({args.cpu_phi} of the {p.jmax - 2 * p.nm1} interior phi cells), {el_cpu:.1f}s, "
                          "oracle/raishin_oracle.c single-threaded"}
-        ref = cpu_baseline_reference("c2cpu") if p.name == "c2ppm" else None
+        ranks = host_cores()
+        ref = cpu_baseline_reference("c2cpu", ranks) if p.name == "c2ppm" else None
         if ref is not None:
             rv, rel, rq = ref
-            print(f"[bench] cpu baseline: reference build {rv:.3g} zone-updates/s ({rel:.1f}s)", file=sys.stderr,
-                  flush=True)
-            cpu = {"value": rv, "unit": "zone-updates/s", "cores": 1, "kind": "reference",
-                   "sample": f"two steps (after one warm-up step) of the same torus on {rq.imax}x{rq.jmax}x{rq.kmax} "
-                             f"({rq.jmax - 2 * rq.nm1} of the {p.jmax - 2 * p.nm1} interior phi cells), "
-                             f"{rel:.1f}s: the reference's Fortran routines (flang -O2, oracle/_ref/c2cpu) in "
-                             "main.f90's step order, bnd4/infchk from the oracle (bnd.f90 needs mpif.h)",
+            print(f"[bench] cpu baseline: reference build on {ranks} cores {rv:.3g} zone-updates/s ({rel:.1f}s)",
+                  file=sys.stderr, flush=True)
+            cpu = {"value": rv, "unit": "zone-updates/s", "cores": ranks, "kind": "reference",
+                   "sample": f"{ranks} processes, one per host core, each stepping its own "
+                             f"{rq.imax}x{rq.jmax}x{rq.kmax} phi slab of the same torus "
+                             f"({rq.jmax - 2 * rq.nm1} of the {p.jmax - 2 * p.nm1} interior phi cells) two steps "
+                             f"after one warm-up step, started together; slowest rank {rel:.1f}s: the reference's "
+                             "Fortran routines (flang -O2, oracle/_ref/c2cpu) in main.f90's step order, bnd4/infchk "
+                             "from the oracle (bnd.f90 needs mpif.h), no halo exchange between slabs",
                    "port": cpu}
 
     line = {
@@ -267,19 +372,19 @@ def main():
             "interior_zones": nint,
             "parallelism": "single GPU",
         },
-        "roofline": {
-            "bound": "hbm",
+        "roofline": step_roofline(p, zups, el / args.steps, prof, {
             "kernel": dom,
             "achieved": achieved,
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
             "traffic": traffic,
+            "traffic_source": PROFILE_NOTE if traffic is not None else None,
             "algorithmic_bytes_per_launch": abytes,
             "units_per_launch": units,
             "unit_name": unit_name,
             "avg_launch_ms": avg_ms,
-        },
+        }),
         "kernel_avg_ms": kernel_ms,
         "cpu_baseline": cpu,
         "workgroup_sizes": block_sizes,
@@ -287,7 +392,7 @@ def main():
         "counters": counters,
     }
     if args.mixed_steps > 0:
-        line["mixed"] = mixed_precision(p, g, m, uri, args, el / args.steps)
+        line["mixed"] = mixed_precision(p, g, m, uri, args, el / args.steps, dom)
     if args.ensemble > 0:
         line["ensemble"] = ensemble(args)
     if args.serve_rates:
@@ -306,30 +411,18 @@ def ensemble(args):
     model queued round robin, and as one batch (one launch per kernel for all
     models, two steps per graph launch), in FP64 and mixed precision.  The
     FP64 results of the three must be bit-identical."""
-    from dataclasses import replace
-
     import numpy as np
 
-    from raishinsource_amd import configs, native
+    from raishinsource_amd import native, serving
 
-    rng = np.random.default_rng(2024)
-    sizes = [(64, 64), (128, 128), (256, 128), (512, 256)]
     steps = max(2, args.ensemble_steps - args.ensemble_steps % 2)
     t_set = time.perf_counter()
     models, states, zones = [], [], 0
-    for q in range(args.ensemble):
-        ni, nk = sizes[rng.integers(len(sizes))]
-        a = float(rng.uniform(0.0, 0.99))
-        beta = float(rng.choice([10.0, 100.0, 1000.0]))
-        rin = float(rng.uniform(6.0, 10.0))
-        p = replace(configs.C1, name=f"ens{q}", imax=ni + 2 * configs.C1.nm1, kmax=nk + 2 * configs.C1.nm1,
-                    akm=a, xmin=None).resolved()
-        g = native.setup_grid(p)
-        mod = native.Model.from_grid(p, g)  # metric tables built on the device
-        uri = native.setup_torus(p, g, mod.metric_row(), rin=rin, rmax=rin + 6.0, beta=beta, noise=0.02, seed=q)
+    for p, init in serving.make_ensemble(args.ensemble, seed=2024):
+        mod, uri = serving.build_torus(p, init)
         models.append(mod)
         states.append(uri)
-        zones += ni * nk
+        zones += (p.imax - 2 * p.nm1) * (p.kmax - 2 * p.nm1)
     # the autotuner once per grid shape; the other models of that shape take its choice
     tuned = {}
     if not args.no_autotune:
@@ -499,7 +592,7 @@ def divb_max(uu, g, p):
     return float(np.abs(div).max())
 
 
-def mixed_precision(p, g, m, uri, args, s_per_step_fp64):
+def mixed_precision(p, g, m, uri, args, s_per_step_fp64, dom):
     """The mixed form (reconstruction and face fluxes in FP32): its rate on
     the same timed steps, and its per-field errors against the FP64 run after
     args.mixed_steps steps from the same initial state."""
@@ -512,18 +605,22 @@ def mixed_precision(p, g, m, uri, args, s_per_step_fp64):
     model.set_precision(True)
     sizes = model.block_sizes() if args.no_autotune else model.autotune(uri, steps=2)
     # as the FP64 run: the warm-up steps time every kernel kind, the timed
-    # steps carry no events
+    # steps carry events around the launches of the FP64 run's dominant
+    # kernel kind only, so both rates are taken under the same events
     model.set_timing(True)
     for _ in range(args.warmup):
         model.step(1, sync=False)
     model.sync()
     kt = model.get_timing()
     model.set_timing(False)
+    if dom is not None:
+        model.set_timing(kinds=[dom])
     model.sync()
     t0 = time.perf_counter()
     model.step(args.steps, sync=False)
     model.sync()
     el = time.perf_counter() - t0
+    model.set_timing(False)
     model.close()
     nint = (p.imax - 2 * p.nm1) * (p.jmax - 2 * p.nm1) * (p.kmax - 2 * p.nm1)
     print(f"[bench] mixed: timed {args.steps} steps: {el:.3f}s", file=sys.stderr, flush=True)

@@ -129,9 +129,10 @@ void *raishin_model_stream(raishin_model *m);
  * HLLE): 0 = FP64, the reference's arithmetic (default); 1 = mixed: those
  * two in FP32 on the vector ALUs, while the metric tables, flux-CT, the RK
  * accumulation, the sources, recov1a, the floors and the state stay FP64.
- * The face fluxes are stored as float in mode 1 (flux-CT averages them in
- * FP64); a change of mode converts the stored fluxes, synchronises the
- * device and allocates a temporary buffer for the conversion.
+ * The face fluxes are stored as float in mode 1 (flux-CT reads them and
+ * averages in FP64, and writes its result back as float); a change of mode
+ * converts the stored fluxes in place, in order on the model's stream, with
+ * no allocation and no device-wide wait.  Call it between steps.
  * Not part of the reference; its error against FP64 is measured by bench.py
  * (its "mixed" object, --mixed-steps).  Takes effect from the next stage. */
 void raishin_model_set_precision(raishin_model *m, int mode);
@@ -154,6 +155,11 @@ void raishin_model_set_timing_mask(raishin_model *m, unsigned mask);
  * primitives as after recov() (main.f90:861; interior cells).  Calling
  * stages 1..irkt in turn is one raishin_model_step. */
 void raishin_model_debug_stage(raishin_model *m, int stage, double *ww, double *wpre);
+/* The stored face fluxes, as the next sweep would find them: ww(3,9,imax,
+ * jmax,kmax) in the layout of main.f90's ww and the six transverse-B flux
+ * copies that flux-CT averages, ct(6,imax,jmax,kmax); either may be NULL.
+ * In the mixed form they are stored as float and returned widened. */
+void raishin_model_debug_fluxes(raishin_model *m, double *ww, double *ct);
 int raishin_model_get_timing(raishin_model *m, double *ms, long long *launches, int nkinds);
 
 /* Workgroup sizes of the tuned launches, in the order: PPM face values,

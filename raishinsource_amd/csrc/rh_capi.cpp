@@ -1,5 +1,6 @@
 // C-ABI of the MI355X-native RAISHIN GRMHD step: resident models (the
 // production path).  The reference-named entry points are in rh_ref.cpp.
+#include <algorithm>
 #include <cstring>
 #include <mutex>
 #include <string>
@@ -276,30 +277,28 @@ bool read_time(raishin_model* m, double* time, double* dt_last, const char* who)
 
 // The flux buffers keep the last fluxes written (a sweep reads its slot back
 // where the wave speeds are unordered), and the mixed form stores them as
-// float: a change of form converts them.
+// float: a change of form converts them in place, nine planes at a time,
+// staged through the recovered-primitive buffer (every stage rewrites all of
+// it before reading it, so it holds nothing between steps).  Narrowing walks
+// the chunks upwards and widening downwards, so a chunk's output overlaps
+// only chunks already staged.  Everything runs in order on the model's
+// stream; a batch that steps the model orders the member's stream after its
+// own work, so no device-wide wait and no allocation is needed.
 bool retype_fluxes(raishin_model* m, bool to_float) {
   if (!m->F || !m->CT) return true;
   const long long N = m->G.N;
-  // a batch may still be stepping this model on its own stream
-  HIPCHK(hipDeviceSynchronize());
-  double* tmp = nullptr;
-  HIPCHK(hipMalloc((void**)&tmp, (size_t)27 * N * sizeof(double)));
-  bool ok = true;
-  const struct { double* p; long long planes; } bufs[] = {{m->F, 27}, {m->CT, 6}};
+  const size_t es = to_float ? sizeof(double) : sizeof(float);  // element size as stored now
+  const size_t ds = to_float ? sizeof(float) : sizeof(double);  // element size after
+  const struct { double* p; int planes; } bufs[] = {{m->F, 27}, {m->CT, 6}};
   for (const auto& b : bufs) {
-    const long long n = b.planes * N;
-    const size_t bytes = (size_t)n * (to_float ? sizeof(double) : sizeof(float));  // as stored now
-    if (hipMemcpyAsync(tmp, b.p, bytes, hipMemcpyDeviceToDevice, m->st) != hipSuccess ||
-        launch_retype(tmp, b.p, n, to_float, m->st) != hipSuccess) {
-      ok = false;
-      break;
+    const int nch = (b.planes + 8) / 9;
+    for (int q = 0; q < nch; q++) {
+      const int c = to_float ? q : nch - 1 - q;
+      const long long p0 = 9LL * c, np = std::min(9, b.planes - 9 * c), n = np * N;
+      char* base = reinterpret_cast<char*>(b.p);
+      HIPCHK(hipMemcpyAsync(m->Wpre, base + (size_t)p0 * N * es, (size_t)n * es, hipMemcpyDeviceToDevice, m->st));
+      HIPCHK(launch_retype(m->Wpre, base + (size_t)p0 * N * ds, n, to_float, m->st));
     }
-  }
-  const hipError_t e = hipStreamSynchronize(m->st);
-  (void)hipFree(tmp);
-  if (!ok || e != hipSuccess) {
-    set_error("raishin_model_set_precision: converting the stored fluxes failed");
-    return false;
   }
   return true;
 }
@@ -391,6 +390,10 @@ void raishin_model_debug_stage(raishin_model* m, int stage, double* ww, double* 
     return run_stage(m, 4, m->Ucur, m->Us, m->Unext, &tap);
   };
   if (go() && stage == m->p.irkt) std::swap(m->Ucur, m->Unext);
+}
+
+void raishin_model_debug_fluxes(raishin_model* m, double* ww, double* ct) {
+  (void)copy_fluxes_out(m, ww, ct);
 }
 
 void raishin_model_set_precision(raishin_model* m, int mode) {

@@ -360,6 +360,62 @@ RH_DEV void b_mc_edges(const Geo& G, const EdgeArgs& A, long long blk) {
   }
 }
 
+// Face physics and HLLE of interface c+1/2 along D from its two face states
+// (wl: the left cell's right edge, wr: the right cell's left edge):
+// caluu3 -> calflx3 -> calcha -> hlle, written to F (and the transverse B
+// fluxes to their CTsrc planes).
+template <int D, class R, int ICHA>
+RH_DEV void sweep_face(const Geo& G, R* __restrict__ F, R* __restrict__ CTsrc, long long c, int i, int j, int k,
+                       const Met& M, R gam, const R wl[9], const R wr[9]) {
+  const long long N = G.N;
+  const R detg = R(M.detg());
+  R Ul[9], Fl[9], vpl, vml;
+  {
+    PrelT<R> sl;
+    prelude<R>(wl, gam, M, sl);
+    const R pw = powr(sl.de, gam);
+    tmunu_row<R>(sl, 0, detg, pw, Ul);
+    tmunu_row<R>(sl, D, detg, pw, Fl);
+    face_speed<R, ICHA>(sl, D, gam, M, vpl, vml);
+  }
+  R Ur[9], Fr[9], vpr, vmr;
+  {
+    PrelT<R> sr;
+    prelude<R>(wr, gam, M, sr);
+    const R pw = powr(sr.de, gam);
+    tmunu_row<R>(sr, 0, detg, pw, Ur);
+    tmunu_row<R>(sr, D, detg, pw, Fr);
+    face_speed<R, ICHA>(sr, D, gam, M, vpr, vmr);
+  }
+  const R cmax = fmaxf_(fmaxf_(R(0), vpr), vpl);
+  const R cmin = (D == 3) ? fminf_(fminf_(R(0), vpr), vml)   // calcha.f90:107
+                          : fminf_(fminf_(R(0), vmr), vml);
+  R* Fo = F + c;
+  // transverse B fluxes of this direction and their CTsrc planes
+  // (0 ww(2,7) 1 ww(3,7) 2 ww(1,8) 3 ww(3,8) 4 ww(1,9) 5 ww(2,9))
+  constexpr int nA = (D == 1) ? 7 : 6, nB = (D == 3) ? 7 : 8;
+  constexpr int pA = (D == 1) ? 2 : (D == 2) ? 0 : 1, pB = (D == 1) ? 4 : (D == 2) ? 5 : 3;
+  const bool in_ct = i >= G.nm1 && i <= G.ni - G.nm1 && j >= G.nm1 && j <= G.nj - G.nm1 && k >= G.nm1 &&
+                     k <= G.nk - G.nm1;
+#pragma unroll
+  for (int n = 0; n < 9; n++) {
+    double o;
+    if (cmin > R(0)) o = double(Fl[n]);
+    else if (cmin <= R(0) && cmax > R(0)) {
+      const R c1 = fmaxf_(absr(cmax), absr(cmin));
+      o = double(R(0.5) * (Fl[n] + Fr[n] - c1 * (Ur[n] - Ul[n])));
+    } else if (cmax <= R(0)) o = double(Fr[n]);
+    else o = Fo[(long long)n * N];  // unordered (NaN) speeds: hlle keeps ww, then flushes it
+    if (absr(o) < 1.0e-40) o = 0.0;
+    if (n == nA || n == nB) {
+      CTsrc[(long long)(n == nA ? pA : pB) * N + c] = R(o);
+      if (!in_ct) Fo[(long long)n * N] = R(o);  // inside the CT box fluxct writes this slot
+    } else {
+      Fo[(long long)n * N] = R(o);
+    }
+  }
+}
+
 // ------------------------------------------------------------------------
 // Direction sweep: reconstruction + face fluxes + wave speeds + HLLE.
 // rec2 (rec.f90:2-142) -> caluu3/calflx3* -> calcha (calcha.f90:2-126) -> hlle
@@ -416,52 +472,7 @@ RH_DEV void b_sweep(const Geo& G, const SweepArgs& A, long long blk) {
     }
   }
   const Met M{metF, G.P2, q2(i, k, G)};
-  const R detg = R(M.detg());
-  R Ul[9], Fl[9], vpl, vml;
-  {
-    PrelT<R> sl;
-    prelude<R>(wl, gam, M, sl);
-    const R pw = powr(sl.de, gam);
-    tmunu_row<R>(sl, 0, detg, pw, Ul);
-    tmunu_row<R>(sl, D, detg, pw, Fl);
-    face_speed<R, ICHA>(sl, D, gam, M, vpl, vml);
-  }
-  R Ur[9], Fr[9], vpr, vmr;
-  {
-    PrelT<R> sr;
-    prelude<R>(wr, gam, M, sr);
-    const R pw = powr(sr.de, gam);
-    tmunu_row<R>(sr, 0, detg, pw, Ur);
-    tmunu_row<R>(sr, D, detg, pw, Fr);
-    face_speed<R, ICHA>(sr, D, gam, M, vpr, vmr);
-  }
-  const R cmax = fmaxf_(fmaxf_(R(0), vpr), vpl);
-  const R cmin = (D == 3) ? fminf_(fminf_(R(0), vpr), vml)   // calcha.f90:107
-                          : fminf_(fminf_(R(0), vmr), vml);
-  R* Fo = F + c;
-  // transverse B fluxes of this direction and their CTsrc planes
-  // (0 ww(2,7) 1 ww(3,7) 2 ww(1,8) 3 ww(3,8) 4 ww(1,9) 5 ww(2,9))
-  constexpr int nA = (D == 1) ? 7 : 6, nB = (D == 3) ? 7 : 8;
-  constexpr int pA = (D == 1) ? 2 : (D == 2) ? 0 : 1, pB = (D == 1) ? 4 : (D == 2) ? 5 : 3;
-  const bool in_ct = i >= G.nm1 && i <= G.ni - G.nm1 && j >= G.nm1 && j <= G.nj - G.nm1 && k >= G.nm1 &&
-                     k <= G.nk - G.nm1;
-#pragma unroll
-  for (int n = 0; n < 9; n++) {
-    double o;
-    if (cmin > R(0)) o = double(Fl[n]);
-    else if (cmin <= R(0) && cmax > R(0)) {
-      const R c1 = fmaxf_(absr(cmax), absr(cmin));
-      o = double(R(0.5) * (Fl[n] + Fr[n] - c1 * (Ur[n] - Ul[n])));
-    } else if (cmax <= R(0)) o = double(Fr[n]);
-    else o = Fo[(long long)n * N];  // unordered (NaN) speeds: hlle keeps ww, then flushes it
-    if (absr(o) < 1.0e-40) o = 0.0;
-    if (n == nA || n == nB) {
-      CTsrc[(long long)(n == nA ? pA : pB) * N + c] = R(o);
-      if (!in_ct) Fo[(long long)n * N] = R(o);  // inside the CT box fluxct writes this slot
-    } else {
-      Fo[(long long)n * N] = R(o);
-    }
-  }
+  sweep_face<D, R, ICHA>(G, F, CTsrc, c, i, j, k, M, gam, wl, wr);
 }
 
 // ------------------------------------------------------------------------

@@ -535,6 +535,12 @@ bool run_stage(raishin_model* m, int rkstage, const double* Ua, const double* Ub
   return true;
 }
 
+bool copy_fluxes_out(raishin_model* m, double* ww, double* ct) {
+  for (int d = 0; ww && d < 3; d++)
+    if (!copy_planes_out(m, flux_plane(m, m->F, (long long)d * 9), 9, 3, d, ww, m->mixed)) return false;
+  return !ct || copy_planes_out(m, m->CT, 6, 1, 0, ct, m->mixed);
+}
+
 bool run_cfl(raishin_model* m, const double* W) {
   CflArgs C{};
   C.W = W; C.metC = m->met[0]; C.dx1b = m->dxb[0]; C.dx2b = m->dxb[1]; C.dx3b = m->dxb[2];

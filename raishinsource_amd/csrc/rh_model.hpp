@@ -133,6 +133,10 @@ bool run_step(raishin_model* m);
 // cdtcfl4 of primitives W: dtcfl, dt1..dt3 to the host (synchronises).
 bool cdtcfl4_on(raishin_model* m, const double* W, double* dtcfl, double* dt1, double* dt2, double* dt3);
 
+// The stored face fluxes to the host: ww(3,9,imax,jmax,kmax) and the six
+// transverse-B copies of flux-CT, ct(6,imax,jmax,kmax); either may be null.
+bool copy_fluxes_out(raishin_model* m, double* ww, double* ct);
+
 struct StageTap {  // optional host copies of a stage's intermediates (reference layouts)
   double *ww = nullptr, *wpre = nullptr;
 };

@@ -55,7 +55,8 @@ EXPORTS = (
     "raishin_model_counters", "raishin_model_reset_counters", "raishin_model_sync", "raishin_model_stream",
     "raishin_cdtcfl4", "raishin_init", "raishin_grid", "raishin_coord1", "raishin_calchrist",
     "raishin_mdtorus1", "raishin_model_set_timing", "raishin_model_get_timing",
-    "raishin_model_debug_stage", "raishin_model_set_precision", "raishin_model_step_graph",
+    "raishin_model_debug_stage", "raishin_model_debug_fluxes", "raishin_model_set_precision",
+    "raishin_model_step_graph",
     "raishin_models_step_graph", "raishin_model_device_bytes", "raishin_batch_create", "raishin_batch_destroy",
     "raishin_batch_set", "raishin_batch_step", "raishin_batch_step_graph", "raishin_batch_times",
     "raishin_batch_sync", "raishin_mdshoctub", "raishin_model_set_block_sizes", "raishin_model_get_block_sizes",
@@ -128,6 +129,7 @@ def lib():
         L.raishin_model_get_timing.argtypes = [vp, C.POINTER(C.c_double), C.POINTER(C.c_longlong), C.c_int]
         L.raishin_model_get_timing.restype = C.c_int
         L.raishin_model_debug_stage.argtypes = [vp, C.c_int, vp, vp]
+        L.raishin_model_debug_fluxes.argtypes = [vp, vp, vp]
         L.raishin_model_set_precision.argtypes = [vp, C.c_int]
         L.raishin_model_step_graph.argtypes = [vp, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_double)]
         L.raishin_models_step_graph.argtypes = [C.POINTER(vp), C.c_int, C.c_int]
@@ -497,6 +499,16 @@ class Model:
         lib().raishin_model_debug_stage(self.h, st, _p(ww), _p(wp))
         check()
         return ww, wp
+
+    def fluxes(self):
+        """The stored face fluxes (ww(3,9,...) and flux-CT's six transverse-B
+        copies (6,...)), widened to FP64 in the mixed form."""
+        p = self.p
+        ww = fzeros(3, 9, p.imax, p.jmax, p.kmax)
+        ct = fzeros(6, p.imax, p.jmax, p.kmax)
+        lib().raishin_model_debug_fluxes(self.h, _p(ww), _p(ct))
+        check()
+        return ww, ct
 
     def set_precision(self, mixed):
         """False: FP64 throughout (the reference's arithmetic); True:

@@ -95,6 +95,38 @@ def footprint(p):
     return b
 
 
+def draw_torus2d(rng, name, seed):
+    """One 2D torus of the ensemble and of the serving mix: r x theta drawn
+    from TORUS_SIZES_2D, spin a ~ U[0, 0.99], beta_min from {10, 100, 1000},
+    r_in ~ U[6, 10], r_max = r_in + 6; C1's scheme (PPM, HLLE, flux-CT, RK2,
+    CFL 0.4).  Returns (pram, initial-data parameters with the noise seed)."""
+    ni, nk = TORUS_SIZES_2D[rng.integers(len(TORUS_SIZES_2D))]
+    a = float(rng.uniform(0.0, 0.99))
+    beta = float(rng.choice([10.0, 100.0, 1000.0]))
+    rin = float(rng.uniform(6.0, 10.0))
+    p = replace(configs.C1, name=name, imax=ni + 2 * configs.C1.nm1, kmax=nk + 2 * configs.C1.nm1, akm=a,
+                xmin=None)
+    return p, {"rin": rin, "rmax": rin + 6.0, "beta": beta, "seed": seed}
+
+
+def make_ensemble(n, seed=2024):
+    """n tori drawn by draw_torus2d from one generator (the noise seed of
+    member q is q).  Returns [(pram, initial-data parameters)]."""
+    rng = np.random.default_rng(seed)
+    return [draw_torus2d(rng, f"ens{q}", q) for q in range(n)]
+
+
+def build_torus(p, init):
+    """Model (metric tables built on the device) and initial primitives of a
+    torus member: seeded 2% pressure noise."""
+    p = p.resolved()
+    g = native.setup_grid(p)
+    mod = native.Model.from_grid(p, g)
+    uri = native.setup_torus(p, g, mod.metric_row(), rin=init["rin"], rmax=init["rmax"], beta=init["beta"],
+                             noise=0.02, seed=init["seed"])
+    return mod, uri
+
+
 def make_requests(n, rate, seed=0, mix=(0.6, 0.3, 0.1), mixed_fraction=0.5, torus2d_steps=500,
                   torus3d_steps=100, tube_t_end=0.4):
     """n requests with exponential inter-arrival times of mean 1/rate."""
@@ -110,14 +142,8 @@ def make_requests(n, rate, seed=0, mix=(0.6, 0.3, 0.1), mixed_fraction=0.5, toru
             out.append(Request(rid, "tube", t, mixed, configs.TUBE, {"left": left, "right": right},
                                t_end=tube_t_end))
         elif u < mix[0] + mix[1]:
-            ni, nk = TORUS_SIZES_2D[rng.integers(len(TORUS_SIZES_2D))]
-            a = float(rng.uniform(0.0, 0.99))
-            beta = float(rng.choice([10.0, 100.0, 1000.0]))
-            rin = float(rng.uniform(6.0, 10.0))
-            p = replace(configs.C1, name=f"torus2d_{rid}", imax=ni + 2 * configs.C1.nm1,
-                        kmax=nk + 2 * configs.C1.nm1, akm=a, xmin=None)
-            out.append(Request(rid, "torus2d", t, mixed, p, {"rin": rin, "rmax": rin + 6.0, "beta": beta,
-                                                           "seed": rid}, nsteps=torus2d_steps))
+            p, init = draw_torus2d(rng, f"torus2d_{rid}", rid)
+            out.append(Request(rid, "torus2d", t, mixed, p, init, nsteps=torus2d_steps))
         else:
             g = 2 * configs.C1.nm1
             p = replace(configs.C1, name=f"torus3d_{rid}", imax=64 + g, jmax=64 + g, kmax=64 + g, ymax=None)

@@ -169,3 +169,31 @@ def test_workgroup_sizes_do_not_change_results():
     assert_same(ref, tuned, "tuned")
     b.close()
     close_all(ref, tuned)
+
+
+def test_batch_member_switches_precision_between_steps():
+    """A member switches form between batch steps (its stored fluxes are
+    converted on its own stream, which the batch waits for): bit for bit the
+    same as the same switches on the model stepped alone."""
+    members = ppm_members()[:3]
+    alone, batched = pairs(members)
+    b = native.Batch(batched)
+    for mm in alone:
+        mm.step(2)
+    b.step(2)
+    for grp in (alone, batched):
+        grp[0].set_precision(True)
+        grp[2].set_precision(False)
+    for mm in alone:
+        mm.step(3)
+    b.step(3)
+    assert_same(alone, batched, "after the switch")
+    for grp in (alone, batched):
+        grp[0].set_precision(False)
+    for mm in alone:
+        mm.step(2)
+    b.step_graph(1)
+    assert_same(alone, batched, "after switching back, graph")
+    b.set([])
+    b.close()
+    close_all(alone, batched)

@@ -65,3 +65,36 @@ def test_precision_round_trip_keeps_stored_fluxes(name):
     b.close()
     assert ta == tb
     assert np.array_equal(wa, wb, equal_nan=True) and np.array_equal(ua, ub, equal_nan=True)
+
+
+@pytest.mark.parametrize("name,nsteps", [("uppm", 2), ("t2d", 3), ("umc", 2)])
+def test_stored_flux_conversion(name, nsteps):
+    """A change of form converts the stored face fluxes (ww and flux-CT's
+    six transverse-B copies) in place: FP64 -> mixed rounds each value to
+    float, mixed -> FP64 widens it back exactly.  Read through the flux tap,
+    on states whose fluxes are finite and nonzero."""
+    p = configs.ALL[name].resolved()
+    g = native.setup_grid(p)
+    m = native.setup_metric(p, g)
+    uri = native.setup_torus(p, g, m, seed=0, noise=0.02)
+    mod = native.Model(p, g, m)
+    mod.set_state(uri)
+    mod.step(nsteps)
+    ww64, ct64 = mod.fluxes()
+    assert np.isfinite(ww64).all() and np.isfinite(ct64).all()
+    assert np.count_nonzero(ww64) > 1000 and np.count_nonzero(ct64) > 1000
+    exp_ww = ww64.astype(np.float32).astype(np.float64)
+    exp_ct = ct64.astype(np.float32).astype(np.float64)
+    assert not np.array_equal(exp_ww, ww64)  # the rounding is visible
+    mod.set_precision(True)
+    ww32, ct32 = mod.fluxes()
+    assert np.array_equal(ww32, exp_ww) and np.array_equal(ct32, exp_ct)
+    mod.set_precision(False)
+    ww_back, ct_back = mod.fluxes()
+    assert np.array_equal(ww_back, exp_ww) and np.array_equal(ct_back, exp_ct)
+    # a model that switched still steps: the state stays finite
+    mod.set_precision(True)
+    mod.step(2)
+    w, _, _ = mod.get_state()
+    mod.close()
+    assert np.isfinite(w).all()
