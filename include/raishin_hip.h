@@ -119,6 +119,20 @@ void raishin_models_step_graph(raishin_model *const *models, int nmodels, int np
 /* Only the CFL reduction (cdtcfl4 + MIN): dtcfl and dt1..dt3 of the current state. */
 void raishin_model_cdtcfl4(raishin_model *m, double *dtcfl, double *dt1, double *dt2, double *dt3);
 
+/* The same reduction with no allocation and no host synchronisation; no
+ * counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops).  out: 4
+ * doubles in device memory, written in order on the model's stream with
+ * dtcfl, dt1, dt2, dt3 (bit-identical to raishin_model_cdtcfl4).  The
+ * per-workgroup partials go to one fixed 8 MiB device array per GPU (the
+ * library's, held until the process ends) instead of the model's buffers:
+ * models of up to 131072 * 256 CFL cells ((imax-2)(jmax-2)(kmax-2) rounded
+ * up per 256); larger ones are rejected with an error.  The array needs no
+ * clearing (every call writes what it reads); calls that overlap on one GPU
+ * race on it, so order them (one stream, or events).  The model's device
+ * must be current.  Returns 0, or -1 with raishin_last_error set. */
+int raishin_model_cdtcfl4_noalloc(raishin_model *m, double *out);
+
 void raishin_model_counters(raishin_model *m, raishin_counters *c);
 void raishin_model_reset_counters(raishin_model *m);
 void raishin_model_sync(raishin_model *m);

@@ -241,6 +241,7 @@ static bool build_model_(raishin_model* m, const raishin_pram* p, const double* 
   G.njs = G.nj;
   G.jfix = 0;
   const long long N = G.N;
+  HIPCHK(hipGetDevice(&m->dev));
   HIPCHK(hipStreamCreateWithFlags(&m->st, hipStreamNonBlocking));
   m->carry_cap = (long long)(G.ni - 2 * G.nm1) * (G.nj - 2 * G.nm1) * (G.nk - 2 * G.nm1);
   for (int a = 0; a < 6; a++)

@@ -23,6 +23,7 @@ enum { K_SWEEP1 = 0, K_SWEEP2, K_SWEEP3, K_FLUXCT, K_UPDATE, K_CARRY, K_POST, K_
 
 struct raishin_model {
   raishin_pram p;
+  int dev = 0;  // the device the model was created on
   // per-kind HIP event pairs recorded around launches of the kinds whose bit
   // is set in timing (0: none); resolved on read
   unsigned timing = 0;

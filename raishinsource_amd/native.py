@@ -64,6 +64,7 @@ EXPORTS = (
     "raishin_store1", "raishin_restar2", "raishin_output1", "raishin_model_store1", "raishin_model_restar2",
     "raishin_model_output1", "raishin_model_get_pram", "raishin_model_create_ik", "raishin_mdtorus1_ik",
     "raishin_model_create_grid", "raishin_model_tables", "raishin_model_set_timing_mask",
+    "raishin_model_cdtcfl4_noalloc",
 )
 
 TUNE_SLOTS = ("ppm_edges", "sweep_i", "sweep_j", "sweep_k", "fluxct", "update", "recov", "post")
@@ -130,6 +131,8 @@ def lib():
         L.raishin_model_get_timing.restype = C.c_int
         L.raishin_model_debug_stage.argtypes = [vp, C.c_int, vp, vp]
         L.raishin_model_debug_fluxes.argtypes = [vp, vp, vp]
+        L.raishin_model_cdtcfl4_noalloc.argtypes = [vp, vp]
+        L.raishin_model_cdtcfl4_noalloc.restype = C.c_int
         L.raishin_model_set_precision.argtypes = [vp, C.c_int]
         L.raishin_model_step_graph.argtypes = [vp, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_double)]
         L.raishin_models_step_graph.argtypes = [C.POINTER(vp), C.c_int, C.c_int]
