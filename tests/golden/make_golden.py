@@ -6,6 +6,9 @@ routines (oracle/build_ref.py) and records, for small grids, the inputs and
 outputs of the hot-path routines and of short step sequences driven in
 main.f90's order.  The vectors are data only (npz).  Regenerate with
     python tests/golden/make_golden.py [unit] [c1]
+    python tests/golden/make_golden.py c2ppm-ref <work dir>     (hours; resumable)
+    python tests/golden/make_golden.py c2ppm-oracle <work dir>
+    python tests/golden/make_golden.py c2ppm <work dir>         (compares, writes traj_c2ppm)
 """
 import hashlib
 import os
@@ -174,21 +177,64 @@ COUNTER_NAMES = ("recov_fallback", "recov_fail", "recov_neg_rho", "recov_neg_p",
                  "recov_wg_carry", "atm_floor", "pmm_scale", "pmm_floor", "guard")
 
 
-def c2ppm_trajectory(nsteps=100):
-    """The 256 x 128 x 128 torus (PPM, HLLE, flux-CT, RK3, CFL 0.5, 2% pressure
-    noise from seed 0) stepped 100 times by the reference routines in
-    main.f90's order, with the oracle run beside it; the two must agree bit
-    for bit before anything is written."""
-    from concurrent.futures import ProcessPoolExecutor
-    import multiprocessing as mp
-    with ProcessPoolExecutor(1, mp_context=mp.get_context("fork")) as ex:
-        fo = ex.submit(_c2ppm_oracle, nsteps)
-        tr = trajectory("c2ppm", nsteps, snap=(10,), keep_snap_digest=True)
-        orc = fo.result()
+def c2ppm_reference(ckdir, nsteps=100, every=5):
+    """The reference routines stepping the 256 x 128 x 128 torus (PPM, HLLE,
+    flux-CT, RK3, CFL 0.5, 2% pressure noise from seed 0) in main.f90's
+    order, resumable: every `every` steps the state the next step needs
+    (uri, uu, the persistent flux array ww, time, the dt sequence) is saved
+    under ckdir.  Returns the summary of the final state."""
     p = configs.C2_PPM.resolved()
-    ref = summary(p, tr["uri"], tr["uu"])
-    ref.update(dt=tr["dt"], time=np.array(tr["time"]), uri10_digest=tr["uri_10"], uu10_digest=tr["uu_10"],
-               uri0_digest=digest(tr["uri0"]))
+    r, g, m = grid_and_metric("c2ppm")
+    ck = os.path.join(ckdir, "ref_checkpoint.npz")
+    rs = RefStepper("c2ppm", g, m)
+    extra = {}
+    if os.path.exists(ck):
+        z = np.load(ck)
+        uri, uu = np.asfortranarray(z["uri"]), np.asfortranarray(z["uu"])
+        rs.ww[...] = z["ww"]
+        t, dts, done = float(z["time"]), list(z["dt"]), int(z["steps"])
+        extra = {k: str(z[k]) for k in ("uri0_digest", "uri10_digest", "uu10_digest") if k in z}
+        print(f"reference: resuming after step {done}", flush=True)
+    else:
+        uri = native.setup_torus(p, g, m, seed=0)
+        uu = fzeros(9, p.imax, p.jmax, p.kmax)
+        r.call("caluu1", uri, uu, m["gcov"], m["gcon"], m["detg"], g["x1"], *r.bounds())
+        t, dts, done = 0.0, [], 0
+        extra["uri0_digest"] = digest(uri)
+    for s in range(done, nsteps):
+        dt, t = rs.step(uu, uri, t)
+        dts.append(dt)
+        print(f"reference step {s + 1} dt {dt!r}", flush=True)
+        if s + 1 == 10:
+            extra["uri10_digest"], extra["uu10_digest"] = digest(uri), digest(uu)
+        if (s + 1) % every == 0 and s + 1 < nsteps:
+            tmp = ck + ".tmp.npz"
+            np.savez(tmp, uri=uri, uu=uu, ww=rs.ww, time=np.array(t), dt=np.array(dts), steps=np.array(s + 1),
+                     **extra)
+            os.replace(tmp, ck)
+    out = summary(p, uri, uu)
+    out.update(dt=np.array(dts), time=np.array(t), **extra)
+    return out
+
+
+def c2ppm_oracle(ckdir, nsteps=100):
+    out = _c2ppm_oracle(nsteps)
+    np.savez(os.path.join(ckdir, "orc_final.npz"), **out)
+    return out
+
+
+def c2ppm_trajectory(ckdir, nsteps=100):
+    """The reference's 100-step run of the 256 x 128 x 128 torus with the
+    oracle's run of the same steps (each resumable or cached under ckdir);
+    the two must agree bit for bit before anything is written."""
+    fr = os.path.join(ckdir, "ref_final.npz")
+    fo = os.path.join(ckdir, "orc_final.npz")
+    if not os.path.exists(fr):
+        np.savez(fr, **c2ppm_reference(ckdir, nsteps))
+    if not os.path.exists(fo):
+        c2ppm_oracle(ckdir, nsteps)
+    ref = dict(np.load(fr))
+    orc = dict(np.load(fo))
     for k, v in ref.items():
         if k == "uri0_digest":
             continue
@@ -201,13 +247,23 @@ def c2ppm_trajectory(nsteps=100):
 
 
 def main(which):
-    for n in UNIT + ["t2d", "c1"] + (["c2ppm"] if "c2ppm" in which else []):
+    for n in UNIT + ["t2d", "c1"] + (["c2ppm"] if which and which[0].startswith("c2ppm") else []):
         if not have_ref(n):
             subprocess.run([sys.executable, os.path.join(ROOT, "oracle", "build_ref.py"), n], check=True,
                            stdout=subprocess.DEVNULL)
-    if "c2ppm" in which:
-        np.savez_compressed(os.path.join(HERE, "traj_c2ppm.npz"), **c2ppm_trajectory())
-        print("wrote traj_c2ppm")
+    if which and which[0] in ("c2ppm", "c2ppm-ref", "c2ppm-oracle"):
+        # make_golden.py c2ppm[-ref|-oracle] <work dir>: the runs take hours,
+        # so they keep their progress in the work directory
+        ckdir = which[1]
+        os.makedirs(ckdir, exist_ok=True)
+        if which[0] == "c2ppm-ref":
+            np.savez(os.path.join(ckdir, "ref_final.npz"), **c2ppm_reference(ckdir))
+        elif which[0] == "c2ppm-oracle":
+            c2ppm_oracle(ckdir)
+        else:
+            np.savez_compressed(os.path.join(HERE, "traj_c2ppm.npz"), **c2ppm_trajectory(ckdir))
+            print("wrote traj_c2ppm")
+        return
     if "c1" in which:
         np.savez_compressed(os.path.join(HERE, "traj_c1.npz"), **c1_trajectory())
         print("wrote traj_c1")

@@ -9,6 +9,7 @@ the device math library and glibc in some calls, and the fallback branch
 feeds that bit back into the other fields.
 """
 import ctypes as C
+import hashlib
 import os
 from dataclasses import replace
 
@@ -462,3 +463,61 @@ def test_full_subdomain_deterministic():
     b = gpu_run(p, g, m, uri, 2)
     assert bitdiff(a[0], b[0]) == 0 and bitdiff(a[1], b[1]) == 0
     assert a[4] == b[4]
+
+
+# ------------------------------------- 100 steps of the benched 3D workload
+
+
+def test_c2ppm_100_steps_match_reference():
+    """The benched workload itself -- 256 x 128 x 128 torus, 2% pressure
+    noise from seed 0, PPM, HLLE, flux-CT, RK3 -- stepped 100 times on the
+    device against the reference's Fortran routines stepped 100 times in
+    main.f90's order (tests/golden/traj_c2ppm.npz, written by
+    make_golden.py c2ppm with the oracle run beside it, bit for bit): the
+    initial state, the dt of every step, two whole phi rows of the
+    primitives and one of the conserved variables within 1e-11 per field,
+    per-field L1 / L-infinity / signed sums and per-phi-row L1 norms of the
+    interior within 1e-11 relative, the finite fraction, and the inversion
+    and floor counts."""
+    path = os.path.join(GOLD, "traj_c2ppm.npz")
+    if not os.path.exists(path):
+        pytest.skip("traj_c2ppm.npz not written yet (make_golden.py c2ppm)")
+    gold = np.load(path)
+    p = configs.C2_PPM.resolved()
+    g = native.setup_grid(p)
+    m = native.setup_metric(p, g, row=True)
+    uri = native.setup_torus(p, g, m, seed=0, noise=0.02)
+    assert hashlib.sha256(np.asfortranarray(uri).tobytes(order="F")).hexdigest() == str(gold["uri0_digest"])
+    mod = native.Model(p, g, m)
+    mod.set_state(uri)
+    dts = []
+    for _ in range(100):
+        _, dt = mod.step(1)
+        dts.append(dt)
+    w, u, t = mod.get_state()
+    cnt = mod.counters()
+    mod.close()
+    dts = np.array(dts)
+    assert_close("c2ppm dt", dts[None], gold["dt"][None])
+    assert abs(t - float(gold["time"])) <= TOL * abs(float(gold["time"]))
+    n1 = p.nm1
+    inner = w[:, n1:p.imax - n1, n1:p.jmax - n1, n1:p.kmax - n1]
+    fin = float(np.isfinite(inner).all(axis=0).mean())
+    assert fin == float(gold["finite_frac"]) and fin == 1.0
+    for nm, got in (("l1", np.abs(inner).sum(axis=(1, 2, 3))), ("linf", np.abs(inner).max(axis=(1, 2, 3))),
+                    ("sum", inner.sum(axis=(1, 2, 3)))):
+        ref = gold[nm]
+        rel = np.abs(got - ref) / np.maximum(np.abs(gold["l1"]) if nm == "sum" else np.abs(ref), 1e-300)
+        print(f"c2ppm {nm}: max rel {rel.max():.2e}")
+        assert rel.max() <= TOL, (nm, rel)
+    rows = np.abs(inner).sum(axis=(1, 3))
+    rel = np.abs(rows - gold["l1_rows"]) / np.abs(gold["l1_rows"]).max(axis=1, keepdims=True)
+    assert rel.max() <= TOL, rel.max()
+    for j in (5, 66):
+        assert_close(f"c2ppm uri row {j}", w[:, :, j, :], gold[f"uri_row{j}"])
+    assert_close("c2ppm uu row 5", u[:, :, 5, :], gold["uu_row5"])
+    names = [str(x) for x in gold["counter_names"]]
+    ref_cnt = dict(zip(names, (int(x) for x in gold["counters"])))
+    print("counters gpu", cnt, "reference run", ref_cnt)
+    for k in names:
+        assert cnt[k] == ref_cnt[k], k
