@@ -182,10 +182,30 @@ int raishin_model_get_timing(raishin_model *m, double *ms, long long *launches, 
  * depend on them.  get returns the sizes in use. */
 int raishin_model_set_block_sizes(raishin_model *m, const int *bs, int n);
 int raishin_model_get_block_sizes(raishin_model *m, int *bs, int n);
+/* Tiles of the PPM (irec = 5) sweeps.  split = 0 (default): each sweep
+ * computes its face values itself, through LDS; a wave of the j (k) sweep
+ * holds seg_j (seg_k) consecutive cells along j (k) by 64 / seg consecutive
+ * i, a power of two from 2 to 64 (0 keeps the default of 16).  split = 1:
+ * the face values of every cell go through HBM from a kernel of their own.
+ * Results do not depend on them.  0, or -1 with the error set. */
+int raishin_model_set_sweep_tiles(raishin_model *m, int split, int seg_j, int seg_k);
+int raishin_model_get_sweep_tiles(raishin_model *m, int *split, int *seg_j, int *seg_k);
+/* Face flux of the normal magnetic field component.  0 (default): the
+ * reference's HLLE flux, whose dissipation term c(U_r - U_l) is nonzero for
+ * the normal component (hll.f90:109-111) and which flux-CT leaves alone
+ * (ct.f90:98-109), so div B drifts.  1: that flux set to zero, so flux-CT
+ * keeps the corner divergence of sqrt(-g) B at FP64 round-off.  Form 1 is
+ * not the reference's arithmetic and has no counterpart there.  0, or -1
+ * with the error set. */
+int raishin_model_set_divb_form(raishin_model *m, int form);
 /* Autotuner: for each candidate size (64, 128, 256) runs one warm-up step
  * and `steps` timed steps from the state (uri, uu or NULL, time), keeps the
- * fastest size per launch slot, then sets the state again and clears the
- * counters, so the run starts as if untuned.  Returns 0 or -1. */
+ * fastest size per launch slot; for PPM models it then times the fused
+ * sweeps with tiles of 8, 16 and 32 cells along j and k and keeps the
+ * fastest tile per direction (the split form is never chosen: it is slower
+ * at every size measured, and members of a batch must agree on the form);
+ * then it sets the state again and clears the counters, so the run
+ * starts as if untuned.  Returns 0 or -1. */
 int raishin_model_autotune(raishin_model *m, const double *uri, const double *uu, double time, int steps);
 
 /* Axisymmetric models with one interior phi cell (jmax = 2*nm1+1), periodic

@@ -19,7 +19,12 @@ ARCH = os.environ.get("RAISHIN_OFFLOAD_ARCH", "gfx950")
 
 HIP_FLAGS = ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}", "-ffp-contract=off",
              "-fno-fast-math", "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
-             "-Wno-unused-command-line-argument"]
+             "-Wno-unused-command-line-argument",
+             # keep MachineLICM from hoisting every FP64 constant and plane
+             # offset out of the kernels' loops (the PPM sweep's walk along j
+             # and k, the Newton iterations): held across the loop they
+             # spill registers
+             "-mllvm", "-disable-machine-licm"]
 HOST_FLAGS = ["-O2", "-std=c++17", "-fPIC", "-ffp-contract=off", "-fno-fast-math", "-fopenmp"]
 
 FC = os.environ.get("FC_FLANG", "/opt/rocm/lib/llvm/bin/flang")

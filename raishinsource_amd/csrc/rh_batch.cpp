@@ -116,14 +116,14 @@ bool make_plan(raishin_batch* b) {
     const size_t len = rec[h][0].size();
     for (size_t q = 1; q < n; q++)
       if (rec[h][q].size() != len) {
-        set_error("raishin_batch: members make different step sequences (irec, irkt and the boundary types must agree)");
+        set_error("raishin_batch: members make different step sequences (irec, irkt, the boundary types and the PPM sweep form (split or fused) must agree)");
         return false;
       }
     for (size_t pos = 0; pos < len; pos++) {
       const int kind = rec[h][0][pos].kind;
       for (size_t q = 1; q < n; q++)
         if (rec[h][q][pos].kind != kind) {
-          set_error("raishin_batch: members make different step sequences (irec, irkt and the boundary types must agree)");
+          set_error("raishin_batch: members make different step sequences (irec, irkt, the boundary types and the PPM sweep form (split or fused) must agree)");
           return false;
         }
       // one launch per (variant, workgroup size) present at this position,

@@ -444,6 +444,59 @@ int raishin_model_get_block_sizes(raishin_model* m, int* bs, int n) {
   return 0;
 }
 
+int raishin_model_set_sweep_tiles(raishin_model* m, int split, int seg_j, int seg_k) {
+  if (!m || (split != 0 && split != 1) || (seg_j != 0 && !psweep_seg_ok(seg_j)) ||
+      (seg_k != 0 && !psweep_seg_ok(seg_k))) {
+    set_error("raishin_model_set_sweep_tiles: split must be 0 or 1 and seg_j, seg_k 0 or a power of two "
+              "from 2 to 64");
+    return -1;
+  }
+  m->G.ppm_split = (unsigned short)split;
+  m->G.seg[0] = (unsigned short)seg_j;
+  m->G.seg[1] = (unsigned short)seg_k;
+  m->geo_gen++;
+  if (m->gexec) { (void)hipGraphExecDestroy(m->gexec); m->gexec = nullptr; }
+  if (m->graph) { (void)hipGraphDestroy(m->graph); m->graph = nullptr; }
+  return 0;
+}
+
+int raishin_model_set_divb_form(raishin_model* m, int form) {
+  if (!m || (form != 0 && form != 1)) {
+    set_error("raishin_model_set_divb_form: form must be 0 (the reference's fluxes) or 1 (zero normal-B flux)");
+    return -1;
+  }
+  m->G.divb0 = (unsigned short)form;
+  m->geo_gen++;
+  if (m->gexec) { (void)hipGraphExecDestroy(m->gexec); m->gexec = nullptr; }
+  if (m->graph) { (void)hipGraphDestroy(m->graph); m->graph = nullptr; }
+  return 0;
+}
+
+int raishin_model_get_sweep_tiles(raishin_model* m, int* split, int* seg_j, int* seg_k) {
+  if (!m) { set_error("raishin_model_get_sweep_tiles: null model"); return -1; }
+  if (split) *split = m->G.ppm_split;
+  if (seg_j) *seg_j = psweep_segment(m->G, 2);
+  if (seg_k) *seg_k = psweep_segment(m->G, 3);
+  return 0;
+}
+
+namespace {
+// Milliseconds per step of each timed kind over `steps` steps from the state
+// (after one warm-up step); false on failure.
+bool time_kinds(raishin_model* m, const double* uri, const double* uu, double time, int steps, double* ms) {
+  raishin_model_set_state(m, uri, uu, time);
+  if (!run_step(m)) return false;
+  raishin_model_set_timing(m, 1);
+  for (int s = 0; s < steps; s++)
+    if (!run_step(m)) return false;
+  long long cnt[K_NKINDS];
+  raishin_model_get_timing(m, ms, cnt, K_NKINDS);
+  raishin_model_set_timing(m, 0);
+  for (int k = 0; k < K_NKINDS; k++) ms[k] /= steps;
+  return true;
+}
+}  // namespace
+
 int raishin_model_autotune(raishin_model* m, const double* uri, const double* uu, double time, int steps) {
   if (!m || !uri || steps < 1) { set_error("raishin_model_autotune: bad arguments"); return -1; }
   // timed kind of each tuned slot
@@ -472,6 +525,21 @@ int raishin_model_autotune(raishin_model* m, const double* uri, const double* uu
     }
   }
   if (raishin_model_set_block_sizes(m, best, TS_N) != 0) return -1;
+  // PPM sweep tiles: the fused sweeps with each tile height along j and k,
+  // the fastest per direction (the split form, slower at every measured
+  // size, is left to explicit choice: members of one batch must agree on it)
+  if (m->p.irec == 5) {
+    const int segs[] = {8, 16, 32};
+    double ms[K_NKINDS];
+    double bj = 1e300, bk = 1e300;
+    int sj = PSWEEP_SEG_DEFAULT, sk = PSWEEP_SEG_DEFAULT;
+    for (int sgm : segs) {
+      if (raishin_model_set_sweep_tiles(m, 0, sgm, sgm) != 0 || !time_kinds(m, uri, uu, time, steps, ms)) return -1;
+      if (ms[K_SWEEP2] < bj) { bj = ms[K_SWEEP2]; sj = sgm; }
+      if (ms[K_SWEEP3] < bk) { bk = ms[K_SWEEP3]; sk = sgm; }
+    }
+    if (raishin_model_set_sweep_tiles(m, 0, sj, sk) != 0) return -1;
+  }
   // start again from the given state, as if no step had been taken
   raishin_model_set_state(m, uri, uu, time);
   raishin_model_reset_counters(m);

@@ -41,6 +41,17 @@ struct Geo {
   // workgroup size per tuned launch slot (TuneSlot in rh_kernels.hpp);
   // 0 means 256.  Chosen by raishin_model_autotune or set by the caller.
   unsigned short bs[8];
+  // PPM sweeps (irec 5): interfaces per lane along j and k of the fused
+  // sweep (0: the default), and ppm_split = 1 for the split form (face
+  // values through HBM in their own kernel).  Chosen by the autotuner or
+  // set by the caller (raishin_model_set_sweep_tiles); results never depend
+  // on them.
+  unsigned short seg[2];
+  unsigned short ppm_split;
+  // 1: the face flux of the normal field component is set to zero (not the
+  // reference's arithmetic: its HLLE keeps a dissipative normal-B flux that
+  // flux-CT leaves alone); flux-CT then keeps div B at round-off
+  unsigned short divb0;
 };
 
 // One metric location (centre or a face) from the (i,k) tables:

@@ -407,6 +407,10 @@ RH_DEV void sweep_face(const Geo& G, R* __restrict__ F, R* __restrict__ CTsrc, l
     } else if (cmax <= R(0)) o = double(Fr[n]);
     else o = Fo[(long long)n * N];  // unordered (NaN) speeds: hlle keeps ww, then flushes it
     if (absr(o) < 1.0e-40) o = 0.0;
+    // the labelled non-reference form: no flux of the normal field component
+    // (the induction flux itself is zero; HLLE's dissipation term is what
+    // the reference keeps, hll.f90:109-111), so flux-CT keeps div B
+    if (n == 5 + D && G.divb0) o = 0.0;
     if (n == nA || n == nB) {
       CTsrc[(long long)(n == nA ? pA : pB) * N + c] = R(o);
       if (!in_ct) Fo[(long long)n * N] = R(o);  // inside the CT box fluxct writes this slot
@@ -476,6 +480,120 @@ RH_DEV void b_sweep(const Geo& G, const SweepArgs& A, long long blk) {
 }
 
 // ------------------------------------------------------------------------
+// PPM direction sweep with its face values computed in the same kernel:
+// each cell's edges (interp, detect, flaten, monoto; rec.f90:1261-2056) are
+// written to LDS and the interface physics (caluu3 -> calflx3 -> calcha ->
+// hlle) reads them there, so the 18 edge planes never reach HBM.
+//  D = 1: a wave takes 64 consecutive cells of an i row; lane l's interface
+//         (the right face of its cell) takes the left edges of lane l+1's
+//         cell from LDS.  Consecutive waves of a row overlap by one cell.
+//  D = 2, 3: a wave holds 64 / A.seg consecutive i by A.seg consecutive
+//         cells along D; lane l's interface takes the left edges of the
+//         cell A.seg lanes further along D.  Waves overlap by one cell along D.
+// The arithmetic is that of k_ppm_edges followed by k_sweep (the same
+// functions), so the fluxes are the same bit for bit.
+// ------------------------------------------------------------------------
+// Edges of the cell at c (index p along D): left edges to lds[n*ld], right
+// edges to lds[(9+n)*ld].
+template <int D, class R>
+RH_DEV void ppm_edges_lds(const double* __restrict__ W, long long N, long long c, long long s, int p,
+                          const double* __restrict__ dxs, const double* __restrict__ xs,
+                          const double* __restrict__ pc, long long npc, R gam, R* lds, int ld) {
+  PpmCoef<R> kc;
+  ppm_coef<R>(pc, npc, p, kc);
+  R f2, eta;
+  ppm_shape<D, R>(W, N, c, s, p, dxs, xs, gam, f2, eta);
+  R cur[5], nxt[5];
+#pragma unroll
+  for (int o = 0; o < 5; o++) cur[o] = R(W[c + (o - 2) * s]);
+#pragma unroll 1
+  for (int n = 0; n < 9; n++) {
+    if (n < 8) {
+      const double* Pn = W + (long long)(n + 1) * N + c;
+#pragma unroll
+      for (int o = 0; o < 5; o++) nxt[o] = R(Pn[(o - 2) * s]);
+    }
+    R l, r;
+    ppm_var<D, R>(cur, kc, n, f2, eta, l, r);
+    lds[n * ld] = l;
+    lds[(9 + n) * ld] = r;
+#pragma unroll
+    for (int o = 0; o < 5; o++) cur[o] = nxt[o];
+  }
+}
+
+template <int D, class R, int ICHA>
+RH_DEV void b_psweep(const Geo& G, const PSweepArgs& A, long long blk) {
+  extern __shared__ unsigned char rh_dyn_lds[];  // 18 planes of blockDim.x values of R
+  R* const L = reinterpret_cast<R*>(rh_dyn_lds);
+  const int bt = blockDim.x, tid = threadIdx.x, lane = tid & 63;
+  const long long N = G.N;
+  const double* __restrict__ W = A.W;
+  R* __restrict__ F = (R*)A.F;
+  R* __restrict__ CTsrc = (R*)A.CTsrc;
+  const R gam = R(A.gam);
+  const int lj = jlo(G, G.nm0), hj = jhi(G, G.nj - G.nm0);
+  const long long wv = (blk * bt + tid) >> 6;
+  if (D == 1) {
+    const long long nic = (G.ni - 2 * G.nm0 + 1 + 62) / 63, cj = hj - lj + 1, ck = G.nk - 2 * G.nm0 + 1;
+    const long long ic = wv % nic, r = wv / nic;
+    const int j = lj + (int)(r % cj);
+    const long long kk = r / cj;
+    const int k = G.nm0 + (int)kk;
+    const int i = G.nm0 + (int)(ic * 63) + lane;
+    const bool cell_in = kk < ck && i <= G.ni - G.nm0 + 1;
+    const long long c = cell_in ? cidx(i, j, k, G) : 0;
+    if (cell_in) ppm_edges_lds<1, R>(W, N, c, 1, i, A.dxs, A.xs, A.pc, G.ni + 2, gam, L + tid, bt);
+    __syncthreads();  // every thread of the block reaches it: no early exit above
+    if (cell_in && lane < 63 && i <= G.ni - G.nm0) {
+      R wl[9], wr[9];
+#pragma unroll
+      for (int n = 0; n < 9; n++) { wl[n] = L[(9 + n) * bt + tid]; wr[n] = L[n * bt + tid + 1]; }
+      const Met M{A.metF, G.P2, q2(i, k, G)};
+      sweep_face<1, R, ICHA>(G, F, CTsrc, c, i, j, k, M, gam, wl, wr);
+    }
+    return;
+  }
+  // D = 2, 3: a wave holds TI consecutive i (fastest) by TP = 64 / TI
+  // consecutive cells along D, so each load instruction reads TP runs of TI
+  // contiguous values; lane l's interface takes the left edges of lane
+  // l + TI's cell.  A wave's TP cells give TP - 1 interfaces (consecutive
+  // waves along D overlap by one cell).  A collapsed model's one phi row is
+  // its own neighbour along j (stride 0): there a wave holds 64 i of that
+  // row, and each interface takes both edges of its own cell.
+  const bool self = D == 2 && G.jfix;
+  const int TP = self ? 1 : A.seg, TI = 64 / TP, IPW = self ? 1 : TP - 1;
+  const int il = lane % TI, pl = lane / TI;
+  const int plo = (D == 2) ? lj : G.nm0, phi = (D == 2) ? hj : G.nk - G.nm0;
+  const int tlo = (D == 2) ? G.nm0 : lj, thi = (D == 2) ? G.nk - G.nm0 : hj;
+  const long long nig = (G.ni - 2 * G.nm0 + 1 + TI - 1) / TI;
+  const long long ndc = (phi - plo + 1 + IPW - 1) / IPW, ct = thi - tlo + 1;
+  // consecutive waves along i (a workgroup's waves share cache lines), then
+  // along D, then across (measured fastest for both directions: along k,
+  // walking j before the chunks keeps the face metric in L2 but loses more
+  // to the stencil rows)
+  const long long ig = wv % nig, r = wv / nig, dc = r % ndc, tt = r / ndc;
+  const int i = G.nm0 + (int)ig * TI + il;
+  const int p = plo + (int)dc * IPW + pl;
+  const int t = tlo + (int)tt;
+  const bool cell_in = tt < ct && dc < ndc && i <= G.ni - G.nm0 && p <= phi + (self ? 0 : 1);
+  const long long s = (D == 2) ? jstride(G) : kstride(G);
+  const long long npc = ((D == 2) ? G.nj : G.nk) + 2;
+  const long long c = cell_in ? ((D == 2) ? cidx(i, p, t, G) : cidx(i, t, p, G)) : 0;
+  if (cell_in) ppm_edges_lds<D, R>(W, N, c, s, p, A.dxs, A.xs, A.pc, npc, gam, L + tid, bt);
+  __syncthreads();  // every thread of the block reaches it: no early exit above
+  if (cell_in && (self || pl < TP - 1) && p <= phi) {
+    const int nb = self ? tid : tid + TI;
+    R wl[9], wr[9];
+#pragma unroll
+    for (int n = 0; n < 9; n++) { wl[n] = L[(9 + n) * bt + tid]; wr[n] = L[n * bt + nb]; }
+    const int j = (D == 2) ? p : t, k = (D == 2) ? t : p;
+    const Met M{A.metF, G.P2, q2(i, k, G)};
+    sweep_face<D, R, ICHA>(G, F, CTsrc, c, i, j, k, M, gam, wl, wr);
+  }
+}
+
+// ------------------------------------------------------------------------
 // Flux-CT (Toth 2000): fluxct, ct.f90:30-118.  Reads the six transverse B
 // fluxes from CTsrc (planes 0 ww(2,7) 1 ww(3,7) 2 ww(1,8) 3 ww(3,8) 4 ww(1,9)
 // 5 ww(2,9), as the sweeps left them) and writes the averaged values into
@@ -501,12 +619,26 @@ RH_DEV void b_fluxct(const Geo& G, const CtArgs& A, long long blk) {
   const FT* fyz = CTsrc + 5 * N;  // ww(2,9)
   const double c8 = 1.0 / 8.0;
   // 2.0 * x leads each sum, so every term is added in double
-  F2[6 * N + c] = FT(c8 * (2.0 * fyx[c] + fyx[c + si] + fyx[c - si] - fxy[c] - fxy[c + sj] - fxy[c - si] - fxy[c - si + sj]));
-  F3[6 * N + c] = FT(c8 * (2.0 * fzx[c] + fzx[c + si] + fzx[c - si] - fxz[c] - fxz[c + sk] - fxz[c - si] - fxz[c - si + sk]));
-  F1[7 * N + c] = FT(c8 * (2.0 * fxy[c] + fxy[c + sj] + fxy[c - sj] - fyx[c] - fyx[c + si] - fyx[c - sj] - fyx[c + si - sj]));
-  F3[7 * N + c] = FT(c8 * (2.0 * fzy[c] + fzy[c + sj] + fzy[c - sj] - fyz[c] - fyz[c + sk] - fyz[c - sj] - fyz[c - sj + sk]));
-  F1[8 * N + c] = FT(c8 * (2.0 * fxz[c] + fxz[c + sk] + fxz[c - sk] - fzx[c] - fzx[c + si] - fzx[c - sk] - fzx[c + si - sk]));
-  F2[8 * N + c] = FT(c8 * (2.0 * fyz[c] + fyz[c + sk] + fyz[c - sk] - fzy[c] - fzy[c + sj] - fzy[c - sk] - fzy[c + sj - sk]));
+  double v[6];
+  v[0] = c8 * (2.0 * fyx[c] + fyx[c + si] + fyx[c - si] - fxy[c] - fxy[c + sj] - fxy[c - si] - fxy[c - si + sj]);
+  v[1] = c8 * (2.0 * fzx[c] + fzx[c + si] + fzx[c - si] - fxz[c] - fxz[c + sk] - fxz[c - si] - fxz[c - si + sk]);
+  v[2] = c8 * (2.0 * fxy[c] + fxy[c + sj] + fxy[c - sj] - fyx[c] - fyx[c + si] - fyx[c - sj] - fyx[c + si - sj]);
+  v[3] = c8 * (2.0 * fzy[c] + fzy[c + sj] + fzy[c - sj] - fyz[c] - fyz[c + sk] - fyz[c - sj] - fyz[c - sj + sk]);
+  v[4] = c8 * (2.0 * fxz[c] + fxz[c + sk] + fxz[c - sk] - fzx[c] - fzx[c + si] - fzx[c - sk] - fzx[c + si - sk]);
+  v[5] = c8 * (2.0 * fyz[c] + fyz[c + sk] + fyz[c - sk] - fzy[c] - fzy[c + sj] - fzy[c - sk] - fzy[c + sj - sk]);
+  F2[6 * N + c] = FT(v[0]);
+  F3[6 * N + c] = FT(v[1]);
+  F1[7 * N + c] = FT(v[2]);
+  F3[7 * N + c] = FT(v[3]);
+  F1[8 * N + c] = FT(v[4]);
+  F2[8 * N + c] = FT(v[5]);
+  // the mixed form keeps the CT update of B in FP64: the update reads these
+  // (the float copies above are what a later sweep keeps where its wave
+  // speeds are unordered, as ww keeps what ct wrote)
+  if (sizeof(FT) == 4 && A.CTd) {
+#pragma unroll
+    for (int q = 0; q < 6; q++) A.CTd[(long long)q * N + c] = v[q];
+  }
 }
 
 // ------------------------------------------------------------------------
@@ -987,11 +1119,19 @@ RH_DEV void b_rk(const Geo& G, const StageArgs& A, long long blk) {
     sf[0] = dg * ta; sf[1] = dg * tb; sf[2] = dg * tc; sf[3] = dg * td;
   }
 
+  // transverse-B fluxes from flux-CT's FP64 planes in the mixed form
+  // (component m, direction d -> plane; -1: the sweep's own flux)
+  constexpr int ctp[3][3] = {{-1, 0, 1}, {2, -1, 3}, {4, 5, -1}};
+  const bool ctd = sizeof(FT) == 4 && A.CTd;
 #pragma unroll
   for (int m = 0; m < 9; m++) {
     const long long o = (long long)m * N + c;
-    const double d1 = double(F1[o]) - double(F1[o - 1]), d2 = double(F2[o]) - double(F2[o - sj]),
-                 d3 = double(F3[o]) - double(F3[o - sk]);
+    auto fl = [&](const FT* Fd, int d, long long off) -> double {
+      if (m >= 6 && ctd && ctp[m - 6][d] >= 0) return A.CTd[(long long)ctp[m - 6][d] * N + c - off];
+      return double(Fd[o - off]);
+    };
+    const double d1 = fl(F1, 0, 0) - fl(F1, 0, 1), d2 = fl(F2, 1, 0) - fl(F2, 1, sj),
+                 d3 = fl(F3, 2, 0) - fl(F3, 2, sk);
     double v;
     if (STAGE == 1) {
       v = A.Ua[o] - a1 * d1 - a2 * d2 - a3 * d3;
@@ -1735,6 +1875,9 @@ constexpr int sweep_waves = (IREC == 5 || IREC == 1) ? (sizeof(R) == 4 ? 5 : 3) 
 template <int D, int IREC, class R, int ICHA>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(sweep_waves<R, IREC>)))
 k_sweep(Geo G, SweepArgs A) { b_sweep<D, IREC, R, ICHA>(G, A, xcd_block()); }
+template <int D, class R, int ICHA>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(sweep_waves<R, 5>)))
+k_psweep(Geo G, PSweepArgs A) { b_psweep<D, R, ICHA>(G, A, xcd_block()); }
 template <class FT>
 __global__ void __launch_bounds__(256) k_fluxct(Geo G, CtArgs A) { b_fluxct<FT>(G, A, xcd_block()); }
 template <int STAGE, bool SYM, class FT>
@@ -1773,6 +1916,13 @@ kb_sweep(const BatchItem* __restrict__ it, int n) {
   RH_BATCH_PROLOGUE
   const SweepArgs A = it[q].A.s;
   b_sweep<D, IREC, R, ICHA>(G, A, blk);
+}
+template <int D, class R, int ICHA>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(sweep_waves<R, 5>)))
+kb_psweep(const BatchItem* __restrict__ it, int n) {
+  RH_BATCH_PROLOGUE
+  const PSweepArgs A = it[q].A.ps;
+  b_psweep<D, R, ICHA>(G, A, blk);
 }
 template <class FT>
 __global__ void __launch_bounds__(256) kb_fluxct(const BatchItem* __restrict__ it, int n) {
@@ -1947,9 +2097,78 @@ hipError_t launch_sweep(const Geo& G, int dir, int irec, const double* W, const 
   return hipGetLastError();
 }
 
-hipError_t launch_fluxct(const Geo& G, const void* CTsrc, void* F1, void* F2, void* F3, bool f32, hipStream_t st) {
+// Fused PPM sweeps: variant = dir + 4*mixed + 64*icha (as the sweeps').
+static int psweep_variant(int dir, bool mixed, int icha) { return dir + 4 * (int)mixed + 64 * icha; }
+
+template <class F>
+static void with_psweep(int variant, F&& f) {
+  const int d = variant & 3;
+  const bool mx = (variant & 4) != 0;
+  with_icha(variant >> 6, [&](auto C) {
+    auto t = [&](auto D) {
+      if (mx) f(D, TypeTag<float>{}, C);
+      else f(D, TypeTag<double>{}, C);
+    };
+    if (d == 1) t(IC<1>{});
+    else if (d == 2) t(IC<2>{});
+    else t(IC<3>{});
+  });
+}
+
+// Waves of one fused PPM sweep (per model): D = 1, 64-cell chunks of the i
+// rows (63 interfaces each) over (j, k); D = 2, 3, tiles of 64 / seg i by
+// seg cells along D (seg - 1 interfaces) over the transverse axis.
+static long long psweep_waves(const Geo& G, int D, int seg) {
+  const long long nj = G.jfix ? 1 : (long long)(G.nj - 2 * G.nm0 + 1);
+  const long long ni = G.ni - 2 * G.nm0 + 1, nk = G.nk - 2 * G.nm0 + 1;
+  if (ni <= 0 || nj <= 0 || nk <= 0) return 0;
+  if (D == 1) return ((ni + 62) / 63) * nj * nk;
+  if (D == 2 && G.jfix) return ((ni + 63) / 64) * nk;  // one stored row: 64 i per wave
+  const long long along = D == 2 ? nj : nk, across = D == 2 ? nk : nj, ti = 64 / seg;
+  return ((along + seg - 2) / (seg - 1)) * ((ni + ti - 1) / ti) * across;
+}
+
+// The tile height in use along dir: the chosen one (or the default), but no
+// taller than the axis needs (the smallest power of two above its interface
+// count), so short axes (Riemann tubes' transverse axes) waste no lanes.
+int psweep_segment(const Geo& G, int dir) {
+  const int want = (dir == 2 ? G.seg[0] : dir == 3 ? G.seg[1] : 0);
+  int s = want > 0 ? want : PSWEEP_SEG_DEFAULT;
+  const int n = dir == 2 ? (G.jfix ? 1 : G.nj - 2 * G.nm0 + 1) : dir == 3 ? G.nk - 2 * G.nm0 + 1 : 0;
+  int cap = 2;
+  while (cap < 64 && cap - 1 < n) cap *= 2;
+  return s < cap ? s : cap;
+}
+
+size_t psweep_lds_bytes(int threads, bool mixed) { return (size_t)18 * threads * (mixed ? sizeof(float) : sizeof(double)); }
+
+hipError_t launch_psweep(const Geo& G, int dir, const double* W, const double* metF, const double* dxs,
+                         const double* xs, const double* pc, double gam, void* F, void* CTsrc, bool mixed,
+                         hipStream_t st) {
+  const int seg = psweep_segment(G, dir);
+  const long long waves = psweep_waves(G, dir, seg);
+  PSweepArgs A{W, metF, dxs, xs, pc, gam, F, CTsrc, seg};
+  const int bt = slot_threads(G, TS_SWEEP1 + dir - 1);
+  const unsigned nb = nblk(waves * 64, bt);
+  const int variant = psweep_variant(dir, mixed, G.icha);
+  if (g_rec) {
+    AnyArgs a{};
+    a.ps = A;
+    record(LK_PSWEEP, variant, bt, nb, G, a);
+    return hipSuccess;
+  }
+  if (waves == 0) return hipSuccess;
+  const size_t lds = psweep_lds_bytes(bt, mixed);
+  with_psweep(variant, [&](auto D, auto T, auto C) {
+    hipLaunchKernelGGL((k_psweep<D.value, typename decltype(T)::type, C.value>), dim3(nb), dim3(bt), lds, st, G, A);
+  });
+  return hipGetLastError();
+}
+
+hipError_t launch_fluxct(const Geo& G, const void* CTsrc, void* F1, void* F2, void* F3, bool f32, double* CTd,
+                         hipStream_t st) {
   const long long n = box(G.nm1, G.ni - G.nm1) * jbox(G, G.nm1, G.nj - G.nm1) * box(G.nm1, G.nk - G.nm1);
-  CtArgs A{CTsrc, F1, F2, F3};
+  CtArgs A{CTsrc, F1, F2, F3, f32 ? CTd : nullptr};
   const int bt = slot_threads(G, TS_FLUXCT);
   if (g_rec) { AnyArgs a{}; a.c = A; record(LK_FLUXCT, (int)f32, bt, nblk(n, bt), G, a); return hipSuccess; }
   if (n == 0) return hipSuccess;
@@ -2082,6 +2301,13 @@ hipError_t launch_batched(int kind, int variant, int threads, const BatchItem* i
         hipLaunchKernelGGL((kb_sweep<D.value, I.value, typename decltype(T)::type, C.value>), g, b, 0, st, it, n);
       });
       break;
+    case LK_PSWEEP: {
+      const size_t lds = psweep_lds_bytes(threads, (variant & 4) != 0);
+      with_psweep(variant, [&](auto D, auto T, auto C) {
+        hipLaunchKernelGGL((kb_psweep<D.value, typename decltype(T)::type, C.value>), g, b, lds, st, it, n);
+      });
+      break;
+    }
     case LK_FLUXCT:
       if (variant == 1) hipLaunchKernelGGL(kb_fluxct<float>, g, b, 0, st, it, n);
       else hipLaunchKernelGGL(kb_fluxct<double>, g, b, 0, st, it, n);

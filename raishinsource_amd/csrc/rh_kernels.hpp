@@ -27,6 +27,10 @@ struct StageArgs {
   int iter;
   bool chr_sym;                    // Christoffel tables symmetric in the lower indices
   bool f32flux;                    // F1..F3 hold float (the mixed form's sweeps)
+  // the mixed form: flux-CT's six averaged transverse-B fluxes in FP64
+  // (planes ww(2,7) ww(3,7) ww(1,8) ww(3,8) ww(1,9) ww(2,9)), which the
+  // update reads instead of their float copies in F1..F3; null in FP64
+  const double *CTd;
 };
 
 struct PostArgs {
@@ -72,9 +76,17 @@ struct SweepArgs {
   void *F, *CTsrc;  // element type: the sweep's arithmetic type (double, or float when mixed)
 };
 
+struct PSweepArgs {
+  const double *W, *metF, *dxs, *xs, *pc;  // pc: PPM width coefficients of the direction
+  double gam;
+  void *F, *CTsrc;  // element type: the sweep's arithmetic type
+  int seg;          // interfaces per lane along j or k
+};
+
 struct CtArgs {
   const void *CTsrc;  // double, or float when mixed (as the sweeps wrote them)
   void *F1, *F2, *F3;
+  double *CTd;        // mixed form: the six averages also in FP64 (StageArgs::CTd)
 };
 
 struct AdvArgs {
@@ -100,12 +112,13 @@ inline int slot_threads(const Geo& G, int slot) { return G.bs[slot] ? (int)G.bs[
 // Launch kinds of one step, in the order run_step makes them.
 enum LaunchKind {
   LK_EDGES, LK_SWEEP, LK_FLUXCT, LK_RK, LK_RECOV, LK_CARRY, LK_BND, LK_POST, LK_CFL, LK_CFL_FINISH,
-  LK_ADVANCE, LK_COPY, LK_N
+  LK_ADVANCE, LK_COPY, LK_PSWEEP, LK_N
 };
 
 union AnyArgs {
   EdgeArgs e;
   SweepArgs s;
+  PSweepArgs ps;
   CtArgs c;
   StageArgs st;
   FixArgs fx;
@@ -157,8 +170,21 @@ hipError_t launch_edges(const Geo& G, int dir, int irec, const double* W, const 
 // arithmetic produced, so storing it in float loses nothing).
 hipError_t launch_sweep(const Geo& G, int dir, int irec, const double* W, const double* metF, const void* E,
                         double gam, void* F, void* CTsrc, bool mixed, hipStream_t st);
+// PPM (irec 5) sweep along dir with the face values computed in the same
+// kernel (through LDS): reads W and the direction's PPM width coefficients
+// pc, writes what launch_edges + launch_sweep write to F and CTsrc.
+// seg: cells along j or k per wave of the fused sweep (64 / seg consecutive
+// i per wave), a power of two from 2 to 64
+constexpr int PSWEEP_SEG_DEFAULT = 16;
+inline bool psweep_seg_ok(int seg) { return seg >= 2 && seg <= 64 && (seg & (seg - 1)) == 0; }
+hipError_t launch_psweep(const Geo& G, int dir, const double* W, const double* metF, const double* dxs,
+                         const double* xs, const double* pc, double gam, void* F, void* CTsrc, bool mixed,
+                         hipStream_t st);
+int psweep_segment(const Geo& G, int dir);
+size_t psweep_lds_bytes(int threads, bool mixed);
 // f32: CTsrc and F hold floats (the averages are formed in double either way)
-hipError_t launch_fluxct(const Geo& G, const void* CTsrc, void* F1, void* F2, void* F3, bool f32, hipStream_t st);
+hipError_t launch_fluxct(const Geo& G, const void* CTsrc, void* F1, void* F2, void* F3, bool f32, double* CTd,
+                         hipStream_t st);
 // n values of type double (to_float) or float (else) from src into dst, the other type
 hipError_t launch_retype(const void* src, void* dst, long long n, bool to_float, hipStream_t st);
 // RK update + sources into A.Ukeep (every interior cell), then recov1a into A.Wpre.

@@ -464,6 +464,12 @@ bool stage_sweeps(raishin_model* m, const double* W, double* F) {
   const Geo& G = m->G;
   const int irec = m->p.irec;
   for (int d = 1; d <= 3; d++) {
+    if (irec == 5 && !G.ppm_split) {
+      TimedLaunch t(m, K_SWEEP1 + d - 1);
+      HIPCHK(launch_psweep(G, d, W, m->met[d], m->dxs[d - 1], m->xs[d - 1], m->ppmc[d - 1], m->p.gam,
+                           flux_plane(m, F, (d - 1) * 9), m->CT, m->mixed, m->st));
+      continue;
+    }
     if (irec == 5 || irec == 1) {
       TimedLaunch t(m, K_EDGES);
       const double* tab = irec == 5 ? m->ppmc[d - 1] : m->xa[d - 1];
@@ -476,9 +482,14 @@ bool stage_sweeps(raishin_model* m, const double* W, double* F) {
   return true;
 }
 
+// The mixed form's FP64 copies of flux-CT's six averages: the flux buffer's
+// second half, which the float planes leave unused.
+static double* ct_fp64(const raishin_model* m, double* F) { return m->mixed ? F + 14 * m->G.N : nullptr; }
+
 bool stage_ct(raishin_model* m, double* F) {
   TimedLaunch t(m, K_FLUXCT);
-  HIPCHK(launch_fluxct(m->G, m->CT, flux_plane(m, F, 0), flux_plane(m, F, 9), flux_plane(m, F, 18), m->mixed, m->st));
+  HIPCHK(launch_fluxct(m->G, m->CT, flux_plane(m, F, 0), flux_plane(m, F, 9), flux_plane(m, F, 18), m->mixed,
+                       ct_fp64(m, F), m->st));
   return true;
 }
 
@@ -493,6 +504,7 @@ bool stage_update(raishin_model* m, int rkstage, const double* W, const double* 
   A.F2 = flux_plane(m, const_cast<double*>(F), 9);
   A.F3 = flux_plane(m, const_cast<double*>(F), 18);
   A.f32flux = m->mixed;
+  A.CTd = ct_fp64(m, const_cast<double*>(F));
   A.metC = m->met[0]; A.chr = m->chr; A.dx1b = m->dxb[0]; A.dx2b = m->dxb[1]; A.dx3b = m->dxb[2];
   A.r15 = m->r15; A.r25 = m->r25; A.dtbits = m->dtbits; A.Wpre = m->Wpre; A.wgout = m->wgout; A.Ukeep = Uout;
   A.cnt = m->cnt; A.cpart = m->cpart; A.carry = m->carry; A.cflag = m->cflag; A.carry_cap = m->carry_cap;

@@ -64,10 +64,12 @@ EXPORTS = (
     "raishin_store1", "raishin_restar2", "raishin_output1", "raishin_model_store1", "raishin_model_restar2",
     "raishin_model_output1", "raishin_model_get_pram", "raishin_model_create_ik", "raishin_mdtorus1_ik",
     "raishin_model_create_grid", "raishin_model_tables", "raishin_model_set_timing_mask",
-    "raishin_model_cdtcfl4_noalloc",
+    "raishin_model_cdtcfl4_noalloc", "raishin_model_set_sweep_tiles", "raishin_model_get_sweep_tiles",
+    "raishin_model_set_divb_form",
 )
 
 TUNE_SLOTS = ("ppm_edges", "sweep_i", "sweep_j", "sweep_k", "fluxct", "update", "recov", "post")
+TILE_KEYS = ("ppm_split", "seg_j", "seg_k")
 
 # reference-named entry points (include/raishin_ref.h)
 REF_EXPORTS = (
@@ -133,6 +135,12 @@ def lib():
         L.raishin_model_debug_fluxes.argtypes = [vp, vp, vp]
         L.raishin_model_cdtcfl4_noalloc.argtypes = [vp, vp]
         L.raishin_model_cdtcfl4_noalloc.restype = C.c_int
+        L.raishin_model_set_sweep_tiles.argtypes = [vp, C.c_int, C.c_int, C.c_int]
+        L.raishin_model_set_sweep_tiles.restype = C.c_int
+        L.raishin_model_get_sweep_tiles.argtypes = [vp, vp, vp, vp]
+        L.raishin_model_get_sweep_tiles.restype = C.c_int
+        L.raishin_model_set_divb_form.argtypes = [vp, C.c_int]
+        L.raishin_model_set_divb_form.restype = C.c_int
         L.raishin_model_set_precision.argtypes = [vp, C.c_int]
         L.raishin_model_step_graph.argtypes = [vp, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_double)]
         L.raishin_models_step_graph.argtypes = [C.POINTER(vp), C.c_int, C.c_int]
@@ -544,22 +552,44 @@ class Model:
             check()
         return dict(zip(TUNE_SLOTS, list(a)))
 
+    def sweep_tiles(self):
+        """{"ppm_split": 0 or 1, "seg_j": n, "seg_k": n}: the PPM sweep tiles in use."""
+        v = [C.c_int(0) for _ in range(3)]
+        if lib().raishin_model_get_sweep_tiles(self.h, *[C.byref(x) for x in v]) != 0:
+            check()
+        return dict(zip(TILE_KEYS, (x.value for x in v)))
+
+    def set_sweep_tiles(self, split=0, seg_j=0, seg_k=0):
+        if lib().raishin_model_set_sweep_tiles(self.h, int(split), int(seg_j), int(seg_k)) != 0:
+            check()
+
+    def set_divb_form(self, zero_normal_flux):
+        """False: the reference's HLLE flux of the normal field component;
+        True: that flux set to zero, so flux-CT keeps div B at round-off
+        (not the reference's arithmetic)."""
+        if lib().raishin_model_set_divb_form(self.h, 1 if zero_normal_flux else 0) != 0:
+            check()
+
     def set_block_sizes(self, sizes):
-        """sizes: dict by slot name or sequence of 8 (0 = 256)."""
+        """sizes: dict by slot name (and, optionally, the sweep tiles of
+        sweep_tiles()) or a sequence of 8 (0 = 256)."""
         if isinstance(sizes, dict):
+            if all(k in sizes for k in TILE_KEYS):
+                self.set_sweep_tiles(*(sizes[k] for k in TILE_KEYS))
             sizes = [sizes.get(k, 0) for k in TUNE_SLOTS]
         a = (C.c_int * len(TUNE_SLOTS))(*sizes)
         if lib().raishin_model_set_block_sizes(self.h, a, len(TUNE_SLOTS)) != 0:
             check()
 
     def autotune(self, uri, uu=None, time=0.0, steps=2):
-        """Pick the fastest workgroup size per launch slot on this model
-        (then the state is uri again and the counters are clear)."""
+        """Pick the fastest workgroup size per launch slot and, for PPM, the
+        sweep tiles on this model (then the state is uri again and the
+        counters are clear).  Returns the block sizes and the tiles."""
         if lib().raishin_model_autotune(self.h, _p(np.asfortranarray(uri)),
                                         _p(None if uu is None else np.asfortranarray(uu)), time, steps) != 0:
             check()
         check()
-        return self.block_sizes()
+        return dict(self.block_sizes(), **self.sweep_tiles())
 
     def allow_collapse(self, allow=True):
         """Whether set_state may store one phi row for a state whose rows are
