@@ -2,8 +2,10 @@
 
 Kernels and the C-ABI are compiled by hipcc, the host set-up by g++.  Both
 keep IEEE FP64 semantics that the parity with the reference relies on:
--ffp-contract=off (no FMA contraction), no fast-math.  Packed FP32 ops are
-disabled for all device code.
+-ffp-contract=off (no FMA contraction), no fast-math, IEEE FP64 divides and
+square roots.  Packed FP32 ops are disabled for all device code; FP32
+divides and square roots (the mixed form only) use the hardware's
+approximations.
 """
 import concurrent.futures
 import os
@@ -24,7 +26,11 @@ HIP_FLAGS = ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}", "-ffp-contr
              # offset out of the kernels' loops (the PPM sweep's walk along j
              # and k, the Newton iterations): held across the loop they
              # spill registers
-             "-mllvm", "-disable-machine-licm"]
+             "-mllvm", "-disable-machine-licm",
+             # float divides and square roots (the mixed form's sweeps only:
+             # FP64 ones stay IEEE, and the FP64 kernels compile to the same
+             # code) by the hardware reciprocal / square root, a few ulp
+             "-fno-hip-fp32-correctly-rounded-divide-sqrt"]
 HOST_FLAGS = ["-O2", "-std=c++17", "-fPIC", "-ffp-contract=off", "-fno-fast-math", "-fopenmp"]
 
 FC = os.environ.get("FC_FLANG", "/opt/rocm/lib/llvm/bin/flang")

@@ -24,6 +24,13 @@ RH_DEV double powr(double x, double y) { return pow(x, y); }
 RH_DEV float powr(float x, float y) { return powf(x, y); }
 RH_DEV double fsign1(double x) { return copysign(1.0, x); }
 RH_DEV float fsign1(float x) { return copysignf(1.0f, x); }
+// x*y + z: in double a product and a sum, each rounded, which is the
+// reference's arithmetic with contraction off; in float (the mixed form,
+// which has no reference bits to keep) one fused multiply-add.  Written so
+// that the double form keeps the operand order of the expression it
+// replaces (a sum's two operands commute exactly).
+RH_DEV double mad_(double x, double y, double z) { return x * y + z; }
+RH_DEV float mad_(float x, float y, float z) { return __builtin_fmaf(x, y, z); }
 
 // Geometry of one model on the device.
 struct Geo {
@@ -76,7 +83,7 @@ RH_DEV R calgfl(const R u1, const R u2, const R u3, const Met& M) {
 #pragma unroll
   for (int m = 1; m <= 3; m++)
 #pragma unroll
-    for (int n = 1; n <= 3; n++) utsq = utsq + R(M.g(m, n)) * ut[m] * ut[n];
+    for (int n = 1; n <= 3; n++) utsq = mad_(R(M.g(m, n)) * ut[m], ut[n], utsq);
   if (utsq < R(0)) {
     if (absr(utsq) <= R(1.0e-10)) utsq = absr(utsq);
     else utsq = R(1.0e-10);
@@ -93,9 +100,9 @@ RH_DEV void calucon(const R ut[4], R ucon[4], R gfl, const Met& M) {
   const R b2 = alpha1 * alpha1 * R(M.gi(0, 2));
   const R b3 = alpha1 * alpha1 * R(M.gi(0, 3));
   ucon[0] = gfl * (R(1) / alpha1);
-  ucon[1] = ut[1] - gfl * b1 * (R(1) / alpha1);
-  ucon[2] = ut[2] - gfl * b2 * (R(1) / alpha1);
-  ucon[3] = ut[3] - gfl * b3 * (R(1) / alpha1);
+  ucon[1] = mad_(-(gfl * b1), R(1) / alpha1, ut[1]);
+  ucon[2] = mad_(-(gfl * b2), R(1) / alpha1, ut[2]);
+  ucon[3] = mad_(-(gfl * b3), R(1) / alpha1, ut[3]);
 }
 
 // lower: physfunc.f90:158-192
@@ -103,23 +110,23 @@ template <class R>
 RH_DEV void lower(const R v[4], R w[4], const Met& M) {
 #pragma unroll
   for (int m = 0; m < 4; m++)
-    w[m] = R(M.g(m, 0)) * v[0] + R(M.g(m, 1)) * v[1] + R(M.g(m, 2)) * v[2] + R(M.g(m, 3)) * v[3];
+    w[m] = mad_(R(M.g(m, 3)), v[3], mad_(R(M.g(m, 2)), v[2], mad_(R(M.g(m, 1)), v[1], R(M.g(m, 0)) * v[0])));
 }
 // upper: physfunc.f90:195-230
 template <class R>
 RH_DEV void upper(const R v[4], R w[4], const Met& M) {
 #pragma unroll
   for (int m = 0; m < 4; m++)
-    w[m] = R(M.gi(m, 0)) * v[0] + R(M.gi(m, 1)) * v[1] + R(M.gi(m, 2)) * v[2] + R(M.gi(m, 3)) * v[3];
+    w[m] = mad_(R(M.gi(m, 3)), v[3], mad_(R(M.gi(m, 2)), v[2], mad_(R(M.gi(m, 1)), v[1], R(M.gi(m, 0)) * v[0])));
 }
 
 // calbbcon: physfunc.f90:77-95
 template <class R>
 RH_DEV void calbbcon(const R bcon[4], R bb[4], const R ucov[4], const R ucon[4]) {
-  bb[0] = bcon[1] * ucov[1] + bcon[2] * ucov[2] + bcon[3] * ucov[3];
-  bb[1] = (bcon[1] + bb[0] * ucon[1]) * (R(1) / ucon[0]);
-  bb[2] = (bcon[2] + bb[0] * ucon[2]) * (R(1) / ucon[0]);
-  bb[3] = (bcon[3] + bb[0] * ucon[3]) * (R(1) / ucon[0]);
+  bb[0] = mad_(bcon[3], ucov[3], mad_(bcon[2], ucov[2], bcon[1] * ucov[1]));
+  bb[1] = mad_(bb[0], ucon[1], bcon[1]) * (R(1) / ucon[0]);
+  bb[2] = mad_(bb[0], ucon[2], bcon[2]) * (R(1) / ucon[0]);
+  bb[3] = mad_(bb[0], ucon[3], bcon[3]) * (R(1) / ucon[0]);
 }
 
 // cal3met: coord.f90:970-1013; g3/gi3 hold gcov3a/gcon3a(m,n) at (m-1)+3(n-1)
@@ -140,8 +147,8 @@ RH_DEV void cal3met(const Met& M, R g3[9], R gi3[9]) {
   for (int m = 1; m <= 3; m++)
 #pragma unroll
     for (int n = 1; n <= 3; n++) {
-      g3[(m - 1) + 3 * (n - 1)] = R(M.g(m, n)) + (oncov[m] * oncov[n]);
-      gi3[(m - 1) + 3 * (n - 1)] = R(M.gi(m, n)) + (oncon[m] * oncon[n]);
+      g3[(m - 1) + 3 * (n - 1)] = mad_(oncov[m], oncov[n], R(M.g(m, n)));
+      gi3[(m - 1) + 3 * (n - 1)] = mad_(oncon[m], oncon[n], R(M.gi(m, n)));
     }
 }
 
@@ -160,13 +167,13 @@ RH_DEV void prelude(const R w[9], R gam, const Met& M, PrelT<R>& s) {
   s.ut[0] = R(0); s.ut[1] = w[1]; s.ut[2] = w[2]; s.ut[3] = w[3];
   s.pr = w[4];
   s.bcon[0] = R(0); s.bcon[1] = w[6]; s.bcon[2] = w[7]; s.bcon[3] = w[8];
-  s.roh = s.de + (gam / (gam - R(1))) * s.pr;   // caluu.f90:52 (ieos=0)
+  s.roh = mad_(gam / (gam - R(1)), s.pr, s.de);   // caluu.f90:52 (ieos=0)
   s.gfl = calgfl<R>(w[1], w[2], w[3], M);
   calucon<R>(s.ut, s.ucon, s.gfl, M);
   lower<R>(s.ucon, s.ucov, M);
   calbbcon<R>(s.bcon, s.bb, s.ucov, s.ucon);
   lower<R>(s.bb, s.bbcov, M);
-  s.bbsq = s.bb[0] * s.bbcov[0] + s.bb[1] * s.bbcov[1] + s.bb[2] * s.bbcov[2] + s.bb[3] * s.bbcov[3];
+  s.bbsq = mad_(s.bb[3], s.bbcov[3], mad_(s.bb[2], s.bbcov[2], mad_(s.bb[1], s.bbcov[1], s.bb[0] * s.bbcov[0])));
 }
 
 // Row d of sqrtr(-g) * (rho u^d, T^d_i, T^d_t + rho u^d, rho s u^d, induction):
@@ -176,13 +183,13 @@ template <class R>
 RH_DEV void tmunu_row(const PrelT<R>& s, int d, R detg, R powdg, R out[9]) {
   R delta[4] = {R(0), R(0), R(0), R(0)};
   delta[d] = R(1);
-  const R rb = s.roh + s.bbsq, pb = s.pr + R(0.5) * s.bbsq;
+  const R rb = s.roh + s.bbsq, pb = mad_(R(0.5), s.bbsq, s.pr);
   const R ud = s.ucon[d];
   out[0] = s.de * ud;
-  out[1] = rb * ud * s.ucov[1] + pb * delta[1] - s.bb[d] * s.bbcov[1];
-  out[2] = rb * ud * s.ucov[2] + pb * delta[2] - s.bb[d] * s.bbcov[2];
-  out[3] = rb * ud * s.ucov[3] + pb * delta[3] - s.bb[d] * s.bbcov[3];
-  out[4] = rb * ud * s.ucov[0] + pb * delta[0] - s.bb[d] * s.bbcov[0] + s.de * ud;
+#pragma unroll
+  for (int q = 1; q <= 3; q++)
+    out[q] = mad_(-s.bb[d], s.bbcov[q], mad_(rb * ud, s.ucov[q], pb * delta[q]));
+  out[4] = mad_(s.de, ud, mad_(-s.bb[d], s.bbcov[0], mad_(rb * ud, s.ucov[0], pb * delta[0])));
   out[5] = s.de * (s.pr / powdg) * ud;
   if (d == 0) {
     out[6] = s.bcon[1]; out[7] = s.bcon[2]; out[8] = s.bcon[3];
@@ -190,7 +197,7 @@ RH_DEV void tmunu_row(const PrelT<R>& s, int d, R detg, R powdg, R out[9]) {
     const R r0 = R(1) / s.ucon[0];
 #pragma unroll
     for (int c = 1; c <= 3; c++)
-      out[5 + c] = (c == d) ? R(0) : (ud * r0) * s.bcon[c] - (s.ucon[c] * r0) * s.bcon[d];
+      out[5 + c] = (c == d) ? R(0) : mad_(-(s.ucon[c] * r0), s.bcon[d], (ud * r0) * s.bcon[c]);
   }
 #pragma unroll
   for (int n = 0; n < 9; n++) out[n] = detg * out[n];
@@ -213,11 +220,11 @@ RH_DEV void speed_pair(R vd, R vsq, R omsq, R cssq, R vb, R rohb, R gfl, R gii, 
   R f2, f3;
   if constexpr (ICHA == 1) {
     const R r1 = cssq * (vb * vb) * (R(1) / (rohb * (gfl * gfl)));
-    f2 = R(1) - vsq * omsq - r1;
-    f3 = ((vsq - R(1)) * omsq + r1) * ((gii * vsq - vd * vd) * omsq + vd * vd - gii + r1);
+    f2 = mad_(-vsq, omsq, R(1)) - r1;
+    f3 = mad_(vsq - R(1), omsq, r1) * (mad_(mad_(-vd, vd, gii * vsq), omsq, vd * vd) - gii + r1);
   } else {
-    f2 = R(1) - vsq * omsq;
-    f3 = omsq * (R(1) - vsq) * ((R(1) - vsq * omsq) * gii - (R(1) - omsq) * vd * vd);
+    f2 = mad_(-vsq, omsq, R(1));
+    f3 = omsq * (R(1) - vsq) * mad_(-((R(1) - omsq) * vd), vd, mad_(-vsq, omsq, R(1)) * gii);
   }
   p = (f1 + sqrtr(f3)) * (R(1) / f2);
   m = (f1 - sqrtr(f3)) * (R(1) / f2);
@@ -228,8 +235,9 @@ template <class R>
 RH_DEV R v_dot_b(const R* g3, const R vcon[4], const R bcon[4]) {
   R vcov[4];
 #pragma unroll
-  for (int m = 1; m <= 3; m++) vcov[m] = g3[(m - 1) + 0] * vcon[1] + g3[(m - 1) + 3] * vcon[2] + g3[(m - 1) + 6] * vcon[3];
-  return vcov[1] * bcon[1] + vcov[2] * bcon[2] + vcov[3] * bcon[3];
+  for (int m = 1; m <= 3; m++)
+    vcov[m] = mad_(g3[(m - 1) + 6], vcon[3], mad_(g3[(m - 1) + 3], vcon[2], g3[(m - 1) + 0] * vcon[1]));
+  return mad_(vcov[3], bcon[3], mad_(vcov[2], bcon[2], vcov[1] * bcon[1]));
 }
 
 // Fast magnetosonic speeds along direction d for a face state:
@@ -249,18 +257,18 @@ RH_DEV void face_speed(const PrelT<R>& s, int d, R gam, const Met& M, R& vfp, R&
 #pragma unroll
   for (int m = 1; m <= 3; m++)
 #pragma unroll
-    for (int n = 1; n <= 3; n++) vsq = vsq + g3[(m - 1) + 3 * (n - 1)] * vcon[m] * vcon[n];
+    for (int n = 1; n <= 3; n++) vsq = mad_(g3[(m - 1) + 3 * (n - 1)] * vcon[m], vcon[n], vsq);
   R cssq, vasq;
   if (d == 3) cssq = gam * s.pr * (R(1) / s.roh);
   else cssq = gam * s.pr / s.roh;
   if (d == 2) vasq = s.bbsq / (s.roh + s.bbsq);
   else vasq = s.bbsq * (R(1) / (s.roh + s.bbsq));
-  const R omsq = vasq + cssq * (R(1) - vasq);
+  const R omsq = mad_(cssq, R(1) - vasq, vasq);
   const R vb = ICHA == 1 ? v_dot_b<R>(g3, vcon, s.bcon) : R(0);
   R p, m;
   speed_pair<R, ICHA>(vcon[d], vsq, omsq, cssq, vb, s.roh + s.bbsq, s.gfl, gi3[(d - 1) * 4], p, m);
-  p = alpha1 * p - beta_d;
-  m = alpha1 * m - beta_d;
+  p = mad_(alpha1, p, -beta_d);
+  m = mad_(alpha1, m, -beta_d);
   if (absr(p) > R(1)) p = R(1);
   if (absr(m) > R(1)) m = R(-1);
   vfp = p; vfm = m;

@@ -182,14 +182,14 @@ RH_DEV void ppm_shape(const double* __restrict__ W, long long N, long long c, lo
     const R ddm = (R(1) / (dm2 + dm1 + d0)) * ((u0 - um1) / (d0 + dm1) - (um1 - um2) / (dm1 + dm2));
     const R ddp = (R(1) / (d0 + dp1 + dp2)) * ((up2 - up1) / (dp2 + dp1) - (up1 - u0) / (dp1 + d0));
     const R t1 = ddp * ddm;
-    const R t2 = absr(up1 - um1) - R(0.1) * fminf_(absr(up1), absr(um1));
+    const R t2 = mad_(-R(0.1), fminf_(absr(up1), absr(um1)), absr(up1 - um1));
     R etatil;
     if (t1 >= R(0) && t2 <= R(0)) etatil = R(0);
     else {
       const R xm = R(xs[p - 2]), x0 = R(xs[p - 1]), xp = R(xs[p]);
       const R t2a = -(ddp - ddm) / (xp - xm);
       const R dl = x0 - xm, dr = xp - x0;
-      const R t2b = ((dl * dl * dl) + (dr * dr * dr)) / (up1 - um1);
+      const R t2b = mad_(dr * dr, dr, dl * dl * dl) / (up1 - um1);
       etatil = t2a * t2b;
     }
     eta = fmaxf_(R(0), fminf_(R(5.0) * (etatil - R(0.05)), R(1)));
@@ -236,7 +236,7 @@ RH_DEV void ppm_var(const R u[5], const PpmCoef<R>& k, int n, R f2, R eta, R& l_
     const R sp1 = (up - u0);
     const R sm1 = (u0 - um);
     const R spm1 = sp1 * sm1;
-    const R da = t1a * (t1b * sp1 + t1c * sm1);
+    const R da = t1a * mad_(t1b, sp1, t1c * sm1);
     if (spm1 <= R(0)) return R(0);
     const R t1 = (da < R(0)) ? R(-1) : R(1);
     const R t1d = fminf_(absr(sm1), absr(sp1));
@@ -244,9 +244,9 @@ RH_DEV void ppm_var(const R u[5], const PpmCoef<R>& k, int n, R f2, R eta, R& l_
     return t1 * fminf_(absr(da), R(2.0) * t1d);
   };
   auto eq51 = [&](R uq, R uq1, R dmq, R dmq1, int o) -> R {
-    const R t2 = k.e2[o] * (uq1 - uq);
-    const R t3 = k.e3[o] * (uq1 - uq);
-    return uq + t2 + t3 - k.e4[o] * dmq1 + k.e5[o] * dmq;
+    // uq + t2 + t3 - e4 dmq1 + e5 dmq with t2 = e2 (uq1 - uq), t3 = e3 (uq1 - uq)
+    const R du = uq1 - uq;
+    return mad_(k.e5[o], dmq, mad_(-k.e4[o], dmq1, mad_(k.e3[o], du, mad_(k.e2[o], du, uq))));
   };
   const R um2 = u[0], um1 = u[1], u0 = u[2], up1 = u[3], up2 = u[4];
   const R dmm1 = dma_of(um2, um1, u0, 0);
@@ -255,16 +255,16 @@ RH_DEV void ppm_var(const R u[5], const PpmCoef<R>& k, int n, R f2, R eta, R& l_
   R r = eq51(u0, up1, dm0, dmp1, 0);          // value at p+1/2
   R l = eq51(um1, u0, dmm1, dm0, 1);          // value at p-1/2
   if (n == 0) {
-    r = r * (R(1) - eta) + (up1 - R(0.5) * dmp1) * eta;
-    l = l * (R(1) - eta) + (um1 + R(0.5) * dmm1) * eta;
+    r = mad_(r, R(1) - eta, mad_(-R(0.5), dmp1, up1) * eta);
+    l = mad_(l, R(1) - eta, mad_(R(0.5), dmm1, um1) * eta);
   }
-  r = u0 * f2 + r * (R(1) - f2);
-  l = u0 * f2 + l * (R(1) - f2);
+  r = mad_(u0, f2, r * (R(1) - f2));
+  l = mad_(u0, f2, l * (R(1) - f2));
   if ((r - u0) * (l - u0) >= R(0)) { r = u0; l = u0; }            // equation (73)
-  const R t3 = R(6.0) * (r - l) * (u0 - R(0.5) * (r + l));         // equations (74), (75)
+  const R t3 = R(6.0) * (r - l) * mad_(-R(0.5), r + l, u0);        // equations (74), (75)
   const R t4 = (r - l) * (r - l);
-  if (t3 > t4) l = R(3.0) * u0 - R(2.0) * r;
-  if (-t3 > t4) r = R(3.0) * u0 - R(2.0) * l;
+  if (t3 > t4) l = mad_(R(3.0), u0, -(R(2.0) * r));
+  if (-t3 > t4) r = mad_(R(3.0), u0, -(R(2.0) * l));
   l_out = l;
   r_out = r;
 }
@@ -403,7 +403,7 @@ RH_DEV void sweep_face(const Geo& G, R* __restrict__ F, R* __restrict__ CTsrc, l
     if (cmin > R(0)) o = double(Fl[n]);
     else if (cmin <= R(0) && cmax > R(0)) {
       const R c1 = fmaxf_(absr(cmax), absr(cmin));
-      o = double(R(0.5) * (Fl[n] + Fr[n] - c1 * (Ur[n] - Ul[n])));
+      o = double(R(0.5) * mad_(-c1, Ur[n] - Ul[n], Fl[n] + Fr[n]));
     } else if (cmax <= R(0)) o = double(Fr[n]);
     else o = Fo[(long long)n * N];  // unordered (NaN) speeds: hlle keeps ww, then flushes it
     if (absr(o) < 1.0e-40) o = 0.0;
