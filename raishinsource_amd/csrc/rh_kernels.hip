@@ -1662,69 +1662,107 @@ RH_DEV void b_advance_time(const AdvArgs& A) {
 // Resolution of cells whose initial W is inherited from the previous cell in
 // the reference's (k,j,i) loop order (recov.f90:815-817 leaves wg unchanged,
 // recov.f90:632 starts it at 0).  A flagged cell is computed once its
-// predecessor's final W is known: passes over the flagged list, each taking
-// the cells whose predecessor is resolved, until none is left.  Chains of
-// consecutive flagged cells take one pass per link.  One workgroup.
+// predecessor's final W is known.
+//
+// Chain heads (flagged cells whose predecessor is not flagged, or that open
+// the loop) are independent: every workgroup of the launch takes a share of
+// them.  The last workgroup to finish (a device-scope counter after a
+// device-scope fence) then resolves the remaining links in passes over the
+// flagged list, each pass taking the cells whose predecessor is resolved,
+// until none is left: a chain of consecutive flagged cells takes one pass
+// per link.  A batch runs one workgroup per member, which does both parts.
+
+// Predecessor of interior cell c in (k,j,i) order, -1 for the first cell.
+RH_DEV long long carry_pred(const Geo& G, long long c, int& i, int& k) {
+  const long long sk = kstride(G);
+  i = (int)(c % G.ni) + 1;
+  k = (int)(c / sk) + 1;
+  const int j = G.jfix ? G.jfix : (int)((c / G.ni) % G.nj) + 1;
+  int pi = i - 1, pj = j, pk = k;
+  if (pi < G.nm1 + 1) { pi = G.ni - G.nm1; pj = j - 1; if (pj < G.nm1 + 1) { pj = G.nj - G.nm1; pk = k - 1; } }
+  return pk >= G.nm1 + 1 ? cidx(pi, pj, pk, G) : -1;
+}
+
 template <int IW>
-RH_DEV void b_recov_carry(const Geo& G, const FixArgs& A) {
-  __shared__ int s_progress, s_left;
+RH_DEV void carry_cell(const Geo& G, const FixArgs& A, long long c, int i, int k, double wg_prev) {
+  const long long N = G.N;
+  const Met M{A.metC, G.P2, q2(i, k, G)};
+  double u[9], w[9];
+  for (int m = 0; m < 9; m++) { u[m] = A.Ustage[(long long)m * N + c]; w[m] = A.W[(long long)m * N + c]; }
+  RecovOut ro;
+  bool carry;
+  double wgout;
+  recov_any<IW>(u, w, M, A.gam, A.dmin, A.pmin, A.iter, A.r15[i - 1], A.r25[i - 1], wg_prev, carry, ro, wgout);
+  for (int m = 0; m < 9; m++) A.Wpre[(long long)m * N + c] = ro.w[m];
+  A.wgout[c] = wgout;
+  if (ro.fallback) atomicAdd(&A.cnt[C_FALLBACK], 1ULL);
+  if (ro.fail) atomicAdd(&A.cnt[C_FAIL], 1ULL);
+  if (ro.negrho) atomicAdd(&A.cnt[C_NEGRHO], 1ULL);
+  if (ro.negp) atomicAdd(&A.cnt[C_NEGP], 1ULL);
+  if (ro.vsqgt1) atomicAdd(&A.cnt[C_VSQGT1], (unsigned long long)ro.vsqgt1);
+  if (ro.det0) atomicAdd(&A.cnt[C_DET0], (unsigned long long)ro.det0);
+  if (ro.guard) atomicAdd(&A.cnt[C_GUARD], 1ULL);
+}
+
+// grp/ngrp: this workgroup's place among those sharing the chain heads
+// (ngrp == 1: the workgroup does everything and needs no counter).
+// Pass -1 takes the chain heads (pend 2 = resolved, 1 = waits for its
+// predecessor); passes 0, 1, ... (last workgroup) take the links.  One loop
+// body, so the recovery is compiled once.
+template <int IW>
+RH_DEV void b_recov_carry(const Geo& G, const FixArgs& A, int grp, int ngrp) {
+  __shared__ int s_progress, s_left, s_last;
   unsigned long long n = A.carry[0];
   if (n == 0) return;
-  if (n > (unsigned long long)A.carry_cap) {
-    if (threadIdx.x == 0) atomicAdd(&A.cnt[C_GUARD], 1ULL);
-    n = (unsigned long long)A.carry_cap;
-  }
+  if (n > (unsigned long long)A.carry_cap) n = (unsigned long long)A.carry_cap;
   const unsigned long long* list = A.carry + 1;
   unsigned char* pend = A.pend;
-  for (unsigned long long e = threadIdx.x; e < n; e += blockDim.x) pend[e] = 1;
-  __syncthreads();
-  const long long N = G.N;
-  const long long sk = kstride(G);
-  for (int pass = 0; pass < (1 << 22); pass++) {
-    if (threadIdx.x == 0) { s_progress = 0; s_left = 0; }
-    __syncthreads();
-    for (unsigned long long e = threadIdx.x; e < n; e += blockDim.x) {
-      if (pend[e] != 1) continue;
+  for (int pass = -1; pass < (1 << 22); pass++) {
+    const bool heads = pass < 0;
+    if (!heads) {
+      if (threadIdx.x == 0) { s_progress = 0; s_left = 0; }
+      __syncthreads();
+    }
+    const unsigned long long e0 = heads ? (unsigned long long)grp * blockDim.x + threadIdx.x : threadIdx.x;
+    const unsigned long long es = heads ? (unsigned long long)ngrp * blockDim.x : blockDim.x;
+    for (unsigned long long e = e0; e < n; e += es) {
+      if (!heads && pend[e] != 1) continue;
       const long long c = (long long)list[e];
-      const int i = (int)(c % G.ni) + 1, k = (int)(c / sk) + 1;
-      const int j = G.jfix ? G.jfix : (int)((c / G.ni) % G.nj) + 1;
-      int pi = i - 1, pj = j, pk = k;
-      if (pi < G.nm1 + 1) { pi = G.ni - G.nm1; pj = j - 1; if (pj < G.nm1 + 1) { pj = G.nj - G.nm1; pk = k - 1; } }
-      double wg_prev = 0.0;
-      if (pk >= G.nm1 + 1) {
-        const long long pc = cidx(pi, pj, pk, G);
-        if (A.cflag[pc]) { s_left = 1; continue; }  // predecessor not resolved yet
-        wg_prev = A.wgout[pc];
+      int i, k;
+      const long long pc = carry_pred(G, c, i, k);  // -1 only for the first cell, a head
+      if (pc >= 0 && A.cflag[pc]) {  // predecessor not resolved yet
+        if (heads) pend[e] = 1;
+        else s_left = 1;
+        continue;
       }
-      const Met M{A.metC, G.P2, q2(i, k, G)};
-      double u[9], w[9];
-      for (int m = 0; m < 9; m++) { u[m] = A.Ustage[(long long)m * N + c]; w[m] = A.W[(long long)m * N + c]; }
-      RecovOut ro;
-      bool carry;
-      double wgout;
-      recov_any<IW>(u, w, M, A.gam, A.dmin, A.pmin, A.iter, A.r15[i - 1], A.r25[i - 1], wg_prev, carry, ro, wgout);
-      for (int m = 0; m < 9; m++) A.Wpre[(long long)m * N + c] = ro.w[m];
-      A.wgout[c] = wgout;
-      if (ro.fallback) atomicAdd(&A.cnt[C_FALLBACK], 1ULL);
-      if (ro.fail) atomicAdd(&A.cnt[C_FAIL], 1ULL);
-      if (ro.negrho) atomicAdd(&A.cnt[C_NEGRHO], 1ULL);
-      if (ro.negp) atomicAdd(&A.cnt[C_NEGP], 1ULL);
-      if (ro.vsqgt1) atomicAdd(&A.cnt[C_VSQGT1], (unsigned long long)ro.vsqgt1);
-      if (ro.det0) atomicAdd(&A.cnt[C_DET0], (unsigned long long)ro.det0);
-      if (ro.guard) atomicAdd(&A.cnt[C_GUARD], 1ULL);
+      carry_cell<IW>(G, A, c, i, k, pc >= 0 ? A.wgout[pc] : 0.0);
       pend[e] = 2;
-      s_progress = 1;
+      if (!heads) s_progress = 1;
+    }
+    if (heads && ngrp > 1) {
+      unsigned long long* done = A.carry + 1 + A.carry_cap;
+      __threadfence();  // each thread's results visible on the device before the count
+      __syncthreads();
+      if (threadIdx.x == 0) s_last = atomicAdd(done, 1ULL) == (unsigned long long)(ngrp - 1);
+      __syncthreads();
+      if (!s_last) return;
+      __threadfence();  // every workgroup's results before the links are read
     }
     __syncthreads();
+    if (heads && threadIdx.x == 0 && A.carry[0] > (unsigned long long)A.carry_cap) atomicAdd(&A.cnt[C_GUARD], 1ULL);
     for (unsigned long long e = threadIdx.x; e < n; e += blockDim.x)
       if (pend[e] == 2) { pend[e] = 0; A.cflag[list[e]] = 0; }
     __syncthreads();
+    if (heads) continue;
     const bool stop = !s_left || !s_progress;  // no progress: nothing can resolve further
     __syncthreads();                            // all have read before the next reset
     if (stop) break;
   }
   __syncthreads();
-  if (threadIdx.x == 0) A.carry[0] = 0;
+  if (threadIdx.x == 0) {
+    A.carry[0] = 0;
+    if (ngrp > 1) A.carry[1 + A.carry_cap] = 0;
+  }
 }
 
 // Set-up tables of an axisymmetric metric (metric 303 or 1) on the device,
@@ -1885,7 +1923,9 @@ __global__ void __launch_bounds__(256) k_rk(Geo G, StageArgs A) { b_rk<STAGE, SY
 template <int IW>
 __global__ void __launch_bounds__(256) k_recov(Geo G, StageArgs A) { b_recov<IW>(G, A, xcd_block()); }
 template <int IW>
-__global__ void __launch_bounds__(1024) k_recov_carry(Geo G, FixArgs A) { b_recov_carry<IW>(G, A); }
+__global__ void __launch_bounds__(1024) k_recov_carry(Geo G, FixArgs A) {
+  b_recov_carry<IW>(G, A, blockIdx.x, gridDim.x);
+}
 __global__ void __launch_bounds__(256) k_bnd(Geo G, PostArgs A) { b_bnd(G, A, blockIdx.x); }
 __global__ void __launch_bounds__(256) k_post(Geo G, PostArgs A) { b_post(G, A, xcd_block()); }
 template <int ICHA>
@@ -1971,7 +2011,7 @@ __global__ void __launch_bounds__(1024) kb_recov_carry(const BatchItem* __restri
   if ((int)blockIdx.x >= n) return;
   const Geo G = it[blockIdx.x].G;
   const FixArgs A = it[blockIdx.x].A.fx;
-  b_recov_carry<IW>(G, A);
+  b_recov_carry<IW>(G, A, 0, 1);
 }
 __global__ void __launch_bounds__(1024) kb_cfl_finish(const BatchItem* __restrict__ it, int n) {
   if ((int)blockIdx.x >= n) return;
@@ -2213,8 +2253,10 @@ hipError_t launch_recov(const Geo& G, const StageArgs& A, hipStream_t st) {
 
 hipError_t launch_recov_carry(const Geo& G, const FixArgs& A, hipStream_t st) {
   if (g_rec) { AnyArgs a{}; a.fx = A; record(LK_CARRY, G.iwvec, 1024, 1, G, a); return hipSuccess; }
-  if (G.iwvec == 2) hipLaunchKernelGGL(k_recov_carry<2>, dim3(1), dim3(1024), 0, st, G, A);
-  else hipLaunchKernelGGL(k_recov_carry<1>, dim3(1), dim3(1024), 0, st, G, A);
+  // workgroups sharing the chain heads: one per 1024 flaggable cells, at most 64
+  const int ng = (int)std::min<long long>(64, std::max<long long>(1, (A.carry_cap + 1023) / 1024));
+  if (G.iwvec == 2) hipLaunchKernelGGL(k_recov_carry<2>, dim3(ng), dim3(1024), 0, st, G, A);
+  else hipLaunchKernelGGL(k_recov_carry<1>, dim3(ng), dim3(1024), 0, st, G, A);
   return hipGetLastError();
 }
 

@@ -254,7 +254,7 @@ static bool build_model_(raishin_model* m, const raishin_pram* p, const double* 
       !dalloc(&m->Uh, 9 * N) || !dalloc(&m->Us, 9 * N) || !dalloc(&m->Unext, 9 * N) || !dalloc(&m->F, 27 * N) ||
       !dalloc(&m->CT, 6 * N) || !dalloc(&m->wgout, N) || !dalloc(&m->chr, 64 * G.P2) || !dalloc(&m->r15, G.ni) ||
       !dalloc(&m->r25, G.ni) || !dalloc(&m->bc, 54) || !dalloc(&m->dtbits, 4) || !dalloc(&m->cnt, NCOUNT) ||
-      !dalloc(&m->carry, 1 + m->carry_cap) || !dalloc(&m->tdt, 2) || !dalloc(&m->cflag, N) ||
+      !dalloc(&m->carry, 2 + m->carry_cap) || !dalloc(&m->tdt, 2) || !dalloc(&m->cflag, N) ||
       !dalloc(&m->pend, m->carry_cap) || !dalloc(&m->cfl_last, 4 * cfl_blocks(G)) ||
       !dalloc(&m->cfl_min, 4 * cfl_blocks(G)))
     return false;
@@ -265,7 +265,7 @@ static bool build_model_(raishin_model* m, const raishin_pram* p, const double* 
   if (!dalloc(&m->cpart, (size_t)m->crows * NCOUNT)) return false;
   HIPCHK(hipMemsetAsync(m->cpart, 0, (size_t)m->crows * NCOUNT * 8, m->st));
   HIPCHK(hipMemsetAsync(m->cnt, 0, NCOUNT * 8, m->st));
-  HIPCHK(hipMemsetAsync(m->carry, 0, (1 + m->carry_cap) * 8, m->st));
+  HIPCHK(hipMemsetAsync(m->carry, 0, (2 + m->carry_cap) * 8, m->st));
   HIPCHK(hipMemsetAsync(m->tdt, 0, 16, m->st));
   HIPCHK(hipMemsetAsync(m->W, 0, 9 * N * 8, m->st));
   // the reference zeroes its flux array once (main.f90:372-381); an interface

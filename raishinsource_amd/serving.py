@@ -84,7 +84,7 @@ def footprint(p):
     ncfl = max((max(ni - 2, 0) * max(nj - 2, 0) * max(nk - 2, 0) + 255) // 256, 1)
     planes = 7 * 9 + 27 + 6 + 1 + 9 + (9 if need_wo else 0) + (18 if p.irec in (1, 5) else 0)
     b = 8 * n * planes
-    b += n + 8 * (1 + cap) + cap                     # cflag, carry list, pend
+    b += n + 8 * (2 + cap) + cap                     # cflag, carry count + list + counter, pend
     b += 8 * 64 * p2 + 2 * 8 * ni + 4 * 54 + 8 * 4 + 8 * 11 + 8 * 2
     b += 2 * 8 * 4 * ncfl                            # CFL per-block results
     b += 8 * 11 * ((n + 255) // 256) * 4             # per-wavefront counter rows
