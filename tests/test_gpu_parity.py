@@ -362,6 +362,45 @@ def test_tube_failing_inversions_match_oracle(oracle_lib):
     assert_counters(cg, co)
 
 
+def test_carried_guesses_match_oracle(oracle_lib):
+    """Cells whose recov1a quadratic has no admissible root keep the previous
+    cell's guess (recov.f90:815-817), so each depends on its predecessor in
+    (k,j,i) order.  A 64 x 14 x 64 interior PPM/RK3 torus whose conserved
+    field is scaled by 1000 in a band of 16 theta planes gives thousands of
+    such cells per step in hundreds of chains (consecutive cells), all
+    finite, spread over the 56 workgroups that share the chain heads."""
+    p = replace(configs.ALL["t3dppm"], name="carry", imax=74, jmax=24, kmax=74).resolved()
+    g = native.setup_grid(p)
+    m = native.setup_metric(p, g)
+    uri = native.setup_torus(p, g, m, seed=0, noise=0.02)
+    o = Oracle(p)
+    o.reset_counters()
+    uu = fzeros(9, p.imax, p.jmax, p.kmax)
+    o.call("caluu1", uri, uu, m["gcov"], m["gcon"], m["detg"])
+    band = (slice(p.nm1 + 5, p.imax - p.nm1 - 5), slice(None), slice(p.kmax // 2 - 8, p.kmax // 2 + 8))
+    for b in (6, 7, 8):
+        uu[b][band] *= 1000.0
+    # the oracle's step, recording the carried cells in loop order
+    rec = np.zeros(1 << 20, dtype=np.int64)
+    o.lib.orc_debug_carry(rec.ctypes.data_as(C.c_void_p), C.c_longlong(len(rec)))
+    wo, uo = uri.copy(order="F"), uu.copy(order="F")
+    dto, _ = o.step(uo, wo, g, m, 0.0)
+    n = o.lib.orc_debug_carry_count()
+    o.lib.orc_debug_carry(None, C.c_longlong(0))
+    co = o.counters()
+    o.lib.orc_free_work()
+    chains = np.split(rec[:n], np.flatnonzero(np.diff(rec[:n]) != 1) + 1)
+    lens = np.array([len(c) for c in chains])
+    print("carried cells", n, "chains", len(lens), "longest", lens.max(), "counters", co)
+    assert n >= 3000 and len(lens) >= 300 and lens.max() >= 8 and co["guard"] == 0
+    wg, ug, tg, dtg, cg = gpu_run(p, g, m, uri, 1, uu=uu)
+    assert finite_fraction(p, wo) == 1.0
+    assert_close("carry dt", dtg[None], np.array([dto])[None])
+    assert_close("carry uri", wg, wo)
+    assert_close("carry uu", ug, uo)
+    assert_counters(cg, co)
+
+
 EDGE = [
     # ragged extents that are multiples of nothing the kernels tile by
     ("ragged_mp5", replace(configs.G_MP5, imax=17, jmax=10, kmax=13), 3),
