@@ -9,6 +9,7 @@
 #include "../../include/raishin_hip.h"
 #include "../../include/raishin_setup.h"
 #include "rh_metric.hpp"
+#include "rh_torus.hpp"
 
 namespace {
 
@@ -37,11 +38,6 @@ rhmet::ThetaTrig theta_trig(double th) {
   ks_trig(th - delta, t.sb, t.cb);
   return t;
 }
-void kshcon1(double akm, double* g, double rr, double x3) {
-  double s, c;
-  ks_trig(x3, s, c);
-  rhmet::ks_con(akm, g, rr, s, c);
-}
 void metric_cov(const raishin_pram& p, double* g, double r, double th) {
   double s = 0.0, c = 0.0;
   if (p.metric == 303) ks_trig(th, s, c);
@@ -51,61 +47,6 @@ void metric_con(const raishin_pram& p, double* g, double r, double th) {
   double s = 0.0, c = 0.0;
   if (p.metric == 303) ks_trig(th, s, c);
   rhmet::con(p.metric, p.akm, g, r, s, c);
-}
-
-// kercov1: coord.f90:369-450 (Boyer-Lindquist, used by the torus set-up)
-void kercov1(double akm, double* g, double rr, double x3) {
-  const double small = 1.0e-12;
-  const double theta = (x3 > PI) ? 2.0 * PI - x3 : x3;
-  double sth = std::sin(theta);
-  if (std::fabs(sth) < small) sth = (sth > 0.0) ? small : -small;
-  const double s2 = sth * sth;
-  double cth = std::cos(theta);
-  if (std::fabs(cth) < small) cth = (cth > 0.0) ? small : -small;
-  const double a2 = akm * akm, r2 = rr * rr;
-  const double del = 1.0 - (2.0 / rr) + (a2 * (1.0 / r2));
-  const double sig = 1.0 + a2 * cth * cth * (1.0 / r2);
-  for (int e = 0; e < 16; e++) g[e] = 0.0;
-  GM(g, 0, 0) = -(1.0 - 2.0 * (1.0 / (rr * sig)));
-  GM(g, 0, 2) = -2.0 * akm * s2 * (1.0 / (rr * sig));
-  GM(g, 1, 1) = sig / del;
-  GM(g, 2, 0) = GM(g, 0, 2);
-  GM(g, 2, 2) = r2 * s2 * (1.0 + (a2 * (1.0 / r2)) + 2.0 * a2 * s2 * (1.0 / (r2 * rr * sig)));
-  GM(g, 3, 3) = r2 * sig;
-}
-
-// transbl2ks: physfunc.f90:366-505 (metric 303 branch): BL u~ -> KS u~
-void transbl2ks(double akm, const double ut[4], double out[4], double rr, double th) {
-  const double rbh1 = 1.0 + std::sqrt(1.0 - akm * akm);
-  double gi[16];
-  kshcon1(akm, gi, rr, th);
-  const double alpha1 = 1.0 / std::sqrt(-GM(gi, 0, 0));
-  double beta1[4] = {0.0, alpha1 * alpha1 * GM(gi, 0, 1), alpha1 * alpha1 * GM(gi, 0, 2), alpha1 * alpha1 * GM(gi, 0, 3)};
-  if (rr > rbh1) {
-    double g[16];
-    kercov1(akm, g, rr, th);
-    double ucon[4] = {0.0, ut[1], ut[2], ut[3]};
-    const double aa = GM(g, 0, 0);
-    const double bb = 2.0 * (GM(g, 0, 1) * ucon[1] + GM(g, 0, 2) * ucon[2] + GM(g, 0, 3) * ucon[3]);
-    const double cc = 1.0 + GM(g, 1, 1) * ucon[1] * ucon[1] + GM(g, 2, 2) * ucon[2] * ucon[2] +
-                      GM(g, 3, 3) * ucon[3] * ucon[3] +
-                      2.0 * (GM(g, 1, 2) * ucon[1] * ucon[2] + GM(g, 1, 3) * ucon[1] * ucon[3] + GM(g, 2, 3) * ucon[2] * ucon[3]);
-    const double discr = bb * bb - 4.0 * aa * cc;
-    ucon[0] = (-bb - std::sqrt(discr)) * (1.0 / (2.0 * aa));
-    double tr[16] = {0.0};
-    const double tmp2 = rr * rr - (2.0 * rr) + akm * akm;
-    if (tmp2 > 0.0) {
-      GM(tr, 0, 0) = 1.0; GM(tr, 0, 1) = 2.0 * rr * (1.0 / tmp2); GM(tr, 1, 1) = 1.0;
-      GM(tr, 2, 2) = 1.0; GM(tr, 2, 1) = akm * (1.0 / tmp2); GM(tr, 3, 3) = 1.0;
-    }
-    double u1[4] = {0.0, 0.0, 0.0, 0.0};
-    for (int n = 0; n < 4; n++)
-      for (int m = 0; m < 4; m++) u1[m] = u1[m] + GM(tr, m, n) * ucon[n];
-    const double gfl = u1[0] * alpha1;
-    for (int m = 1; m <= 3; m++) out[m] = u1[m] + gfl * beta1[m] * (1.0 / alpha1);
-  } else {
-    for (int m = 1; m <= 3; m++) out[m] = beta1[m] * (1.0 / alpha1);
-  }
 }
 
 // callfish: mdgrmhd.f90:808-834 (angular momentum of the FM torus)
@@ -119,27 +60,6 @@ double callfish(double akm, double rmax) {
   const double t2a = (rmax * rmax * rmax) * std::sqrt(2.0 * akm * std::sqrt(rmax) + (-3.0 + rmax) * rmax);
   const double t2b = ((akm * akm) + (-2.0 + rmax) * rmax);
   return (t1a * ((t1b / t1c) + t1d * t1e / t1f)) / (t2a * t2b);
-}
-
-// b^2 with the reference's helpers (physfunc.f90) for the beta normalisation
-double bsq_of(const double* w, const double* g, const double* gi) {
-  double ut[4] = {0.0, w[1], w[2], w[3]};
-  double utsq = 0.0;
-  for (int m = 1; m <= 3; m++)
-    for (int n = 1; n <= 3; n++) utsq = utsq + GM(g, m, n) * ut[m] * ut[n];
-  if (utsq < 0.0) utsq = (std::fabs(utsq) <= 1.0e-10) ? std::fabs(utsq) : 1.0e-10;
-  if (std::fabs(utsq) > 5.0e2) utsq = 5.0e2;
-  const double gfl = std::sqrt(1.0 + utsq);
-  const double alpha1 = 1.0 / std::sqrt(-GM(gi, 0, 0));
-  double ucon[4], ucov[4], bb[4], bbcov[4];
-  ucon[0] = gfl * (1.0 / alpha1);
-  for (int m = 1; m <= 3; m++) ucon[m] = ut[m] - gfl * (alpha1 * alpha1 * GM(gi, 0, m)) * (1.0 / alpha1);
-  for (int m = 0; m < 4; m++) ucov[m] = GM(g, m, 0) * ucon[0] + GM(g, m, 1) * ucon[1] + GM(g, m, 2) * ucon[2] + GM(g, m, 3) * ucon[3];
-  const double b[4] = {0.0, w[6], w[7], w[8]};
-  bb[0] = b[1] * ucov[1] + b[2] * ucov[2] + b[3] * ucov[3];
-  for (int m = 1; m <= 3; m++) bb[m] = (b[m] + bb[0] * ucon[m]) * (1.0 / ucon[0]);
-  for (int m = 0; m < 4; m++) bbcov[m] = GM(g, m, 0) * bb[0] + GM(g, m, 1) * bb[1] + GM(g, m, 2) * bb[2] + GM(g, m, 3) * bb[3];
-  return bb[0] * bbcov[0] + bb[1] * bbcov[1] + bb[2] * bbcov[2] + bb[3] * bbcov[3];
 }
 
 inline double splitmix_uniform(uint64_t& s) {
@@ -192,6 +112,7 @@ static int mdtorus1_impl(const raishin_pram* pp, const raishin_torus* tp, double
   const double t2d = 0.5 * std::log((1.0 + std::sqrt(1.0 + t2a)) * (1.0 / t2c));
   const double t2e = 0.5 * std::sqrt(1.0 + t2a);
   const double t2f = 2.0 * akm * rin * clm * (1.0 / aain);
+  const rhtorus::Consts tc{gam, akm, kpol1, rin, clm, t2d, t2e, t2f, thedge, thedge1, small};
   double rhomax = 0.0;
 #pragma omp parallel for schedule(dynamic) reduction(max : rhomax)
   for (int k = 1; k <= nk; k++)
@@ -200,36 +121,15 @@ static int mdtorus1_impl(const raishin_pram* pp, const raishin_torus* tp, double
         const double rr = x1[i - 1], th = x3[k - 1];
         double sth = std::sin(th);
         if (std::fabs(sth) < small) sth = (sth >= 0.0) ? small : -small;
-        const double cth = std::cos(th);
-        const double del = (rr * rr) - (2.0 * rr) + (akm * akm);
-        const double aa = (rr * rr + akm * akm) * (rr * rr + akm * akm) - del * akm * akm * sth * sth;
-        const double sig = rr * rr + akm * akm * cth * cth;
-        double al = 0.0;
-        if (rr >= rin && th >= thedge && th <= thedge1) {
-          const double t1a = 4.0 * (clm * clm * sig * sig) * del * (1.0 / (aa * sth * aa * sth));
-          const double t1c = sig * del * (1.0 / aa);
-          const double t1d = 0.5 * std::log((1.0 + std::sqrt(1.0 + t1a)) * (1.0 / t1c));
-          const double t1e = 0.5 * std::sqrt(1.0 + t1a);
-          const double t1f = 2.0 * akm * rr * clm * (1.0 / aa);
-          al = t1d - t1e - t1f - (t2d - t2e - t2f);
-          if (al <= 0.0) al = 0.0;
-        }
-        alnh[cell(p, i, j, k)] = al;
-        const double hm1 = std::exp(al) - 1.0;
-        if (hm1 > 0.0 && rr >= rin && th >= thedge && th <= thedge1) {
-          const double de = std::pow(hm1 * (gam - 1.0) / (kpol1 * gam), 1.0 / (gam - 1.0));
-          const double pr = kpol1 * std::pow(de, gam);
-          const double expm2chi = sig * sig * del * (1.0 / (aa * aa * sth * sth));
-          const double up1 = std::sqrt((-1.0 + std::sqrt(1.0 + 4.0 * clm * clm * expm2chi)) / 2.0);
-          const double t3a = 2.0 * akm * rr * std::sqrt(1.0 + up1 * up1);
-          const double t3b = std::sqrt(aa * sig * del);
-          const double t3c = std::sqrt(sig * (1.0 / aa)) * up1 * (1.0 / sth);
-          double ut[4] = {0.0, 0.0, t3a * (1.0 / t3b) + t3c, 0.0}, un[4];
-          transbl2ks(akm, ut, un, rr, th);
+        double s_ks, c_ks;
+        ks_trig(th, s_ks, c_ks);
+        const rhtorus::Column col = rhtorus::column(tc, rr, th, sth, std::cos(th), s_ks, c_ks);
+        alnh[cell(p, i, j, k)] = col.al;
+        if (col.in) {
           double* w = &uri[9 * cell(p, i, j, k)];
-          w[0] = de; w[1] = un[1]; w[2] = un[2]; w[3] = un[3]; w[4] = pr;
-          w[5] = pr * (1.0 / std::pow(de, gam - 1.0));
-          if (de > rhomax) rhomax = de;
+          w[0] = col.de; w[1] = col.u[1]; w[2] = col.u[2]; w[3] = col.u[3]; w[4] = col.pr;
+          w[5] = col.pr * (1.0 / std::pow(col.de, gam - 1.0));
+          if (col.de > rhomax) rhomax = col.de;
         }
       }
   if (rhomax <= 0.0) return -1;
@@ -305,7 +205,7 @@ static int mdtorus1_impl(const raishin_pram* pp, const raishin_torus* tp, double
     for (int j = 1; j <= nj; j++)
       for (int i = 1; i <= ni; i++) {
         const size_t c = cell(p, i, j, k), q = mq(i, j, k);
-        const double b2 = bsq_of(&uri[9 * c], &gcov[16 * q], &gcon[16 * q]);
+        const double b2 = rhtorus::bsq(&uri[9 * c], &gcov[16 * q], &gcon[16 * q]);
         const double pb = (b2 != 0.0) ? 2.0 * uri[9 * c + 4] / b2 : 1.0e5;
         if (pb < bmin) bmin = pb;
       }
