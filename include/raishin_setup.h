@@ -52,6 +52,17 @@ int raishin_mdtorus1(const raishin_pram *p, const raishin_torus *t, double *uri,
 int raishin_mdtorus1_ik(const raishin_pram *p, const raishin_torus *t, double *uri, const double *gcov,
                         const double *gcon, const double *detg, const double *x1, const double *x2,
                         const double *x3);
+/* The same torus built on the device, straight into the model's state
+ * (primitives, conserved variables by caluu1, time 0, fluxes zeroed), from
+ * the model's own device metric tables: a set-up with no host arrays, for a
+ * model from raishin_model_create_grid (or any model of metric 303).  It
+ * follows raishin_mdtorus1 step for step, including its noise stream; the
+ * values differ from the host torus only where the device's log, exp and pow
+ * round differently from glibc's (about 1e-15 relative).  Reference:
+ * mdgrmhd.f90:879-1654 (mdtorus1), as raishin_mdtorus1; no counterpart of
+ * its own in the reference, whose set-up is host Fortran.  Returns 0, or -1
+ * with the error set (metric not 303, no torus cells on the grid). */
+int raishin_model_set_torus(raishin_model *m, const raishin_torus *t);
 
 /* Riemann-tube states (left [0], right [1]): density, pressure, 3-velocity,
  * magnetic field. */

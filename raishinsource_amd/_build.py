@@ -38,7 +38,7 @@ FORTRAN = ["raishin_c.f90", "raishin_wrappers.f90", "raishin_main.f90"]
 FORTRAN_MAIN = os.path.join(HERE, "fortran", "raishin_main")
 
 UNITS = [("rh_kernels.hip", "hip"), ("rh_model.cpp", "hip"), ("rh_capi.cpp", "hip"), ("rh_ref.cpp", "hip"),
-         ("rh_batch.cpp", "hip"), ("rh_noalloc.hip", "hip"),
+         ("rh_batch.cpp", "hip"), ("rh_noalloc.hip", "hip"), ("rh_torus.hip", "hip"),
          ("rh_setup.cpp", "host"), ("rh_io.cpp", "host")]
 
 

@@ -72,6 +72,27 @@ inline double splitmix_uniform(uint64_t& s) {
 
 }  // namespace
 
+namespace rhm {
+// The torus constants both set-ups use (evaluated on the host, glibc): the
+// Fishbone-Moncrief l at r_max (callfish) and the inner-edge terms of ln h.
+rhtorus::Consts torus_consts(const raishin_pram& p, const raishin_torus& t) {
+  const double gam = p.gam, akm = p.akm, kpol1 = 1.0e-3;
+  const double rin = t.rin, rmax = t.rmax;
+  const double thedge = (double)0.02f * PI, thedge1 = (double)0.98f * PI, small = 1.0e-12;
+  const double clm = callfish(akm, rmax);
+  const double delin = rin * rin - 2.0 * rin + akm * akm;
+  const double aain = (rin * rin + akm * akm) * (rin * rin + akm * akm) - delin * akm * akm;
+  const double sigin = rin * rin;
+  const double t2a = 4.0 * (clm * clm * sigin * sigin) * delin * (1.0 / (aain * aain));
+  const double t2c = sigin * delin * (1.0 / aain);
+  const double t2d = 0.5 * std::log((1.0 + std::sqrt(1.0 + t2a)) * (1.0 / t2c));
+  const double t2e = 0.5 * std::sqrt(1.0 + t2a);
+  const double t2f = 2.0 * akm * rin * clm * (1.0 / aain);
+  return rhtorus::Consts{gam, akm, kpol1, rin, clm, t2d, t2e, t2f, thedge, thedge1, small};
+}
+}  // namespace rhm
+using rhm::torus_consts;
+
 static int mdtorus1_impl(const raishin_pram* pp, const raishin_torus* tp, double* uri, const double* gcov,
                          const double* gcon, const double* detg, const double* x1, const double* x2,
                          const double* x3, bool slice) {
@@ -86,9 +107,7 @@ static int mdtorus1_impl(const raishin_pram* pp, const raishin_torus* tp, double
   const raishin_pram& p = *pp;
   const raishin_torus& t = *tp;
   (void)x2;
-  const double gam = p.gam, akm = p.akm, kpol1 = 1.0e-3;
-  const double rin = t.rin, rmax = t.rmax;
-  const double thedge = (double)0.02f * PI, thedge1 = (double)0.98f * PI, small = 1.0e-12;
+  const double gam = p.gam;
   const int ni = p.imax, nj = p.jmax, nk = p.kmax;
   const size_t N = (size_t)ni * nj * nk;
   std::vector<double> alnh(N, 0.0);
@@ -103,16 +122,7 @@ static int mdtorus1_impl(const raishin_pram* pp, const raishin_torus* tp, double
         w[6] = w[7] = w[8] = 0.0;
       }
   if (p.metric != 303) return 0;
-  const double clm = callfish(akm, rmax);
-  const double delin = rin * rin - 2.0 * rin + akm * akm;
-  const double aain = (rin * rin + akm * akm) * (rin * rin + akm * akm) - delin * akm * akm;
-  const double sigin = rin * rin;
-  const double t2a = 4.0 * (clm * clm * sigin * sigin) * delin * (1.0 / (aain * aain));
-  const double t2c = sigin * delin * (1.0 / aain);
-  const double t2d = 0.5 * std::log((1.0 + std::sqrt(1.0 + t2a)) * (1.0 / t2c));
-  const double t2e = 0.5 * std::sqrt(1.0 + t2a);
-  const double t2f = 2.0 * akm * rin * clm * (1.0 / aain);
-  const rhtorus::Consts tc{gam, akm, kpol1, rin, clm, t2d, t2e, t2f, thedge, thedge1, small};
+  const rhtorus::Consts tc = torus_consts(p, t);
   double rhomax = 0.0;
 #pragma omp parallel for schedule(dynamic) reduction(max : rhomax)
   for (int k = 1; k <= nk; k++)
@@ -120,7 +130,7 @@ static int mdtorus1_impl(const raishin_pram* pp, const raishin_torus* tp, double
       for (int i = 1; i <= ni; i++) {
         const double rr = x1[i - 1], th = x3[k - 1];
         double sth = std::sin(th);
-        if (std::fabs(sth) < small) sth = (sth >= 0.0) ? small : -small;
+        if (std::fabs(sth) < tc.small) sth = (sth >= 0.0) ? tc.small : -tc.small;
         double s_ks, c_ks;
         ks_trig(th, s_ks, c_ks);
         const rhtorus::Column col = rhtorus::column(tc, rr, th, sth, std::cos(th), s_ks, c_ks);
@@ -146,7 +156,7 @@ static int mdtorus1_impl(const raishin_pram* pp, const raishin_torus* tp, double
         const double rr = x1[i - 1], th = x3[k - 1];
         const double rhomin1 = p.dmin * std::pow(rr, -1.5), prmin1 = p.pmin * (gam - 1.0) * std::pow(rr, -2.5);
         const double hm1 = std::exp(alnh[cell(p, i, j, k)]) - 1.0;
-        if (!(hm1 > 0.0 && rr >= rin && th >= thedge && th <= thedge1)) continue;
+        if (!(hm1 > 0.0 && rr >= tc.rin && th >= tc.thedge && th <= tc.thedge1)) continue;
         double* w = &uri[9 * cell(p, i, j, k)];
         double de = w[0] * (1.0 / rhomax), pr = w[4] * (1.0 / rhomax);
         double u;

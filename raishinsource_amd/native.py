@@ -62,7 +62,7 @@ EXPORTS = (
     "raishin_batch_sync", "raishin_mdshoctub", "raishin_model_set_block_sizes", "raishin_model_get_block_sizes",
     "raishin_model_autotune", "raishin_model_allow_collapse", "raishin_model_stored_rows",
     "raishin_store1", "raishin_restar2", "raishin_output1", "raishin_model_store1", "raishin_model_restar2",
-    "raishin_model_output1", "raishin_model_get_pram", "raishin_model_create_ik", "raishin_mdtorus1_ik",
+    "raishin_model_output1", "raishin_model_get_pram", "raishin_model_create_ik", "raishin_mdtorus1_ik", "raishin_model_set_torus",
     "raishin_model_create_grid", "raishin_model_tables", "raishin_model_set_timing_mask",
     "raishin_model_cdtcfl4_noalloc", "raishin_model_set_sweep_tiles", "raishin_model_get_sweep_tiles",
     "raishin_model_set_divb_form",
@@ -125,6 +125,8 @@ def lib():
         L.raishin_mdtorus1.restype = C.c_int
         L.raishin_mdtorus1_ik.argtypes = [C.POINTER(PramS), C.POINTER(TorusS)] + [vp] * 7
         L.raishin_mdtorus1_ik.restype = C.c_int
+        L.raishin_model_set_torus.argtypes = [vp, C.POINTER(TorusS)]
+        L.raishin_model_set_torus.restype = C.c_int
         L.raishin_mdshoctub.argtypes = [C.POINTER(PramS), C.POINTER(TubeS), vp]
         L.raishin_mdshoctub.restype = C.c_int
         L.raishin_model_set_timing.argtypes = [vp, C.c_int]
@@ -429,6 +431,14 @@ class Model:
             check()
             raise RaishinError("raishin_model_create_grid failed")
         return self
+
+    def set_torus(self, rin=6.0, rmax=12.0, beta=100.0, acut=0.2, noise=0.02, seed=0):
+        """The magnetised torus built on the device into this model's state
+        (raishin_model_set_torus): setup_torus's torus, time 0."""
+        t = TorusS(rin, rmax, beta, acut, noise, seed)
+        if lib().raishin_model_set_torus(self.h, C.byref(t)) != 0:
+            check()
+            raise RaishinError("raishin_model_set_torus failed")
 
     def tables(self, which):
         """(i,k) table `which` (0..3 metric at centres, i-, j-, k-faces: 33
