@@ -244,7 +244,8 @@ extern "C" int raishin_model_set_torus(raishin_model* m, const raishin_torus* t)
     const long long N = G.N, P2 = (long long)ni * nk;
     // host: the constants and per-theta sin / cos (glibc, as the host set-up)
     std::vector<double> x3(nk);
-    HIPCHK(hipMemcpy(x3.data(), m->xs[2], nk * sizeof(double), hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpyAsync(x3.data(), m->xs[2], nk * sizeof(double), hipMemcpyDeviceToHost, m->st));
+    HIPCHK(hipStreamSynchronize(m->st));
     const rhtorus::Consts c = torus_consts(m->p, *t);
     std::vector<double> trig(4 * (size_t)nk), tt(8 * (size_t)nk);
     theta_trig_table(m->p, x3.data(), x3.data(), tt.data());  // entries 0, 1: the folded, clamped pair

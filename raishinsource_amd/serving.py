@@ -153,16 +153,17 @@ def make_requests(n, rate, seed=0, mix=(0.6, 0.3, 0.1), mixed_fraction=0.5, toru
 
 
 def build(req):
-    """Host set-up, device model and initial state of one request."""
+    """Device model and initial state of one request: metric tables built on
+    the device, a torus too (raishin_model_set_torus: no host arrays), a tube
+    on the host."""
     p = req.p.resolved()
     g = native.setup_grid(p)
     mod = native.Model.from_grid(p, g)  # metric tables built on the device
     if req.kind == "tube":
-        uri = native.setup_tube(p, req.init["left"], req.init["right"])
+        mod.set_state(native.setup_tube(p, req.init["left"], req.init["right"]))
     else:
-        uri = native.setup_torus(p, g, mod.metric_row(), rin=req.init["rin"], rmax=req.init["rmax"],
-                                 beta=req.init["beta"], noise=0.02, seed=req.init["seed"])
-    mod.set_state(uri)
+        mod.set_torus(rin=req.init["rin"], rmax=req.init["rmax"], beta=req.init["beta"], noise=0.02,
+                      seed=req.init["seed"])
     mod.set_precision(req.mixed)
     mod.sync()
     return mod
