@@ -3,7 +3,9 @@
 metric tables on the device.  Without noise the torus is axisymmetric, and
 the step keeps it so bit for bit: every phi row of the state after two steps
 equals the first (a check that holds at any size and catches any index that
-overflows or wraps)."""
+overflows or wraps).  The same torus built on the device
+(raishin_model_set_torus) matches the host one within 1e-12 of each field's
+maximum at this size too."""
 from dataclasses import replace
 
 import numpy as np
@@ -20,6 +22,14 @@ def test_largest_model_stays_axisymmetric():
     mod = native.Model.from_grid(p, g)
     assert mod.device_bytes() > 30 * 2**30
     uri = native.setup_torus(p, g, mod.metric_row(), seed=0, noise=0.0)
+    mod.set_torus(seed=0, noise=0.0)
+    w_d, _, t_d = mod.get_state()
+    assert t_d == 0.0
+    for v in range(9):
+        scale = max(float(np.abs(uri[v]).max()), 1e-300)
+        err = float(np.abs(w_d[v] - uri[v]).max()) / scale
+        assert err <= 1e-12, (v, err)
+    del w_d
     mod.set_state(uri)
     assert mod.stored_rows() == p.jmax  # 246 interior phi cells: every row is kept
     t, dt = mod.step(2)
